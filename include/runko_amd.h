@@ -1,0 +1,183 @@
+/* runko_amd -- MI355X-native (gfx950) PIC time-step hot path behind runko's
+ * pic/emf tile-method API.
+ *
+ * C ABI only: plain pointers, sizes and status codes; no C++ or torch types.
+ * Device buffers (fields, particle pools, migration records) are allocated by
+ * the caller (the Python layer allocates them as torch tensors) and borrowed by
+ * the library through the rk_world_set_* calls; the library owns only small
+ * scratch (counters, reduction partials, sort staging).
+ *
+ * Every entry point returns 0 on success or a nonzero RK_E* code; the message
+ * of the last failure on the calling thread is returned by rk_last_error().
+ * Calls enqueue work on the given HIP stream (hipStream_t passed as void*) and
+ * return without waiting, except where a comment says "synchronous".
+ *
+ * Tile arguments: tile >= 0 names one tile (layout_right index over n_tiles,
+ * as the reference's tile tag); tile == -1 runs one launch over every tile
+ * (the batched form).
+ *
+ * Reference interfaces replaced (hel-astro-lab/runko v5.0.0):
+ *   pic::Tile<3> methods   src/runko/bindings/pypic.c++:105-136
+ *   emf::Tile<3> methods   src/runko/bindings/pyemf.c++:225-260
+ *   comm_mode enum         src/runko/communication_common.h:30-38
+ *   corgi local_communication (pycorgi.Grid.local_communication), called from
+ *                          runko/simulation.py:292-293
+ */
+#ifndef RUNKO_AMD_H
+#define RUNKO_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define RK_MAX_SPECIES 4
+
+enum rk_status {
+  RK_OK = 0,
+  RK_EINVAL = 1,     /* invalid argument / precondition (Python: RuntimeError) */
+  RK_EHIP = 2,       /* HIP runtime error */
+  RK_EOVERFLOW = 3,  /* a particle bin or the migration buffer is full: repack */
+  RK_ESTATE = 4      /* call sequence not allowed (e.g. filter twice in a pass) */
+};
+
+/* comm_mode values, src/runko/communication_common.h:30-38 */
+enum rk_comm_mode {
+  RK_COMM_EMF_J = 0,
+  RK_COMM_EMF_E = 1,
+  RK_COMM_EMF_B = 2,
+  RK_COMM_PIC_PARTICLE = 3,
+  RK_COMM_PIC_PARTICLE_EXTRA = 4,
+  RK_COMM_NUMBER_OF_PARTICLES = 5,
+  RK_COMM_EMF_J_EXCHANGE = 6
+};
+
+typedef struct rk_world rk_world;
+
+/* Geometry and physics constants of one world.  A world is either the whole
+ * tile grid (periodic = 1) or one stand-alone tile (periodic = 0, n_tiles =
+ * {1,1,1}, tile_offset = its grid index).  Fields live on one haloed lattice of
+ * (N+6)^3 nodes per component, N = n_tiles * n_cells (the reference's per-tile
+ * YeeLattice, emf/yee_lattice.h:67-108, scaled to the whole domain). */
+typedef struct rk_world_desc {
+  int32_t n_cells[3];      /* cells per tile                      (config n_cells_per_tile) */
+  int32_t n_tiles[3];      /* tiles in this world                  (config n_tiles or 1,1,1) */
+  int32_t tile_offset[3];  /* grid index of this world's tile (0,0,0) */
+  int32_t grid_tiles[3];   /* tiles of the full grid (tile tags and periodic wrap box) */
+  int32_t brick[3];        /* particle bin shape in cells; divides n_cells */
+  int32_t periodic;        /* 1: comm_local allowed (periodic ghost sync) */
+  int32_t n_species;
+  int32_t current_filter;  /* 0 binomial2 (27-tap), 1 binomial2_unrolled */
+  double cfl;
+  double charge[RK_MAX_SPECIES];
+  double mass[RK_MAX_SPECIES];
+} rk_world_desc;
+
+const char* rk_last_error(void);
+int rk_version(void);
+
+/* Creates a world (synchronous).  Allocates library scratch only. */
+int rk_world_create(rk_world** out, const rk_world_desc* desc);
+int rk_world_destroy(rk_world* w);
+
+/* Number of haloed lattice nodes per component, and particle bins (species x bricks). */
+int64_t rk_world_field_nodes(const rk_world* w);
+int32_t rk_world_num_bricks(const rk_world* w);
+
+/* Borrow the field arrays: E, B, J, Jf (filter target), each 3 device
+ * pointers of rk_world_field_nodes() floats, layout [x][y][z], z fastest. */
+int rk_world_set_fields(rk_world* w, float* const E[3], float* const B[3], float* const J[3],
+                        float* const Jf[3]);
+
+/* Device pointer of component comp of the buffer that currently holds J
+ * (filter passes write the other buffer and swap the two). */
+const void* rk_world_j_pointer(const rk_world* w, int32_t comp);
+
+/* Borrow the particle pool of one species.  pos/vel: 3 device pointers each of
+ * capacity floats; id: capacity uint64.  bin_start (int64), bin_cap and
+ * bin_count (uint32) have rk_world_num_bricks() entries (device memory). */
+int rk_world_set_species(rk_world* w, int32_t s, int64_t capacity, float* const pos[3],
+                         float* const vel[3], uint64_t* id, int64_t* bin_start, uint32_t* bin_cap,
+                         uint32_t* bin_count);
+
+/* Borrow the migration record buffer (capacity records): pos[3], vel[3] float,
+ * id uint64, meta uint32 (species | kind | direction | source bin).  Two
+ * record sets are used: set 0 holds tile leavers (consumed by comm_local
+ * pic_particle), set 1 holds in-tile brick changes (consumed at once). */
+int rk_world_set_migration(rk_world* w, int32_t set, int64_t capacity, float* const pos[3],
+                           float* const vel[3], uint64_t* id, uint32_t* meta);
+
+/* ---- emf::Tile methods (src/runko/emf/tile.c++:358-426) ---- */
+int rk_push_half_b(rk_world* w, int32_t tile, void* stream);   /* B += cfl/2 curl E, fdtd2 */
+int rk_push_e(rk_world* w, int32_t tile, void* stream);        /* E += cfl curl B, fdtd2 */
+int rk_add_current(rk_world* w, int32_t tile, void* stream);   /* E -= J */
+int rk_filter_current(rk_world* w, int32_t tile, void* stream);/* binomial (1,2,1)^3 pass */
+
+/* ---- pic::Tile methods (src/runko/pic/tile.c++:324-438, tile_communication.c++:66-96) ---- */
+int rk_push_particles(rk_world* w, int32_t tile, void* stream);  /* interpolation + Boris */
+int rk_deposit_current(rk_world* w, int32_t tile, void* stream); /* zigzag, first order */
+int rk_pack_outgoing_particles(rk_world* w, int32_t tile, void* stream);
+int rk_sort_particles(rk_world* w, int32_t tile, void* stream);
+
+/* Fused batched particle phase: interpolation + Boris + tile/brick
+ * classification + zigzag deposit of every particle of the world in one
+ * launch, followed by the migration append.  Equivalent to push_particles,
+ * pack_outgoing_particles, comm_local(pic_particle), deposit_current on every
+ * tile.  No counterpart in the reference (its driver issues those four). */
+int rk_particle_step_fused(rk_world* w, void* stream);
+
+/* corgi local communication for one comm_mode (runko/simulation.py:292-293).
+ * Fields: periodic ghost copy (E, B, J) or ghost fold-add (J_exchange).
+ * pic_particle: append every pending tile-leaver record to its destination
+ * bin with the global periodic wrap (src/runko/pic/particle.h:449-572). */
+int rk_comm_local(rk_world* w, int32_t mode, void* stream);
+
+/* Particle injection: n particles of species s (device pointers, float32 pos/vel,
+ * uint64 ids) appended to the bins of tile by position.  May return
+ * RK_EOVERFLOW: then the caller repacks (rk_repack) and calls rk_retry_append. */
+int rk_inject(rk_world* w, int32_t tile, int32_t s, int64_t n, const float* const pos[3],
+              const float* const vel[3], const uint64_t* id, void* stream);
+
+/* Overflow handling (synchronous).  rk_pending_overflow reports how many
+ * records are waiting because their bin was full, and fills per-bin demand
+ * (device uint32[n_bricks] per species, counts clamped to capacity). */
+int rk_pending_overflow(rk_world* w, int64_t* n_pending, void* stream);
+int rk_bin_demand(rk_world* w, int32_t s, uint32_t* demand_dev, void* stream);
+/* Copies every bin of species s from the borrowed pool into a new pool with
+ * new bin starts/caps (device), then borrows the new pool. */
+int rk_repack(rk_world* w, int32_t s, int64_t new_capacity, float* const pos[3], float* const vel[3],
+              uint64_t* id, int64_t* bin_start, uint32_t* bin_cap, void* stream);
+int rk_retry_append(rk_world* w, void* stream);
+
+/* Copies of one tile's alive particles of species s in bin order into caller
+ * device buffers (capacity >= count); *count_out set synchronously. */
+int rk_tile_particle_count(rk_world* w, int32_t tile, int32_t s, int64_t* count_out, void* stream);
+int rk_tile_particles(rk_world* w, int32_t tile, int32_t s, float* const pos[3], float* const vel[3],
+                      uint64_t* id, int64_t capacity, int64_t* count_out, void* stream);
+
+/* Pending tile-leaver records (after pack, before comm): count and copies
+ * (pos, vel, id, source tile, direction index, species).  Synchronous. */
+int rk_outgoing(rk_world* w, float* const pos[3], float* const vel[3], uint64_t* id, int32_t* src_tile,
+                int32_t* dir, int32_t* species, int64_t capacity, int64_t* count_out, void* stream);
+
+/* FP64 reductions over the tile (or world interior for tile = -1):
+ * out[0] = sum(E^2)/2, out[1] = sum(B^2)/2, out[2+s] = sum_alive(gamma-1).
+ * Synchronous.  src/runko/emf/yee_lattice.c++:383-428, src/runko/pic/particle.c++:351-377 */
+int rk_energies(rk_world* w, int32_t tile, double* out, void* stream);
+
+/* Gauss-law residual (FP64): rho = first-order (CIC) charge on the nodes where
+ * div E is centred, optionally passed through the same number of binomial
+ * passes as the current; out[0] = max|div E - rho|, out[1] = rms(rho),
+ * out[2] = rms(div E).  Synchronous. */
+int rk_gauss_residual(rk_world* w, int32_t filter_passes, double* out, void* stream);
+
+/* Device error counters: particles whose stencil left the haloed lattice
+ * (reference behaviour undefined; clamped here).  Synchronous. */
+int rk_error_counters(rk_world* w, int64_t* out2, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* RUNKO_AMD_H */

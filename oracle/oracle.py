@@ -1,0 +1,211 @@
+"""TEST INFRASTRUCTURE ONLY -- ctypes wrapper of the oracle (liborc.so).
+
+The oracle is the CPU restatement of runko's PIC time step (see pic_oracle.cpp
+for the per-function citations).  Only tests/, __graft_entry__.smoke() and the
+CPU-baseline leg of bench.py may import this module, and only as the checker /
+the timed CPU baseline.  The product package runko_amd never imports it.
+"""
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = os.path.join(_HERE, "liborc.so")
+_lib = None
+
+COMM = {"emf_J": 0, "emf_E": 1, "emf_B": 2, "pic_particle": 3, "emf_J_exchange": 6}
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB):
+            build()
+        L = ctypes.CDLL(_LIB)
+        vp, ip, dp = ctypes.c_void_p, ctypes.c_int, ctypes.c_double
+        L.orc_new.restype = vp
+        L.orc_new.argtypes = [ip, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int), dp, ip,
+                              ctypes.POINTER(dp), ctypes.POINTER(dp), ip]
+        L.orc_free.argtypes = [vp]
+        L.orc_last_error.restype = ctypes.c_char_p
+        for name in ["push_half_b", "push_e", "add_current", "filter_current", "push_particles",
+                     "deposit_current", "pack_outgoing", "sort_particles"]:
+            f = getattr(L, "orc_" + name)
+            f.argtypes = [vp, ip]
+            f.restype = ip
+        L.orc_comm_local.argtypes = [vp, ip]
+        L.orc_lap.argtypes = [vp, ctypes.c_long]
+        L.orc_set_fields.argtypes = [vp, ip, vp]
+        L.orc_get_fields_halo.argtypes = [vp, ip, vp]
+        L.orc_add_particles.argtypes = [vp, ip, ip, ctypes.c_long, vp, vp]
+        L.orc_num_slots.argtypes = [vp, ip, ip]
+        L.orc_num_slots.restype = ctypes.c_long
+        L.orc_num_alive.argtypes = [vp, ip, ip]
+        L.orc_num_alive.restype = ctypes.c_long
+        L.orc_get_particles.argtypes = [vp, ip, ip, vp, vp]
+        L.orc_num_outgoing.argtypes = [vp, ip]
+        L.orc_num_outgoing.restype = ctypes.c_long
+        L.orc_get_outgoing.argtypes = [vp, ip, vp, vp, vp, vp]
+        L.orc_energies.argtypes = [vp, vp]
+        L.orc_max_threads.restype = ip
+        L.orc_set_threads.argtypes = [ip]
+        _lib = L
+    return _lib
+
+
+def _check(rc):
+    if rc != 0:
+        raise RuntimeError(lib().orc_last_error().decode())
+
+
+class OracleWorld:
+    """A grid of reference-style tiles (each with its own haloed lattice and
+    particle containers).  prec = 32 (FP32, the reference arithmetic) or 64."""
+
+    def __init__(self, n_tiles, n_cells, cfl, charges, masses, prec=32, current_filter="binomial2"):
+        L = lib()
+        self.n_tiles = tuple(int(v) for v in n_tiles)
+        self.n = tuple(int(v) for v in n_cells)
+        self.nsp = len(charges)
+        nt = (ctypes.c_int * 3)(*self.n_tiles)
+        nn = (ctypes.c_int * 3)(*self.n)
+        q = (ctypes.c_double * self.nsp)(*charges)
+        m = (ctypes.c_double * self.nsp)(*masses)
+        fk = {"binomial2": 0, "binomial2_unrolled": 1}[current_filter]
+        self._h = L.orc_new(prec, nt, nn, float(cfl), self.nsp, q, m, fk)
+        self.prec = prec
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            lib().orc_free(self._h)
+            self._h = None
+
+    def tile_index(self, i, j, k):
+        return (i * self.n_tiles[1] + j) * self.n_tiles[2] + k
+
+    @property
+    def ntiles(self):
+        return self.n_tiles[0] * self.n_tiles[1] * self.n_tiles[2]
+
+    # ---- fields
+    def set_fields(self, tile, F):
+        """F: float64 [9, nx, ny, nz] interior values Ex..Jz."""
+        F = np.ascontiguousarray(F, dtype=np.float64)
+        assert F.shape == (9,) + self.n
+        _check(lib().orc_set_fields(self._h, tile, F.ctypes.data))
+
+    def get_fields_halo(self, tile):
+        Ls = tuple(v + 6 for v in self.n)
+        out = np.empty((9,) + Ls, dtype=np.float64)
+        _check(lib().orc_get_fields_halo(self._h, tile, out.ctypes.data))
+        return out
+
+    def get_fields(self, tile):
+        f = self.get_fields_halo(tile)
+        return f[:, 3:-3, 3:-3, 3:-3].copy()
+
+    def global_fields(self):
+        """[9, Nx, Ny, Nz] interior fields assembled from all tiles."""
+        N = tuple(a * b for a, b in zip(self.n_tiles, self.n))
+        out = np.empty((9,) + N)
+        for i in range(self.n_tiles[0]):
+            for j in range(self.n_tiles[1]):
+                for k in range(self.n_tiles[2]):
+                    t = self.tile_index(i, j, k)
+                    nx, ny, nz = self.n
+                    out[:, i * nx:(i + 1) * nx, j * ny:(j + 1) * ny, k * nz:(k + 1) * nz] = self.get_fields(t)
+        return out
+
+    # ---- particles
+    def add_particles(self, tile, species, states, ids=None):
+        """states: float64 [n, 6] (x y z ux uy uz)."""
+        st = np.ascontiguousarray(states, dtype=np.float64).reshape(-1, 6)
+        idp = None
+        if ids is not None:
+            ids = np.ascontiguousarray(ids, dtype=np.uint64)
+            idp = ids.ctypes.data
+        _check(lib().orc_add_particles(self._h, tile, species, st.shape[0], st.ctypes.data, idp))
+
+    def num_slots(self, tile, s):
+        return lib().orc_num_slots(self._h, tile, s)
+
+    def num_alive(self, tile, s):
+        return lib().orc_num_alive(self._h, tile, s)
+
+    def get_particles(self, tile, s):
+        n = self.num_alive(tile, s)
+        out = np.empty((n, 6), dtype=np.float64)
+        ids = np.empty(n, dtype=np.uint64)
+        _check(lib().orc_get_particles(self._h, tile, s, out.ctypes.data, ids.ctypes.data))
+        return out, ids
+
+    def get_outgoing(self, tile):
+        n = lib().orc_num_outgoing(self._h, tile)
+        out = np.empty((n, 6), dtype=np.float64)
+        ids = np.empty(n, dtype=np.uint64)
+        dirs = np.empty(n, dtype=np.int32)
+        sp = np.empty(n, dtype=np.int32)
+        _check(lib().orc_get_outgoing(self._h, tile, out.ctypes.data, ids.ctypes.data, dirs.ctypes.data,
+                                      sp.ctypes.data))
+        return out, ids, dirs, sp
+
+    # ---- methods (tile=-1: all tiles)
+    def push_half_b(self, tile=-1):
+        _check(lib().orc_push_half_b(self._h, tile))
+
+    def push_e(self, tile=-1):
+        _check(lib().orc_push_e(self._h, tile))
+
+    def add_current(self, tile=-1):
+        _check(lib().orc_add_current(self._h, tile))
+
+    def filter_current(self, tile=-1):
+        _check(lib().orc_filter_current(self._h, tile))
+
+    def push_particles(self, tile=-1):
+        _check(lib().orc_push_particles(self._h, tile))
+
+    def deposit_current(self, tile=-1):
+        _check(lib().orc_deposit_current(self._h, tile))
+
+    def pack_outgoing(self, tile=-1):
+        _check(lib().orc_pack_outgoing(self._h, tile))
+
+    def sort_particles(self, tile=-1):
+        _check(lib().orc_sort_particles(self._h, tile))
+
+    def comm_local(self, mode):
+        _check(lib().orc_comm_local(self._h, COMM[mode] if isinstance(mode, str) else int(mode)))
+
+    def lap(self, lapno):
+        _check(lib().orc_lap(self._h, lapno))
+
+    def energies(self):
+        out = np.zeros(2 + self.nsp)
+        _check(lib().orc_energies(self._h, out.ctypes.data))
+        return out
+
+    def all_particles(self, s):
+        """(tile index per particle, states [n,6], ids) over all tiles."""
+        ts, sts, ids = [], [], []
+        for t in range(self.ntiles):
+            st, i = self.get_particles(t, s)
+            ts.append(np.full(len(i), t, dtype=np.int64))
+            sts.append(st)
+            ids.append(i)
+        return np.concatenate(ts), np.concatenate(sts), np.concatenate(ids)
+
+
+def set_threads(n):
+    lib().orc_set_threads(int(n))
+
+
+def max_threads():
+    return lib().orc_max_threads()

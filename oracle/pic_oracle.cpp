@@ -1,0 +1,1132 @@
+// TEST INFRASTRUCTURE ONLY -- the oracle.
+//
+// CPU restatement of runko's 3D PIC time step (hel-astro-lab/runko v5.0.0) in
+// plain C++20 + OpenMP, templated on the scalar type so one source gives the
+// FP32 build (parity with the reference arithmetic) and the FP64 build (the
+// accuracy yardstick).  It keeps the reference's own data model: every tile owns
+// a haloed Yee lattice (halo 3) and one particle container per species with
+// dead slots, and tiles talk to each other only through the 26-neighbour
+// copy/add/migration semantics of corgi's local communication.
+//
+// Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg may load
+// this library, and only as the checker / the timed CPU baseline.  The product
+// (runko_amd) never links or calls it.
+//
+// Floating-point contract: compiled with -ffp-contract=off, every expression is
+// evaluated exactly in the order the reference source writes it (no FMA, no
+// reassociation).  Where the reference order is unpinned (atomic deposit order,
+// corgi neighbour iteration order, reduce order) this file states the order it
+// uses.
+//
+// Parity status per function (how each is pinned, see tests/test_oracle_*.py):
+//   fdtd2 B/E push          affine-field curl exact, div B preserved, vacuum
+//                           dispersion relation, reference test behaviour
+//   add_current             closed form
+//   binomial2 (+unrolled)   constants preserved, Nyquist annihilated, transfer
+//                           function cos^2 product, reference test behaviour
+//   interpolate + Boris     uniform-field response, gyro phase 2*atan|B0|,
+//                           |u| conservation, FP64 vs FP32 agreement
+//   zigzag deposit          discrete continuity per node, sum J = q dx, static
+//                           particles give zero, reference cancellation tests
+//   pack / append / wrap    brute-force conservation of ids, wrap positions,
+//                           per-direction counts
+//   sort                    permutation invariance, key order
+//   halo copy / J_exchange  J0/2J0/4J0/8J0 multiplicity counts
+//   energies                closed forms
+
+#include <algorithm>
+#include <array>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <memory>
+#include <numeric>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+namespace orc {
+
+constexpr int H = 3;  // halo_size, src/runko/emf/common.h:9
+constexpr std::uint64_t DEAD = UINT64_MAX;  // src/runko/particles_common.h:23
+
+thread_local std::string g_err;
+
+// -------------------------------------------------------------------------
+// Yee lattice of one tile: E, B, J as 3 planar FP arrays of (n+6)^3,
+// layout_right (z fastest).  src/runko/emf/yee_lattice.h:67-108
+// -------------------------------------------------------------------------
+template<class T>
+struct Lattice {
+  int n[3] {};
+  int L[3] {};
+  std::vector<T> E[3], B[3], J[3];
+
+  void init(const int nn[3])
+  {
+    for(int d = 0; d < 3; ++d) {
+      n[d] = nn[d];
+      L[d] = nn[d] + 2 * H;
+    }
+    const std::size_t N = std::size_t(L[0]) * L[1] * L[2];
+    for(int c = 0; c < 3; ++c) {
+      E[c].assign(N, T(0));
+      B[c].assign(N, T(0));
+      J[c].assign(N, T(0));
+    }
+  }
+  std::size_t at(int i, int j, int k) const
+  {
+    return (std::size_t(i) * L[1] + std::size_t(j)) * L[2] + std::size_t(k);
+  }
+};
+
+template<class T>
+struct State {
+  T pos[3];
+  T vel[3];
+  std::uint64_t id;
+};
+
+// ParticleContainer, src/runko/pic/particle.h:55-87
+template<class T>
+struct Container {
+  double charge = 0, mass = 1;
+  std::vector<T> x[3], u[3];
+  std::vector<std::uint64_t> id;
+  std::size_t size() const { return id.size(); }
+  void resize(std::size_t n)
+  {
+    for(int d = 0; d < 3; ++d) {
+      x[d].resize(n);
+      u[d].resize(n);
+    }
+    id.resize(n, DEAD);
+  }
+};
+
+template<class T>
+struct Tile {
+  int idx[3] {};
+  double mins[3] {}, maxs[3] {};
+  std::uint64_t tag = 0;
+  Lattice<T> lat;
+  std::vector<Container<T>> sp;
+  std::vector<std::uint64_t> next_ordinal;
+  // pack_outgoing_particles state, src/runko/pic/tile_communication.c++:66-96
+  std::vector<State<T>> out;
+  std::vector<std::size_t> ends;
+  // incoming spans per species: (source tile, begin, end)
+  std::vector<std::vector<std::array<std::size_t, 3>>> incoming;
+};
+
+template<class T>
+struct World {
+  using value_type = T;
+  int nt[3] {};
+  int n[3] {};
+  double cfl = 0.45;
+  int filter_kind = 0;  // 0 binomial2 (27-tap), 1 binomial2_unrolled
+  double gmin[3] {0, 0, 0}, gmax[3] {};
+  std::vector<Tile<T>> tiles;
+
+  int ntiles() const { return nt[0] * nt[1] * nt[2]; }
+  int tile_index(int i, int j, int k) const { return (i * nt[1] + j) * nt[2] + k; }
+  // periodic Moore neighbour of tile t in direction d
+  int neighbour(int t, const int d[3]) const
+  {
+    const auto& a = tiles[t].idx;
+    int i = (a[0] + d[0] + nt[0]) % nt[0];
+    int j = (a[1] + d[1] + nt[1]) % nt[1];
+    int k = (a[2] + d[2] + nt[2]) % nt[2];
+    return tile_index(i, j, k);
+  }
+};
+
+// neighbour index of a direction, src/runko/communication_common.h:127-147
+inline int dir_index(int i, int j, int k) { return 9 * (i + 1) + 3 * (j + 1) + (k + 1); }
+inline void dir_from_index(int n, int d[3])
+{
+  d[0] = n / 9 - 1;
+  d[1] = (n / 3) % 3 - 1;
+  d[2] = n % 3 - 1;
+}
+
+// -------------------------------------------------------------------------
+// Fields
+// -------------------------------------------------------------------------
+
+// push_b_fdtd2, src/runko/emf/yee_lattice_fdtd2.c++:19-58 (dt = cfl/2 cast to
+// the lattice type, src/runko/emf/tile.c++:358-375)
+template<class T>
+void push_half_b(Tile<T>& t, double cfl)
+{
+  auto& l = t.lat;
+  const T dt = static_cast<T>(cfl / 2);
+  auto &Ex = l.E[0], &Ey = l.E[1], &Ez = l.E[2];
+  auto &Bx = l.B[0], &By = l.B[1], &Bz = l.B[2];
+  const std::size_t si = std::size_t(l.L[1]) * l.L[2], sj = l.L[2];
+  for(int i = H; i < H + l.n[0]; ++i)
+    for(int j = H; j < H + l.n[1]; ++j)
+      for(int k = H; k < H + l.n[2]; ++k) {
+        const std::size_t c = l.at(i, j, k);
+        const T DkEy = Ey[c + 1] - Ey[c];
+        const T DjEz = Ez[c + sj] - Ez[c];
+        Bx[c] = Bx[c] + dt * (DkEy - DjEz);
+        const T DiEz = Ez[c + si] - Ez[c];
+        const T DkEx = Ex[c + 1] - Ex[c];
+        By[c] = By[c] + dt * (DiEz - DkEx);
+        const T DjEx = Ex[c + sj] - Ex[c];
+        const T DiEy = Ey[c + si] - Ey[c];
+        Bz[c] = Bz[c] + dt * (DjEx - DiEy);
+      }
+}
+
+// push_e_fdtd2, src/runko/emf/yee_lattice_fdtd2.c++:68-111 (dt = cfl)
+template<class T>
+void push_e(Tile<T>& t, double cfl)
+{
+  auto& l = t.lat;
+  const T dt = static_cast<T>(cfl);
+  auto &Ex = l.E[0], &Ey = l.E[1], &Ez = l.E[2];
+  auto &Bx = l.B[0], &By = l.B[1], &Bz = l.B[2];
+  const std::size_t si = std::size_t(l.L[1]) * l.L[2], sj = l.L[2];
+  for(int i = H; i < H + l.n[0]; ++i)
+    for(int j = H; j < H + l.n[1]; ++j)
+      for(int k = H; k < H + l.n[2]; ++k) {
+        const std::size_t c = l.at(i, j, k);
+        const T DkBy = By[c - 1] - By[c];
+        const T DjBz = Bz[c - sj] - Bz[c];
+        Ex[c] = Ex[c] + dt * (DkBy - DjBz);
+        const T DiBz = Bz[c - si] - Bz[c];
+        const T DkBx = Bx[c - 1] - Bx[c];
+        Ey[c] = Ey[c] + dt * (DiBz - DkBx);
+        const T DjBx = Bx[c - sj] - Bx[c];
+        const T DiBy = By[c - si] - By[c];
+        Ez[c] = Ez[c] + dt * (DjBx - DiBy);
+      }
+}
+
+// add_current, src/runko/emf/yee_lattice.c++:170-179
+template<class T>
+void add_current(Tile<T>& t)
+{
+  auto& l = t.lat;
+  for(int c = 0; c < 3; ++c)
+    for(int i = H; i < H + l.n[0]; ++i)
+      for(int j = H; j < H + l.n[1]; ++j)
+        for(int k = H; k < H + l.n[2]; ++k) {
+          const std::size_t q = l.at(i, j, k);
+          l.E[c][q] = l.E[c][q] - l.J[c][q];
+        }
+}
+
+// filter_current_binomial2 (27-tap, layout_right tap order, into a fresh
+// zero grid), src/runko/emf/yee_lattice_current_filter_binomial2.c++:23-76
+template<class T>
+void filter_binomial2(Tile<T>& t)
+{
+  auto& l = t.lat;
+  static const T C3[3][3][3] = {
+    {{T(1. / 64.), T(2. / 64.), T(1. / 64.)},
+     {T(2. / 64.), T(4. / 64.), T(2. / 64.)},
+     {T(1. / 64.), T(2. / 64.), T(1. / 64.)}},
+    {{T(2. / 64.), T(4. / 64.), T(2. / 64.)},
+     {T(4. / 64.), T(8. / 64.), T(4. / 64.)},
+     {T(2. / 64.), T(4. / 64.), T(2. / 64.)}},
+    {{T(1. / 64.), T(2. / 64.), T(1. / 64.)},
+     {T(2. / 64.), T(4. / 64.), T(2. / 64.)},
+     {T(1. / 64.), T(2. / 64.), T(1. / 64.)}}};
+  for(int c = 0; c < 3; ++c) {
+    std::vector<T> nj(l.J[c].size(), T(0));
+    const auto& J = l.J[c];
+    for(int i = 1; i < l.L[0] - 1; ++i)
+      for(int j = 1; j < l.L[1] - 1; ++j)
+        for(int k = 1; k < l.L[2] - 1; ++k) {
+          T acc = T(0);
+          for(int a = 0; a < 3; ++a)
+            for(int b = 0; b < 3; ++b)
+              for(int cc = 0; cc < 3; ++cc)
+                acc = acc + C3[a][b][cc] * J[l.at(i + a - 1, j + b - 1, k + cc - 1)];
+          nj[l.at(i, j, k)] = acc;
+        }
+    l.J[c] = std::move(nj);
+  }
+}
+
+// filter_current_binomial2_unrolled: separable z, y, x passes, outer shell
+// untouched, src/runko/emf/yee_lattice_current_filter_binomial2.c++:88-157
+template<class T>
+void filter_binomial2_unrolled(Tile<T>& t)
+{
+  auto& l = t.lat;
+  const T B1[3] = {T(0.25f), T(0.5f), T(0.25f)};
+  const int Nx = l.L[0], Ny = l.L[1], Nz = l.L[2];
+  std::vector<T> t1(std::size_t(Nx) * Ny * (Nz - 2)), t2(std::size_t(Nx) * (Ny - 2) * (Nz - 2));
+  auto i1 = [&](int i, int j, int k) { return (std::size_t(i) * Ny + j) * (Nz - 2) + k; };
+  auto i2 = [&](int i, int j, int k) { return (std::size_t(i) * (Ny - 2) + j) * (Nz - 2) + k; };
+  for(int c = 0; c < 3; ++c) {
+    auto& J = l.J[c];
+    for(int i = 0; i < Nx; ++i)
+      for(int j = 0; j < Ny; ++j)
+        for(int k = 0; k < Nz - 2; ++k)
+          t1[i1(i, j, k)] = B1[0] * J[l.at(i, j, k)] + B1[1] * J[l.at(i, j, k + 1)] +
+                            B1[2] * J[l.at(i, j, k + 2)];
+    for(int i = 0; i < Nx; ++i)
+      for(int j = 0; j < Ny - 2; ++j)
+        for(int k = 0; k < Nz - 2; ++k)
+          t2[i2(i, j, k)] = B1[0] * t1[i1(i, j, k)] + B1[1] * t1[i1(i, j + 1, k)] +
+                            B1[2] * t1[i1(i, j + 2, k)];
+    for(int i = 0; i < Nx - 2; ++i)
+      for(int j = 0; j < Ny - 2; ++j)
+        for(int k = 0; k < Nz - 2; ++k)
+          J[l.at(i + 1, j + 1, k + 1)] = B1[0] * t2[i2(i, j, k)] + B1[1] * t2[i2(i + 1, j, k)] +
+                                         B1[2] * t2[i2(i + 2, j, k)];
+  }
+}
+
+// -------------------------------------------------------------------------
+// Particles
+// -------------------------------------------------------------------------
+
+template<class T>
+inline T lerp(T x, T A, T B)
+{
+  return (T(1) - x) * A + x * B;
+}
+
+// lerp3D, src/runko/emf/yee_lattice_interpolate_linear_1st.h:29-51
+// m[ic][jc][kc]
+template<class T>
+inline T lerp3D(const T d[3], const T m[2][2][2])
+{
+  const T c00 = lerp(d[0], m[0][0][0], m[1][0][0]);
+  const T c01 = lerp(d[0], m[0][0][1], m[1][0][1]);
+  const T c10 = lerp(d[0], m[0][1][0], m[1][1][0]);
+  const T c11 = lerp(d[0], m[0][1][1], m[1][1][1]);
+  const T c0 = lerp(d[1], c00, c10);
+  const T c1 = lerp(d[1], c01, c11);
+  return lerp(d[2], c0, c1);
+}
+
+// interpolate_EB_linear_1st, src/runko/emf/yee_lattice_interpolate_linear_1st.h:57-138
+// mean(...) is a right fold divided by the count: (a+(b+(c+d)))/4.
+template<class T>
+inline void interpolate(const Lattice<T>& l, const T origo[3], const T pos[3], T E[3], T B[3])
+{
+  const T p[3] = {pos[0] - origo[0], pos[1] - origo[1], pos[2] - origo[2]};
+  const std::uint32_t i = static_cast<std::uint32_t>(p[0]);
+  const std::uint32_t j = static_cast<std::uint32_t>(p[1]);
+  const std::uint32_t k = static_cast<std::uint32_t>(p[2]);
+  T ex[2][2][2], ey[2][2][2], ez[2][2][2], bx[2][2][2], by[2][2][2], bz[2][2][2];
+  for(int ic = 0; ic < 2; ++ic)
+    for(int jc = 0; jc < 2; ++jc)
+      for(int kc = 0; kc < 2; ++kc) {
+        const int ii = int(i) + ic, jj = int(j) + jc, kk = int(k) + kc;
+        ex[ic][jc][kc] = (l.E[0][l.at(ii - 1, jj, kk)] + l.E[0][l.at(ii, jj, kk)]) / T(2);
+        ey[ic][jc][kc] = (l.E[1][l.at(ii, jj - 1, kk)] + l.E[1][l.at(ii, jj, kk)]) / T(2);
+        ez[ic][jc][kc] = (l.E[2][l.at(ii, jj, kk - 1)] + l.E[2][l.at(ii, jj, kk)]) / T(2);
+        bx[ic][jc][kc] =
+          (l.B[0][l.at(ii, jj, kk)] +
+           (l.B[0][l.at(ii, jj - 1, kk)] +
+            (l.B[0][l.at(ii, jj, kk - 1)] + l.B[0][l.at(ii, jj - 1, kk - 1)]))) /
+          T(4);
+        by[ic][jc][kc] =
+          (l.B[1][l.at(ii, jj, kk)] +
+           (l.B[1][l.at(ii - 1, jj, kk)] +
+            (l.B[1][l.at(ii, jj, kk - 1)] + l.B[1][l.at(ii - 1, jj, kk - 1)]))) /
+          T(4);
+        bz[ic][jc][kc] =
+          (l.B[2][l.at(ii, jj, kk)] +
+           (l.B[2][l.at(ii - 1, jj, kk)] +
+            (l.B[2][l.at(ii, jj - 1, kk)] + l.B[2][l.at(ii - 1, jj - 1, kk)]))) /
+          T(4);
+      }
+  const T d[3] = {p[0] - static_cast<T>(i), p[1] - static_cast<T>(j), p[2] - static_cast<T>(k)};
+  E[0] = lerp3D(d, ex);
+  E[1] = lerp3D(d, ey);
+  E[2] = lerp3D(d, ez);
+  B[0] = lerp3D(d, bx);
+  B[1] = lerp3D(d, by);
+  B[2] = lerp3D(d, bz);
+}
+
+template<class T>
+inline T dot3(const T a[3], const T b[3])
+{
+  T r = T(0);
+  r += a[0] * b[0];
+  r += a[1] * b[1];
+  r += a[2] * b[2];
+  return r;
+}
+
+// toolbox::cross, src/runko/tools/vector.h (ret(1) = -v1(0)*v2(2) + v1(2)*v2(0))
+template<class T>
+inline void cross3(const T a[3], const T b[3], T r[3])
+{
+  r[0] = a[1] * b[2] - a[2] * b[1];
+  r[1] = -a[0] * b[2] + a[2] * b[0];
+  r[2] = a[0] * b[1] - a[1] * b[0];
+}
+
+inline int sign_of(double v) { return (0.0 < v) - (v < 0.0); }
+
+// push_particles_boris, src/runko/pic/particle_boris.h:15-62
+template<class T>
+void boris_one(const T cfl, const T qm, const T EBE[3], const T EBB[3], T pos[3], T vel[3])
+{
+  T v0[3], E0[3], u0[3];
+  const T hq = T(0.5) * qm;
+  for(int c = 0; c < 3; ++c) v0[c] = vel[c] * cfl;
+  for(int c = 0; c < 3; ++c) E0[c] = EBE[c] * hq;
+  for(int c = 0; c < 3; ++c) u0[c] = v0[c] + E0[c];
+  const T ginv = cfl / std::sqrt(cfl * cfl + dot3(u0, u0));
+  const T bfac = hq * ginv;
+  T B0[3];
+  for(int c = 0; c < 3; ++c) B0[c] = (EBB[c] * bfac) / cfl;
+  const T f = T(2) / (T(1) + dot3(B0, B0));
+  T cr[3], u1[3], u2[3];
+  cross3(u0, B0, cr);
+  for(int c = 0; c < 3; ++c) u1[c] = (u0[c] + cr[c]) * f;
+  cross3(u1, B0, cr);
+  for(int c = 0; c < 3; ++c) u2[c] = (u0[c] + cr[c]) + E0[c];
+  for(int c = 0; c < 3; ++c) vel[c] = u2[c] / cfl;
+  const T ginv2 = cfl / std::sqrt(cfl * cfl + dot3(u2, u2));
+  for(int c = 0; c < 3; ++c) pos[c] = pos[c] + (vel[c] * ginv2) * cfl;
+}
+
+template<class T>
+inline void tile_origo(const Tile<T>& t, T o[3])
+{
+  // static_cast<float>(mins) - halo_size, src/runko/pic/tile.c++:329-332
+  for(int d = 0; d < 3; ++d) o[d] = static_cast<T>(t.mins[d]) - static_cast<T>(H);
+}
+
+template<class T>
+void push_particles(Tile<T>& t, double cfl_d)
+{
+  T origo[3];
+  tile_origo(t, origo);
+  const T cfl = static_cast<T>(cfl_d);
+  for(auto& c : t.sp) {
+    const T qm = static_cast<T>(sign_of(c.charge) / c.mass);
+    const std::size_t N = c.size();
+    for(std::size_t n = 0; n < N; ++n) {
+      if(c.id[n] == DEAD) continue;
+      T pos[3] = {c.x[0][n], c.x[1][n], c.x[2][n]};
+      T vel[3] = {c.u[0][n], c.u[1][n], c.u[2][n]};
+      T E[3], B[3];
+      interpolate(t.lat, origo, pos, E, B);
+      boris_one(cfl, qm, E, B, pos, vel);
+      for(int d = 0; d < 3; ++d) {
+        c.x[d][n] = pos[d];
+        c.u[d][n] = vel[d];
+      }
+    }
+  }
+}
+
+// current_zigzag_1st (atomic overload; the sort+reduce overload produces the
+// same contributions), src/runko/pic/particle_current_zigzag_1st.c++:225-339.
+// Contributions are summed in particle order, then in the store order below.
+template<class T>
+void zigzag(Container<T>& c, std::vector<T> (&Jg)[3], const Lattice<T>& l, const T origo[3], double cfl_d)
+{
+  const T charge = static_cast<T>(c.charge);
+  const T cfl = static_cast<T>(cfl_d);
+  const T one = T(1);
+  auto store = [&](std::uint32_t i, std::uint32_t j, std::uint32_t k, T jx, T jy, T jz) {
+    const std::size_t q = l.at(int(i), int(j), int(k));
+    Jg[0][q] += jx;
+    Jg[1][q] += jy;
+    Jg[2][q] += jz;
+  };
+  const std::size_t N = c.size();
+  for(std::size_t n = 0; n < N; ++n) {
+    if(c.id[n] == DEAD) continue;
+    const T u[3] = {c.u[0][n], c.u[1][n], c.u[2][n]};
+    const T invgam = one / std::sqrt(one + dot3(u, u));
+    T x2[3], x1[3], fi1[3], fi2[3], xr[3];
+    const T s = cfl * invgam;
+    for(int d = 0; d < 3; ++d) x2[d] = c.x[d][n] - origo[d];
+    for(int d = 0; d < 3; ++d) x1[d] = x2[d] - u[d] * s;
+    for(int d = 0; d < 3; ++d) {
+      fi1[d] = std::floor(x1[d]);
+      fi2[d] = std::floor(x2[d]);
+    }
+    for(int d = 0; d < 3; ++d) {
+      const T a = (fi1[d] < fi2[d] ? fi1[d] : fi2[d]) + one;
+      const T b1 = (fi1[d] > fi2[d] ? fi1[d] : fi2[d]);
+      const T b2 = T(0.5) * (x1[d] + x2[d]);
+      const T b = (b1 > b2 ? b1 : b2);
+      xr[d] = (a < b ? a : b);
+    }
+    T F1[3], F2[3], W1[3], W2[3];
+    for(int d = 0; d < 3; ++d) {
+      F1[d] = (xr[d] - x1[d]) * charge;
+      F2[d] = (x2[d] - xr[d]) * charge;
+      W1[d] = (x1[d] + xr[d]) * T(0.5) - fi1[d];
+      W2[d] = (x2[d] + xr[d]) * T(0.5) - fi2[d];
+    }
+    const std::uint32_t a1 = static_cast<std::uint32_t>(fi1[0]), b1 = static_cast<std::uint32_t>(fi1[1]),
+                        c1 = static_cast<std::uint32_t>(fi1[2]);
+    const std::uint32_t a2 = static_cast<std::uint32_t>(fi2[0]), b2 = static_cast<std::uint32_t>(fi2[1]),
+                        c2 = static_cast<std::uint32_t>(fi2[2]);
+    const T Fx1 = F1[0], Fy1 = F1[1], Fz1 = F1[2], Fx2 = F2[0], Fy2 = F2[1], Fz2 = F2[2];
+    const T Wx1 = W1[0], Wy1 = W1[1], Wz1 = W1[2], Wx2 = W2[0], Wy2 = W2[1], Wz2 = W2[2];
+    store(a1, b1, c1, Fx1 * (one - Wy1) * (one - Wz1), Fy1 * (one - Wx1) * (one - Wz1),
+          Fz1 * (one - Wx1) * (one - Wy1));
+    store(a2, b2, c2, Fx2 * (one - Wy2) * (one - Wz2), Fy2 * (one - Wx2) * (one - Wz2),
+          Fz2 * (one - Wx2) * (one - Wy2));
+    store(a1 + 1, b1, c1, T(0), Fy1 * Wx1 * (one - Wz1), Fz1 * Wx1 * (one - Wy1));
+    store(a2 + 1, b2, c2, T(0), Fy2 * Wx2 * (one - Wz2), Fz2 * Wx2 * (one - Wy2));
+    store(a1, b1 + 1, c1, Fx1 * Wy1 * (one - Wz1), T(0), Fz1 * (one - Wx1) * Wy1);
+    store(a2, b2 + 1, c2, Fx2 * Wy2 * (one - Wz2), T(0), Fz2 * (one - Wx2) * Wy2);
+    store(a1, b1, c1 + 1, Fx1 * (one - Wy1) * Wz1, Fy1 * (one - Wx1) * Wz1, T(0));
+    store(a2, b2, c2 + 1, Fx2 * (one - Wy2) * Wz2, Fy2 * (one - Wx2) * Wz2, T(0));
+    store(a1, b1 + 1, c1 + 1, Fx1 * Wy1 * Wz1, T(0), T(0));
+    store(a2, b2 + 1, c2 + 1, Fx2 * Wy2 * Wz2, T(0), T(0));
+    store(a1 + 1, b1, c1 + 1, T(0), Fy1 * Wx1 * Wz1, T(0));
+    store(a2 + 1, b2, c2 + 1, T(0), Fy2 * Wx2 * Wz2, T(0));
+    store(a1 + 1, b1 + 1, c1, T(0), T(0), Fz1 * Wx1 * Wy1);
+    store(a2 + 1, b2 + 1, c2, T(0), T(0), Fz2 * Wx2 * Wy2);
+  }
+}
+
+// pic::Tile::deposit_current (atomic depositer), src/runko/pic/tile.c++:367-415
+template<class T>
+void deposit_current(Tile<T>& t, double cfl)
+{
+  auto& l = t.lat;
+  for(int c = 0; c < 3; ++c) std::fill(l.J[c].begin(), l.J[c].end(), T(0));
+  std::vector<T> genJ[3];
+  for(int c = 0; c < 3; ++c) genJ[c].assign(l.J[c].size(), T(0));
+  T origo[3];
+  tile_origo(t, origo);
+  for(auto& c : t.sp) zigzag(c, genJ, l, origo, cfl);
+  for(int c = 0; c < 3; ++c)
+    for(std::size_t q = 0; q < l.J[c].size(); ++q) l.J[c][q] = l.J[c][q] + genJ[c][q];
+}
+
+// direction of a position w.r.t. [lo, hi), src/runko/pic/particle.c++:227-236
+template<class T>
+inline int region_of(const T p[3], const T lo[3], const T hi[3])
+{
+  int d[3];
+  for(int a = 0; a < 3; ++a) d[a] = int(p[a] >= lo[a]) - int(p[a] < hi[a]);
+  return dir_index(d[0], d[1], d[2]);
+}
+
+// divide_to_subregions + pack_outgoing_particles,
+// src/runko/pic/particle.c++:198-348, src/runko/pic/tile_communication.c++:66-96.
+// The reference sorts the leavers with an unstable comparator sort; this
+// restatement uses a stable sort (one admissible order).
+template<class T>
+void pack_outgoing(Tile<T>& t)
+{
+  T lo[3], hi[3];
+  for(int d = 0; d < 3; ++d) {
+    lo[d] = static_cast<T>(t.mins[d]);
+    hi[d] = static_cast<T>(t.maxs[d]);
+  }
+  const std::size_t S = t.sp.size();
+  t.ends.assign(27 * S, 0);
+  t.out.clear();
+  for(std::size_t s = 0; s < S; ++s) {
+    auto& c = t.sp[s];
+    const std::size_t prev = t.out.size();
+    std::vector<int> rg;
+    for(std::size_t n = 0; n < c.size(); ++n) {
+      const T p[3] = {c.x[0][n], c.x[1][n], c.x[2][n]};
+      const int r = region_of(p, lo, hi);
+      if(c.id[n] != DEAD && r != 13) {
+        State<T> st;
+        for(int d = 0; d < 3; ++d) {
+          st.pos[d] = c.x[d][n];
+          st.vel[d] = c.u[d][n];
+        }
+        st.id = c.id[n];
+        t.out.push_back(st);
+        rg.push_back(r);
+      }
+    }
+    for(std::size_t n = 0; n < c.size(); ++n) {
+      const T p[3] = {c.x[0][n], c.x[1][n], c.x[2][n]};
+      if(region_of(p, lo, hi) != 13) c.id[n] = DEAD;
+    }
+    // stable sort of the appended part by region index
+    std::vector<std::size_t> perm(rg.size());
+    std::iota(perm.begin(), perm.end(), 0);
+    std::stable_sort(perm.begin(), perm.end(), [&](std::size_t a, std::size_t b) { return rg[a] < rg[b]; });
+    std::vector<State<T>> tmp(rg.size());
+    for(std::size_t q = 0; q < perm.size(); ++q) tmp[q] = t.out[prev + perm[q]];
+    std::copy(tmp.begin(), tmp.end(), t.out.begin() + std::ptrdiff_t(prev));
+    std::size_t cnt[27] = {};
+    for(int r : rg) cnt[r]++;
+    std::size_t next = prev;
+    for(int i = 0; i < 27; ++i) {
+      if(i == 13) {
+        t.ends[27 * s + 13] = next;
+        continue;
+      }
+      next += cnt[i];
+      t.ends[27 * s + i] = next;
+    }
+  }
+}
+
+// ParticleContainer::append, src/runko/pic/particle.h:449-572
+template<class T>
+void append(Container<T>& c, const std::vector<const State<T>*>& spans_ptr,
+            const std::vector<std::size_t>& spans_len, const T* wmin, const T* wmax)
+{
+  if(spans_ptr.empty()) return;
+  std::size_t dead_end = 0;
+  for(std::size_t n = c.size(); n > 0; --n) {
+    if(c.id[n - 1] != DEAD) break;
+    ++dead_end;
+  }
+  const std::size_t P = c.size() - dead_end;
+  std::size_t total = 0;
+  for(auto s : spans_len) total += s;
+  c.resize(P + total);
+  std::size_t off = P;
+  for(std::size_t q = 0; q < spans_ptr.size(); ++q) {
+    const State<T>* sp = spans_ptr[q];
+    for(std::size_t i = 0; i < spans_len[q]; ++i) {
+      const State<T>& st = sp[i];
+      const std::size_t j = off + i;
+      for(int d = 0; d < 3; ++d) {
+        if(wmin && wmax) {
+          const T a = wmin[d], b = wmax[d];
+          const T Ld = b - a;
+          c.x[d][j] = (st.pos[d] < 0 ? b : a) + std::fmod(st.pos[d], Ld);
+        } else {
+          c.x[d][j] = st.pos[d];
+        }
+        c.u[d][j] = st.vel[d];
+      }
+      c.id[j] = st.id;
+    }
+    off += spans_len[q];
+  }
+}
+
+// corgi local communication of particles: gather the neighbours' spans for the
+// inverted direction (src/runko/pic/tile_communication.c++:119-195), then append
+// with a global wrap (postlude, :98-117).  Neighbour iteration order: direction
+// index 0..26 (skipping 13) per tile in tile order.
+template<class T>
+void comm_particles(World<T>& w)
+{
+  const int NT = w.ntiles();
+  for(int t = 0; t < NT; ++t) {
+    auto& me = w.tiles[t];
+    me.incoming.assign(me.sp.size(), {});
+    for(int dn = 0; dn < 27; ++dn) {
+      if(dn == 13) continue;
+      int d[3];
+      dir_from_index(dn, d);
+      const int o = w.neighbour(t, d);
+      const auto& other = w.tiles[o];
+      if(other.ends.empty()) continue;  // nothing packed on that tile
+      const int inv = dir_index(-d[0], -d[1], -d[2]);
+      for(std::size_t s = 0; s < me.sp.size(); ++s) {
+        const std::size_t index = 27 * s + std::size_t(inv);
+        const std::size_t end = other.ends[index];
+        const std::size_t begin = index == 0 ? 0 : other.ends[index - 1];
+        me.incoming[s].push_back({std::size_t(o), begin, end});
+      }
+    }
+  }
+  T wmin[3], wmax[3];
+  for(int d = 0; d < 3; ++d) {
+    wmin[d] = static_cast<T>(w.gmin[d]);
+    wmax[d] = static_cast<T>(w.gmax[d]);
+  }
+#pragma omp parallel for schedule(dynamic)
+  for(int t = 0; t < NT; ++t) {
+    auto& me = w.tiles[t];
+    for(std::size_t s = 0; s < me.incoming.size(); ++s) {
+      std::vector<const State<T>*> ptrs;
+      std::vector<std::size_t> lens;
+      for(const auto& in : me.incoming[s]) {
+        ptrs.push_back(w.tiles[in[0]].out.data() + in[1]);
+        lens.push_back(in[2] - in[1]);
+      }
+      append(me.sp[s], ptrs, lens, wmin, wmax);
+    }
+  }
+  for(int t = 0; t < NT; ++t) {
+    w.tiles[t].incoming.clear();
+  }
+  for(int t = 0; t < NT; ++t) w.tiles[t].ends.clear();
+}
+
+// sort_particles, src/runko/pic/tile.c++:417-438 and
+// ParticleContainer::sort, src/runko/pic/particle.h:574-703 (stable by key)
+template<class T>
+void sort_particles(Tile<T>& t)
+{
+  T origo[3];
+  tile_origo(t, origo);
+  const auto& l = t.lat;
+  for(auto& c : t.sp) {
+    const std::size_t N = c.size();
+    std::vector<std::uint32_t> key(N);
+    for(std::size_t n = 0; n < N; ++n) {
+      if(c.id[n] == DEAD) {
+        key[n] = UINT32_MAX;
+        continue;
+      }
+      const std::uint32_t i = static_cast<std::uint32_t>(c.x[0][n] - origo[0]);
+      const std::uint32_t j = static_cast<std::uint32_t>(c.x[1][n] - origo[1]);
+      const std::uint32_t k = static_cast<std::uint32_t>(c.x[2][n] - origo[2]);
+      key[n] = (i * std::uint32_t(l.L[1]) + j) * std::uint32_t(l.L[2]) + k;
+    }
+    std::vector<std::size_t> perm(N);
+    std::iota(perm.begin(), perm.end(), 0);
+    std::stable_sort(perm.begin(), perm.end(), [&](std::size_t a, std::size_t b) { return key[a] < key[b]; });
+    Container<T> o = c;
+    for(std::size_t q = 0; q < N; ++q) {
+      for(int d = 0; d < 3; ++d) {
+        c.x[d][q] = o.x[d][perm[q]];
+        c.u[d][q] = o.u[d][perm[q]];
+      }
+      c.id[q] = o.id[perm[q]];
+    }
+  }
+}
+
+// -------------------------------------------------------------------------
+// Halo communication, src/runko/emf/tile.c++:476-542,
+// src/runko/emf/yee_lattice.c++:181-261, regions yee_lattice.h:493-602
+// -------------------------------------------------------------------------
+inline void subregion(int dir, int n, int& a, int& b)
+{
+  if(dir == -1) { a = 0; b = H; }
+  else if(dir == 0) { a = H; b = H + n; }
+  else { a = H + n; b = 2 * H + n; }
+}
+inline void corresponding_subregion(int dir, int n, int& a, int& b)
+{
+  const int inv = -dir;
+  if(inv == -1) { a = H; b = 2 * H; }
+  else if(inv == 0) { a = H; b = H + n; }
+  else { a = n; b = H + n; }
+}
+
+template<class T>
+void comm_fields(World<T>& w, int mode)
+{
+  // mode: 0 J copy, 1 E copy, 2 B copy, 6 J exchange (add)
+  const int NT = w.ntiles();
+#pragma omp parallel for schedule(dynamic)
+  for(int t = 0; t < NT; ++t) {
+    auto& me = w.tiles[t];
+    for(int dn = 0; dn < 27; ++dn) {
+      if(dn == 13) continue;
+      int d[3];
+      dir_from_index(dn, d);
+      const int o = w.neighbour(t, d);
+      const auto& other = w.tiles[o];
+      int ma[3], mb[3], oa[3], ob[3];
+      if(mode == 6) {
+        // my corresponding_subregion(inv) += other's subregion(inv)
+        for(int a = 0; a < 3; ++a) {
+          corresponding_subregion(-d[a], me.lat.n[a], ma[a], mb[a]);
+          subregion(-d[a], other.lat.n[a], oa[a], ob[a]);
+        }
+      } else {
+        for(int a = 0; a < 3; ++a) {
+          subregion(d[a], me.lat.n[a], ma[a], mb[a]);
+          corresponding_subregion(d[a], other.lat.n[a], oa[a], ob[a]);
+        }
+      }
+      for(int c = 0; c < 3; ++c) {
+        std::vector<T>* dst;
+        const std::vector<T>* src;
+        if(mode == 1) { dst = &me.lat.E[c]; src = &other.lat.E[c]; }
+        else if(mode == 2) { dst = &me.lat.B[c]; src = &other.lat.B[c]; }
+        else { dst = &me.lat.J[c]; src = &other.lat.J[c]; }
+        for(int i = 0; i < mb[0] - ma[0]; ++i)
+          for(int j = 0; j < mb[1] - ma[1]; ++j)
+            for(int k = 0; k < mb[2] - ma[2]; ++k) {
+              const std::size_t qd = me.lat.at(ma[0] + i, ma[1] + j, ma[2] + k);
+              const std::size_t qs = other.lat.at(oa[0] + i, oa[1] + j, oa[2] + k);
+              if(mode == 6) (*dst)[qd] = (*dst)[qd] + (*src)[qs];
+              else (*dst)[qd] = (*src)[qs];
+            }
+      }
+    }
+  }
+}
+
+template<class T>
+void filter(World<T>& w, Tile<T>& t)
+{
+  if(w.filter_kind == 0) filter_binomial2(t);
+  else filter_binomial2_unrolled(t);
+}
+
+// one lap of projects/pic-turbulence/pic.py:187-221 (IO excluded)
+template<class T>
+void lap(World<T>& w, long lapno)
+{
+  const int NT = w.ntiles();
+  auto each = [&](auto&& f) {
+#pragma omp parallel for schedule(dynamic)
+    for(int t = 0; t < NT; ++t) f(w.tiles[t]);
+  };
+  each([&](Tile<T>& t) { push_half_b(t, w.cfl); });
+  comm_fields(w, 2);
+  each([&](Tile<T>& t) { push_particles(t, w.cfl); });
+  each([&](Tile<T>& t) { pack_outgoing(t); });
+  comm_particles(w);
+  if(lapno % 5 == 0) each([&](Tile<T>& t) { sort_particles(t); });
+  each([&](Tile<T>& t) { deposit_current(t, w.cfl); });
+  comm_fields(w, 6);
+  comm_fields(w, 0);
+  each([&](Tile<T>& t) { filter(w, t); });
+  comm_fields(w, 0);
+  each([&](Tile<T>& t) { filter(w, t); });
+  each([&](Tile<T>& t) { filter(w, t); });
+  each([&](Tile<T>& t) { push_half_b(t, w.cfl); });
+  comm_fields(w, 2);
+  each([&](Tile<T>& t) { push_e(t, w.cfl); });
+  each([&](Tile<T>& t) { add_current(t); });
+  comm_fields(w, 1);
+}
+
+// -------------------------------------------------------------------------
+// Handle
+// -------------------------------------------------------------------------
+struct Handle {
+  int prec;
+  std::unique_ptr<World<float>> wf;
+  std::unique_ptr<World<double>> wd;
+};
+
+template<class T>
+World<T>& W(Handle* h);
+template<>
+World<float>& W<float>(Handle* h) { return *h->wf; }
+template<>
+World<double>& W<double>(Handle* h) { return *h->wd; }
+
+template<class T>
+void init_world(World<T>& w, const int nt[3], const int n[3], double cfl, int nsp, const double* q,
+                const double* m, int filter_kind)
+{
+  for(int d = 0; d < 3; ++d) {
+    w.nt[d] = nt[d];
+    w.n[d] = n[d];
+    w.gmin[d] = 0;
+    w.gmax[d] = double(nt[d]) * n[d];
+  }
+  w.cfl = cfl;
+  w.filter_kind = filter_kind;
+  w.tiles.resize(std::size_t(w.ntiles()));
+  for(int i = 0; i < nt[0]; ++i)
+    for(int j = 0; j < nt[1]; ++j)
+      for(int k = 0; k < nt[2]; ++k) {
+        auto& t = w.tiles[std::size_t(w.tile_index(i, j, k))];
+        t.idx[0] = i;
+        t.idx[1] = j;
+        t.idx[2] = k;
+        const int ii[3] = {i, j, k};
+        for(int d = 0; d < 3; ++d) {
+          t.mins[d] = double(ii[d]) * n[d];
+          t.maxs[d] = double(ii[d] + 1) * n[d];
+        }
+        t.tag = std::uint64_t(w.tile_index(i, j, k));
+        t.lat.init(n);
+        t.sp.resize(std::size_t(nsp));
+        t.next_ordinal.assign(std::size_t(nsp), 0);
+        for(int s = 0; s < nsp; ++s) {
+          t.sp[std::size_t(s)].charge = q[s];
+          t.sp[std::size_t(s)].mass = m[s];
+        }
+      }
+}
+
+}  // namespace orc
+
+using namespace orc;
+
+template<class F>
+void dispatch(Handle* h, F&& f)
+{
+  if(h->prec == 32) f(*h->wf);
+  else f(*h->wd);
+}
+
+template<class F>
+int guard(F&& f)
+{
+  try {
+    f();
+    return 0;
+  } catch(const std::exception& e) {
+    orc::g_err = e.what();
+    return 1;
+  }
+}
+
+#define WT using T [[maybe_unused]] = typename std::remove_reference_t<decltype(w)>::value_type
+
+extern "C" {
+
+const char* orc_last_error() { return orc::g_err.c_str(); }
+
+int orc_max_threads()
+{
+#ifdef _OPENMP
+  return omp_get_max_threads();
+#else
+  return 1;
+#endif
+}
+
+void orc_set_threads(int n)
+{
+#ifdef _OPENMP
+  if(n > 0) omp_set_num_threads(n);
+#else
+  (void)n;
+#endif
+}
+
+void* orc_new(int prec, const int* nt, const int* n, double cfl, int nsp, const double* q, const double* m,
+              int filter_kind)
+{
+  auto* h = new Handle {};
+  h->prec = prec;
+  if(prec == 32) {
+    h->wf = std::make_unique<World<float>>();
+    init_world(*h->wf, nt, n, cfl, nsp, q, m, filter_kind);
+  } else {
+    h->wd = std::make_unique<World<double>>();
+    init_world(*h->wd, nt, n, cfl, nsp, q, m, filter_kind);
+  }
+  return h;
+}
+
+void orc_free(void* hp) { delete static_cast<Handle*>(hp); }
+
+// fields: [9][nx][ny][nz] float64 interior, order Ex Ey Ez Bx By Bz Jx Jy Jz
+int orc_set_fields(void* hp, int tile, const double* f)
+{
+  auto* h = static_cast<Handle*>(hp);
+  return guard([&] { dispatch(h, [&](auto& w) { WT;
+    auto& t = w.tiles.at(std::size_t(tile));
+    auto& l = t.lat;
+    std::size_t q = 0;
+    for(int c = 0; c < 9; ++c) {
+      auto& A = c < 3 ? l.E[c] : (c < 6 ? l.B[c - 3] : l.J[c - 6]);
+      for(int i = 0; i < l.n[0]; ++i)
+        for(int j = 0; j < l.n[1]; ++j)
+          for(int k = 0; k < l.n[2]; ++k) A[l.at(i + H, j + H, k + H)] = static_cast<T>(f[q++]);
+    }
+  }); });
+}
+
+// out: [9][L0][L1][L2] float64 with halo
+int orc_get_fields_halo(void* hp, int tile, double* f)
+{
+  auto* h = static_cast<Handle*>(hp);
+  return guard([&] { dispatch(h, [&](auto& w) { WT;
+    auto& l = w.tiles.at(std::size_t(tile)).lat;
+    std::size_t q = 0;
+    for(int c = 0; c < 9; ++c) {
+      auto& A = c < 3 ? l.E[c] : (c < 6 ? l.B[c - 3] : l.J[c - 6]);
+      for(std::size_t p = 0; p < A.size(); ++p) f[q++] = static_cast<double>(A[p]);
+    }
+  }); });
+}
+
+// particles given as float64 [n][6] = x y z ux uy uz; ids assigned per tile
+// (tile_tag << 40 | ordinal, src/runko/pic/tile.c++:468-481) unless ids given
+int orc_add_particles(void* hp, int tile, int s, long n, const double* xv, const std::uint64_t* ids)
+{
+  auto* h = static_cast<Handle*>(hp);
+  return guard([&] { dispatch(h, [&](auto& w) { WT;
+    auto& t = w.tiles.at(std::size_t(tile));
+    std::vector<State<T>> st(static_cast<std::size_t>(n));
+    for(long p = 0; p < n; ++p) {
+      for(int d = 0; d < 3; ++d) {
+        st[std::size_t(p)].pos[d] = static_cast<T>(xv[6 * p + d]);
+        st[std::size_t(p)].vel[d] = static_cast<T>(xv[6 * p + 3 + d]);
+      }
+      if(ids) st[std::size_t(p)].id = ids[p];
+      else {
+        const std::uint64_t ord = t.next_ordinal.at(std::size_t(s))++;
+        if(ord >= (std::uint64_t(2) << 40)) throw std::runtime_error("PIC tile ran out of particle ids (2^40 per species)!");
+        st[std::size_t(p)].id = (t.tag << 40) | ord;
+      }
+    }
+    std::vector<const State<T>*> ptrs {st.data()};
+    std::vector<std::size_t> lens {st.size()};
+    append(t.sp.at(std::size_t(s)), ptrs, lens, static_cast<const T*>(nullptr), static_cast<const T*>(nullptr));
+  }); });
+}
+
+long orc_num_slots(void* hp, int tile, int s)
+{
+  auto* h = static_cast<Handle*>(hp);
+  long r = 0;
+  dispatch(h, [&](auto& w) { WT; r = long(w.tiles.at(std::size_t(tile)).sp.at(std::size_t(s)).size()); });
+  return r;
+}
+
+long orc_num_alive(void* hp, int tile, int s)
+{
+  auto* h = static_cast<Handle*>(hp);
+  long r = 0;
+  dispatch(h, [&](auto& w) { WT;
+    const auto& c = w.tiles.at(std::size_t(tile)).sp.at(std::size_t(s));
+    for(auto id : c.id) r += (id != DEAD);
+  });
+  return r;
+}
+
+// alive particles in storage order: out [alive][6] float64 and ids
+int orc_get_particles(void* hp, int tile, int s, double* out, std::uint64_t* ids)
+{
+  auto* h = static_cast<Handle*>(hp);
+  return guard([&] { dispatch(h, [&](auto& w) { WT;
+    const auto& c = w.tiles.at(std::size_t(tile)).sp.at(std::size_t(s));
+    std::size_t q = 0;
+    for(std::size_t p = 0; p < c.size(); ++p) {
+      if(c.id[p] == DEAD) continue;
+      for(int d = 0; d < 3; ++d) {
+        out[6 * q + std::size_t(d)] = static_cast<double>(c.x[d][p]);
+        out[6 * q + 3 + std::size_t(d)] = static_cast<double>(c.u[d][p]);
+      }
+      ids[q] = c.id[p];
+      ++q;
+    }
+  }); });
+}
+
+// outgoing buffer after pack: count, then records [cnt][6], ids, dir index
+long orc_num_outgoing(void* hp, int tile)
+{
+  auto* h = static_cast<Handle*>(hp);
+  long r = 0;
+  dispatch(h, [&](auto& w) { WT; r = long(w.tiles.at(std::size_t(tile)).out.size()); });
+  return r;
+}
+
+int orc_get_outgoing(void* hp, int tile, double* out, std::uint64_t* ids, int* dirs, int* species)
+{
+  auto* h = static_cast<Handle*>(hp);
+  return guard([&] { dispatch(h, [&](auto& w) { WT;
+    const auto& t = w.tiles.at(std::size_t(tile));
+    for(std::size_t q = 0; q < t.out.size(); ++q) {
+      for(int d = 0; d < 3; ++d) {
+        out[6 * q + std::size_t(d)] = static_cast<double>(t.out[q].pos[d]);
+        out[6 * q + 3 + std::size_t(d)] = static_cast<double>(t.out[q].vel[d]);
+      }
+      ids[q] = t.out[q].id;
+    }
+    // recover (species, dir) from the cumulative ends
+    std::size_t begin = 0;
+    for(std::size_t idx = 0; idx < t.ends.size(); ++idx) {
+      const std::size_t end = t.ends[idx];
+      for(std::size_t q = begin; q < end; ++q) {
+        dirs[q] = int(idx % 27);
+        species[q] = int(idx / 27);
+      }
+      begin = end;
+    }
+  }); });
+}
+
+// per-tile methods; tile < 0 means every tile (in parallel over tiles)
+#define TILE_METHOD(name, body)                                                   \
+  int orc_##name(void* hp, int tile)                                              \
+  {                                                                               \
+    auto* h = static_cast<Handle*>(hp);                                           \
+    return guard([&] {                                                            \
+      dispatch(h, [&](auto& w) {                                                  \
+        WT;                                                                       \
+        auto f = [&](Tile<T>& t) { body; };                                       \
+        if(tile >= 0) f(w.tiles.at(std::size_t(tile)));                           \
+        else {                                                                    \
+          const int NT = w.ntiles();                                              \
+          _Pragma("omp parallel for schedule(dynamic)") for(int q = 0; q < NT; ++q) \
+            f(w.tiles[std::size_t(q)]);                                           \
+        }                                                                         \
+      });                                                                         \
+    });                                                                           \
+  }
+
+TILE_METHOD(push_half_b, push_half_b(t, w.cfl))
+TILE_METHOD(push_e, push_e(t, w.cfl))
+TILE_METHOD(add_current, add_current(t))
+TILE_METHOD(filter_current, filter(w, t))
+TILE_METHOD(push_particles, push_particles(t, w.cfl))
+TILE_METHOD(deposit_current, deposit_current(t, w.cfl))
+TILE_METHOD(pack_outgoing, pack_outgoing(t))
+TILE_METHOD(sort_particles, sort_particles(t))
+
+// comm_local(mode): emf_J=0, emf_E=1, emf_B=2, pic_particle=3, emf_J_exchange=6
+int orc_comm_local(void* hp, int mode)
+{
+  auto* h = static_cast<Handle*>(hp);
+  return guard([&] { dispatch(h, [&](auto& w) { WT;
+    if(mode == 3) comm_particles(w);
+    else if(mode == 0 || mode == 1 || mode == 2 || mode == 6) comm_fields(w, mode);
+    else throw std::runtime_error("orc_comm_local: unsupported comm mode");
+  }); });
+}
+
+int orc_lap(void* hp, long lapno)
+{
+  auto* h = static_cast<Handle*>(hp);
+  return guard([&] { dispatch(h, [&](auto& w) { WT; lap(w, lapno); }); });
+}
+
+// energies: out[0] = sum_interior(E^2)/2, out[1] = sum(B^2)/2 (per-cell sums of
+// squares in the lattice type, accumulated in FP64), out[2+s] = sum_alive(gamma-1)
+// per species.  src/runko/emf/yee_lattice.c++:383-428, src/runko/pic/particle.c++:351-377
+int orc_energies(void* hp, double* out)
+{
+  auto* h = static_cast<Handle*>(hp);
+  return guard([&] { dispatch(h, [&](auto& w) { WT;
+    double eE = 0, eB = 0;
+    std::vector<double> ke(w.tiles.empty() ? 0 : w.tiles[0].sp.size(), 0.0);
+    for(const auto& t : w.tiles) {
+      const auto& l = t.lat;
+      for(int i = H; i < H + l.n[0]; ++i)
+        for(int j = H; j < H + l.n[1]; ++j)
+          for(int k = H; k < H + l.n[2]; ++k) {
+            const std::size_t q = l.at(i, j, k);
+            const T e2 = l.E[0][q] * l.E[0][q] + l.E[1][q] * l.E[1][q] + l.E[2][q] * l.E[2][q];
+            const T b2 = l.B[0][q] * l.B[0][q] + l.B[1][q] * l.B[1][q] + l.B[2][q] * l.B[2][q];
+            eE += static_cast<double>(e2);
+            eB += static_cast<double>(b2);
+          }
+      for(std::size_t s = 0; s < t.sp.size(); ++s) {
+        const auto& c = t.sp[s];
+        for(std::size_t p = 0; p < c.size(); ++p) {
+          if(c.id[p] == DEAD) continue;
+          const T v[3] = {c.u[0][p], c.u[1][p], c.u[2][p]};
+          const T e = std::sqrt(T(1) + dot3(v, v)) - T(1);
+          ke[s] += static_cast<double>(e);
+        }
+      }
+    }
+    out[0] = eE / 2.0;
+    out[1] = eB / 2.0;
+    for(std::size_t s = 0; s < ke.size(); ++s) out[2 + s] = ke[s];
+  }); });
+}
+
+}  // extern "C"

@@ -1,0 +1,108 @@
+"""ctypes binding of librunko_amd.so (the C ABI declared in include/runko_amd.h).
+
+The HIP library is the only compute path: if it is missing or no GPU is
+visible, every compute call raises.  There is no CPU fallback.
+"""
+
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "librunko_amd.so")
+
+RK_MAX_SPECIES = 4
+
+RK_OK, RK_EINVAL, RK_EHIP, RK_EOVERFLOW, RK_ESTATE = 0, 1, 2, 3, 4
+
+
+class RkError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(msg)
+        self.code = code
+
+
+class WorldDesc(ctypes.Structure):
+    _fields_ = [
+        ("n_cells", ctypes.c_int32 * 3),
+        ("n_tiles", ctypes.c_int32 * 3),
+        ("tile_offset", ctypes.c_int32 * 3),
+        ("grid_tiles", ctypes.c_int32 * 3),
+        ("brick", ctypes.c_int32 * 3),
+        ("periodic", ctypes.c_int32),
+        ("n_species", ctypes.c_int32),
+        ("current_filter", ctypes.c_int32),
+        ("cfl", ctypes.c_double),
+        ("charge", ctypes.c_double * RK_MAX_SPECIES),
+        ("mass", ctypes.c_double * RK_MAX_SPECIES),
+    ]
+
+
+_lib = None
+
+P3 = ctypes.c_void_p * 3
+
+
+def lib():
+    """Load the HIP library (raises if it was not built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(
+            f"runko_amd: HIP library not built ({LIB_PATH} missing); run __graft_entry__.build() "
+            "or python -m runko_amd._build")
+    L = ctypes.CDLL(LIB_PATH)
+    vp, i32, i64 = ctypes.c_void_p, ctypes.c_int32, ctypes.c_int64
+    L.rk_last_error.restype = ctypes.c_char_p
+    L.rk_version.restype = ctypes.c_int
+    L.rk_world_create.argtypes = [ctypes.POINTER(vp), ctypes.POINTER(WorldDesc)]
+    L.rk_world_destroy.argtypes = [vp]
+    L.rk_world_field_nodes.argtypes = [vp]
+    L.rk_world_field_nodes.restype = i64
+    L.rk_world_num_bricks.argtypes = [vp]
+    L.rk_world_num_bricks.restype = i32
+    L.rk_world_set_fields.argtypes = [vp, P3, P3, P3, P3]
+    L.rk_world_j_pointer.argtypes = [vp, i32]
+    L.rk_world_j_pointer.restype = vp
+    L.rk_world_set_species.argtypes = [vp, i32, i64, P3, P3, vp, vp, vp, vp]
+    L.rk_world_set_migration.argtypes = [vp, i32, i64, P3, P3, vp, vp]
+    for name in ["rk_push_half_b", "rk_push_e", "rk_add_current", "rk_filter_current", "rk_push_particles",
+                 "rk_deposit_current", "rk_pack_outgoing_particles", "rk_sort_particles"]:
+        f = getattr(L, name)
+        f.argtypes = [vp, i32, vp]
+    L.rk_particle_step_fused.argtypes = [vp, vp]
+    L.rk_comm_local.argtypes = [vp, i32, vp]
+    L.rk_inject.argtypes = [vp, i32, i32, i64, P3, P3, vp, vp]
+    L.rk_pending_overflow.argtypes = [vp, ctypes.POINTER(i64), vp]
+    L.rk_repack.argtypes = [vp, i32, i64, P3, P3, vp, vp, vp, vp]
+    L.rk_retry_append.argtypes = [vp, vp]
+    L.rk_tile_particle_count.argtypes = [vp, i32, i32, ctypes.POINTER(i64), vp]
+    L.rk_tile_particles.argtypes = [vp, i32, i32, P3, P3, vp, i64, ctypes.POINTER(i64), vp]
+    L.rk_outgoing.argtypes = [vp, P3, P3, vp, vp, vp, vp, i64, ctypes.POINTER(i64), vp]
+    L.rk_energies.argtypes = [vp, i32, vp, vp]
+    L.rk_gauss_residual.argtypes = [vp, i32, vp, vp]
+    L.rk_error_counters.argtypes = [vp, vp, vp]
+    _lib = L
+    return _lib
+
+
+def check(rc):
+    if rc != RK_OK:
+        raise RkError(rc, lib().rk_last_error().decode())
+    return rc
+
+
+def p3(tensors):
+    return P3(*[t.data_ptr() for t in tensors])
+
+
+# exported symbols declared in include/runko_amd.h (checked by the CPU tests)
+EXPORTED = [
+    "rk_last_error", "rk_version", "rk_world_create", "rk_world_destroy", "rk_world_field_nodes",
+    "rk_world_num_bricks", "rk_world_set_fields", "rk_world_j_pointer", "rk_world_set_species",
+    "rk_world_set_migration", "rk_push_half_b", "rk_push_e", "rk_add_current", "rk_filter_current",
+    "rk_push_particles", "rk_deposit_current", "rk_pack_outgoing_particles", "rk_sort_particles",
+    "rk_particle_step_fused", "rk_comm_local", "rk_inject", "rk_pending_overflow", "rk_bin_demand",
+    "rk_repack", "rk_retry_append", "rk_tile_particle_count", "rk_tile_particles", "rk_outgoing",
+    "rk_energies", "rk_gauss_residual", "rk_error_counters",
+]

@@ -1,0 +1,562 @@
+// C-ABI entry points (include/runko_amd.h): world lifecycle, the per-tile /
+// batched methods with the call-sequence rules that make one world lattice
+// behave like the reference's tiles, communication, overflow handling and
+// readback.
+#include "rk_internal.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <new>
+#include <string>
+
+namespace rk {
+
+thread_local std::string g_err;
+
+void set_error(const std::string& msg) { g_err = msg; }
+
+int hip_fail(hipError_t e, const char* where)
+{
+  g_err = std::string("HIP error ") + hipGetErrorString(e) + " at " + where;
+  return RK_EHIP;
+}
+
+}  // namespace rk
+
+using namespace rk;
+
+namespace {
+
+hipStream_t S(void* s) { return reinterpret_cast<hipStream_t>(s); }
+
+int fail(int code, const std::string& msg)
+{
+  set_error(msg);
+  return code;
+}
+
+int sign_of(double v) { return (0.0 < v) - (v < 0.0); }
+
+// interior region of one tile (or the world interior for tile < 0)
+void interior(const rk_world* w, int tile, int r0[3], int r1[3])
+{
+  const Geo& g = w->g;
+  if(tile < 0) {
+    for(int d = 0; d < 3; ++d) {
+      r0[d] = HALO;
+      r1[d] = HALO + g.N[d];
+    }
+    return;
+  }
+  int tc[3];
+  g.tile_coords(tile, tc);
+  for(int d = 0; d < 3; ++d) {
+    r0[d] = HALO + tc[d] * g.n[d];
+    r1[d] = r0[d] + g.n[d];
+  }
+}
+
+// filter range: interior, extended to [1, L-1) on the world boundary (the
+// reference filters [1, L-1) of every tile's haloed lattice)
+void filter_region(const rk_world* w, int tile, int r0[3], int r1[3])
+{
+  const Geo& g = w->g;
+  if(tile < 0) {
+    for(int d = 0; d < 3; ++d) {
+      r0[d] = 1;
+      r1[d] = g.L[d] - 1;
+    }
+    return;
+  }
+  int tc[3];
+  g.tile_coords(tile, tc);
+  interior(w, tile, r0, r1);
+  for(int d = 0; d < 3; ++d) {
+    if(tc[d] == 0) r0[d] = 1;
+    if(tc[d] == g.nt[d] - 1) r1[d] = g.L[d] - 1;
+  }
+}
+
+int check_tile(const rk_world* w, int tile)
+{
+  if(!w) return fail(RK_EINVAL, "null world");
+  if(tile < -1 || tile >= w->g.ntiles()) return fail(RK_EINVAL, "tile index out of range");
+  if(!w->E[0]) return fail(RK_EINVAL, "world fields not set");
+  return RK_OK;
+}
+
+int check_particles(const rk_world* w)
+{
+  for(int s = 0; s < w->d.n_species; ++s)
+    if(!w->sp[s].bin_start) return fail(RK_EINVAL, "particle pool of a species is not set");
+  return RK_OK;
+}
+
+// a consumer of J closes the current deposit generation
+void close_j(rk_world* w) { w->j_accumulating = false; }
+
+int read_counter(rk_world* w, int which, uint32_t& v, hipStream_t s)
+{
+  RK_HIP(hipMemcpyAsync(&v, &w->counters[which], sizeof(uint32_t), hipMemcpyDeviceToHost, s));
+  RK_HIP(hipStreamSynchronize(s));
+  return RK_OK;
+}
+
+int check_lost(rk_world* w, hipStream_t s)
+{
+  uint32_t lost = 0;
+  int rc = read_counter(w, C_MIGLOST, lost, s);
+  if(rc) return rc;
+  if(lost) return fail(RK_EOVERFLOW, "migration record buffer too small (" + std::to_string(lost) +
+                                         " particles could not migrate); grow it and re-pack");
+  return RK_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* rk_last_error(void) { return rk::g_err.c_str(); }
+
+int rk_version(void) { return 1; }
+
+int rk_world_create(rk_world** out, const rk_world_desc* d)
+{
+  if(!out || !d) return fail(RK_EINVAL, "null argument");
+  *out = nullptr;
+  if(d->n_species < 0 || d->n_species > RK_MAX_SPECIES) return fail(RK_EINVAL, "at most 4 particle species");
+  for(int a = 0; a < 3; ++a) {
+    if(d->n_cells[a] < HALO)
+      return fail(RK_EINVAL, "Yee Lattice extents (" + std::to_string(d->n_cells[0]) + ", " +
+                               std::to_string(d->n_cells[1]) + ", " + std::to_string(d->n_cells[2]) +
+                               ") are assumed to be at least halo size: 3");
+    if(d->n_tiles[a] < 1 || d->grid_tiles[a] < 1) return fail(RK_EINVAL, "tile counts must be positive");
+    if(d->brick[a] < 1 || d->n_cells[a] % d->brick[a]) return fail(RK_EINVAL, "brick must divide n_cells");
+  }
+  if(!(d->cfl > 0)) return fail(RK_EINVAL, "cfl must be positive");
+  auto* w = new(std::nothrow) rk_world();
+  if(!w) return fail(RK_EINVAL, "out of host memory");
+  w->d = *d;
+  Geo& g = w->g;
+  int nb = 1;
+  for(int a = 0; a < 3; ++a) {
+    g.n[a] = d->n_cells[a];
+    g.nt[a] = d->n_tiles[a];
+    g.N[a] = d->n_cells[a] * d->n_tiles[a];
+    g.L[a] = g.N[a] + 2 * HALO;
+    g.br[a] = d->brick[a];
+    g.nb[a] = d->n_cells[a] / d->brick[a];
+    g.toff[a] = d->tile_offset[a];
+    g.wmin[a] = 0.0f;
+    g.wmax[a] = float(int64_t(d->grid_tiles[a]) * d->n_cells[a]);
+    nb *= g.nb[a];
+  }
+  g.bpt = nb;
+  g.nbricks = nb * g.ntiles();
+  g.periodic = d->periodic;
+  g.cfl = float(d->cfl);
+  if(int64_t(g.L[0]) * g.L[1] * g.L[2] >= (int64_t(1) << 32)) {
+    delete w;
+    return fail(RK_EINVAL, "world lattice too large for 32-bit node indices");
+  }
+  if(g.nbricks >= (1 << 24)) {
+    delete w;
+    return fail(RK_EINVAL, "too many bricks (>= 2^24)");
+  }
+  for(int s = 0; s < d->n_species; ++s) {
+    w->sp[s].qm = static_cast<float>(double(sign_of(d->charge[s])) / d->mass[s]);
+    w->sp[s].charge = static_cast<float>(d->charge[s]);
+  }
+  hipError_t e = hipMalloc(reinterpret_cast<void**>(&w->counters), sizeof(uint32_t) * C_NUM);
+  if(e == hipSuccess) e = hipMemset(w->counters, 0, sizeof(uint32_t) * C_NUM);
+  w->red_cap = std::max<int64_t>(2 * int64_t(g.L[0]) + 16, 2048);
+  if(e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&w->red), sizeof(double) * size_t(w->red_cap));
+  if(e != hipSuccess) {
+    (void)hipFree(w->counters);
+    (void)hipFree(w->red);
+    delete w;
+    return hip_fail(e, "rk_world_create");
+  }
+  w->filter_done.assign(size_t(g.ntiles()), 0);
+  *out = w;
+  return RK_OK;
+}
+
+int rk_world_destroy(rk_world* w)
+{
+  if(!w) return RK_OK;
+  (void)hipFree(w->counters);
+  (void)hipFree(w->red);
+  (void)hipFree(w->ovf_list);
+  delete w;
+  return RK_OK;
+}
+
+int64_t rk_world_field_nodes(const rk_world* w) { return w ? w->g.nodes() : 0; }
+
+int32_t rk_world_num_bricks(const rk_world* w) { return w ? w->g.nbricks : 0; }
+
+int rk_world_set_fields(rk_world* w, float* const E[3], float* const B[3], float* const J[3], float* const Jf[3])
+{
+  if(!w) return fail(RK_EINVAL, "null world");
+  for(int d = 0; d < 3; ++d) {
+    if(!E[d] || !B[d] || !J[d] || !Jf[d]) return fail(RK_EINVAL, "null field pointer");
+    w->E[d] = E[d];
+    w->B[d] = B[d];
+    w->J[d] = J[d];
+    w->Jf[d] = Jf[d];
+  }
+  return RK_OK;
+}
+
+int rk_world_set_species(rk_world* w, int32_t s, int64_t capacity, float* const pos[3], float* const vel[3],
+                         uint64_t* id, int64_t* bin_start, uint32_t* bin_cap, uint32_t* bin_count)
+{
+  if(!w || s < 0 || s >= w->d.n_species) return fail(RK_EINVAL, "invalid species");
+  auto& p = w->sp[s];
+  p.capacity = capacity;
+  for(int d = 0; d < 3; ++d) {
+    p.pos[d] = pos[d];
+    p.vel[d] = vel[d];
+  }
+  p.id = id;
+  p.bin_start = bin_start;
+  p.bin_cap = bin_cap;
+  p.bin_count = bin_count;
+  return RK_OK;
+}
+
+int rk_world_set_migration(rk_world* w, int32_t set, int64_t capacity, float* const pos[3], float* const vel[3],
+                           uint64_t* id, uint32_t* meta)
+{
+  if(!w || set < 0 || set > 1) return fail(RK_EINVAL, "invalid record set");
+  if(capacity >= (int64_t(1) << 31)) return fail(RK_EINVAL, "record set capacity must be < 2^31");
+  auto& m = w->mig[set];
+  m.capacity = capacity;
+  for(int d = 0; d < 3; ++d) {
+    m.pos[d] = pos[d];
+    m.vel[d] = vel[d];
+  }
+  m.id = id;
+  m.meta = meta;
+  const int64_t need = w->mig[0].capacity + w->mig[1].capacity;
+  if(need > w->ovf_cap) {
+    (void)hipFree(w->ovf_list);
+    w->ovf_list = nullptr;
+    RK_HIP(hipMalloc(reinterpret_cast<void**>(&w->ovf_list), sizeof(uint32_t) * size_t(std::max<int64_t>(need, 1))));
+    w->ovf_cap = need;
+  }
+  return RK_OK;
+}
+
+int rk_push_half_b(rk_world* w, int32_t tile, void* stream)
+{
+  if(int rc = check_tile(w, tile)) return rc;
+  int r0[3], r1[3];
+  interior(w, tile, r0, r1);
+  return launch_curl_b(w, r0, r1, static_cast<float>(w->d.cfl / 2), S(stream));
+}
+
+int rk_push_e(rk_world* w, int32_t tile, void* stream)
+{
+  if(int rc = check_tile(w, tile)) return rc;
+  int r0[3], r1[3];
+  interior(w, tile, r0, r1);
+  return launch_curl_e(w, r0, r1, static_cast<float>(w->d.cfl), false, S(stream));
+}
+
+int rk_add_current(rk_world* w, int32_t tile, void* stream)
+{
+  if(int rc = check_tile(w, tile)) return rc;
+  close_j(w);
+  int r0[3], r1[3];
+  interior(w, tile, r0, r1);
+  return launch_add_current(w, r0, r1, S(stream));
+}
+
+int rk_filter_current(rk_world* w, int32_t tile, void* stream)
+{
+  if(int rc = check_tile(w, tile)) return rc;
+  close_j(w);
+  int r0[3], r1[3];
+  filter_region(w, tile, r0, r1);
+  auto swap = [&] {
+    for(int d = 0; d < 3; ++d) std::swap(w->J[d], w->Jf[d]);
+  };
+  if(tile < 0) {
+    if(w->filter_open)
+      return fail(RK_ESTATE, "filter_current: a per-tile filter pass is open; finish it on every tile first");
+    if(int rc = launch_filter(w, r0, r1, S(stream))) return rc;
+    swap();
+    return RK_OK;
+  }
+  if(!w->filter_open) {
+    std::fill(w->filter_done.begin(), w->filter_done.end(), 0);
+    w->filter_done_count = 0;
+    w->filter_open = true;
+  }
+  if(w->filter_done[size_t(tile)])
+    return fail(RK_ESTATE,
+                "filter_current: tile filtered twice in one pass (every tile of the grid must be filtered once "
+                "per pass)");
+  if(int rc = launch_filter(w, r0, r1, S(stream))) return rc;
+  w->filter_done[size_t(tile)] = 1;
+  if(++w->filter_done_count == w->g.ntiles()) {
+    swap();
+    w->filter_open = false;
+  }
+  return RK_OK;
+}
+
+int rk_push_particles(rk_world* w, int32_t tile, void* stream)
+{
+  if(int rc = check_tile(w, tile)) return rc;
+  if(int rc = check_particles(w)) return rc;
+  return launch_push(w, tile, S(stream));
+}
+
+int rk_deposit_current(rk_world* w, int32_t tile, void* stream)
+{
+  if(int rc = check_tile(w, tile)) return rc;
+  if(int rc = check_particles(w)) return rc;
+  if(w->filter_open) return fail(RK_ESTATE, "deposit_current during an open filter pass");
+  if(!w->j_accumulating) {
+    // clear_current + zeroed generated J: J = 0 on the whole lattice (tile.c++:369-398)
+    if(int rc = launch_fill(w->J, w->g.nodes(), 0.0f, S(stream))) return rc;
+    w->j_accumulating = true;
+  }
+  return launch_deposit(w, tile, S(stream));
+}
+
+int rk_pack_outgoing_particles(rk_world* w, int32_t tile, void* stream)
+{
+  if(int rc = check_tile(w, tile)) return rc;
+  if(int rc = check_particles(w)) return rc;
+  if(!w->mig[0].meta || !w->mig[1].meta) return fail(RK_EINVAL, "migration buffers not set");
+  hipStream_t s = S(stream);
+  if(int rc = launch_pack(w, tile, s)) return rc;
+  // brick changes inside the tile are settled at once
+  if(int rc = launch_append(w, 1, s)) return rc;
+  if(int rc = launch_reset_counter(w, C_MIG1, s)) return rc;
+  if(int rc = check_lost(w, s)) return rc;
+  uint32_t ovf = 0;
+  if(int rc = read_counter(w, C_OVF, ovf, s)) return rc;
+  if(ovf) return fail(RK_EOVERFLOW, "particle bins full");
+  return RK_OK;
+}
+
+int rk_sort_particles(rk_world* w, int32_t tile, void* stream)
+{
+  if(int rc = check_tile(w, tile)) return rc;
+  if(int rc = check_particles(w)) return rc;
+  return launch_sort(w, tile, S(stream));
+}
+
+int rk_particle_step_fused(rk_world* w, void* stream)
+{
+  if(int rc = check_tile(w, -1)) return rc;
+  if(int rc = check_particles(w)) return rc;
+  if(!w->g.periodic) return fail(RK_EINVAL, "the fused particle step needs a periodic tile grid");
+  if(w->filter_open) return fail(RK_ESTATE, "particle step during an open filter pass");
+  hipStream_t s = S(stream);
+  if(int rc = launch_fill(w->J, w->g.nodes(), 0.0f, s)) return rc;
+  w->j_accumulating = true;
+  if(int rc = launch_fused(w, s)) return rc;
+  if(int rc = launch_append(w, 0, s)) return rc;
+  if(int rc = launch_append(w, 1, s)) return rc;
+  if(int rc = launch_reset_counter(w, C_MIG0, s)) return rc;
+  return launch_reset_counter(w, C_MIG1, s);
+}
+
+int rk_comm_local(rk_world* w, int32_t mode, void* stream)
+{
+  if(int rc = check_tile(w, -1)) return rc;
+  if(!w->g.periodic) return fail(RK_EINVAL, "comm_local needs a tile grid (stand-alone tile)");
+  hipStream_t s = S(stream);
+  switch(mode) {
+    case RK_COMM_EMF_E: return launch_ghost_copy(w, w->E, s);
+    case RK_COMM_EMF_B: return launch_ghost_copy(w, w->B, s);
+    case RK_COMM_EMF_J: close_j(w); return launch_ghost_copy(w, w->J, s);
+    case RK_COMM_EMF_J_EXCHANGE: close_j(w); return launch_ghost_fold(w, w->J, s);
+    case RK_COMM_PIC_PARTICLE: {
+      if(int rc = check_particles(w)) return rc;
+      if(int rc = launch_append(w, 0, s)) return rc;
+      if(int rc = launch_reset_counter(w, C_MIG0, s)) return rc;
+      if(int rc = check_lost(w, s)) return rc;
+      uint32_t ovf = 0;
+      if(int rc = read_counter(w, C_OVF, ovf, s)) return rc;
+      if(ovf) return fail(RK_EOVERFLOW, "particle bins full");
+      return RK_OK;
+    }
+    default:
+      return fail(RK_EINVAL, "pic::Tile::local_communication does not support given communication mode: " +
+                               std::to_string(mode));
+  }
+}
+
+int rk_inject(rk_world* w, int32_t tile, int32_t s, int64_t n, const float* const pos[3], const float* const vel[3],
+              const uint64_t* id, void* stream)
+{
+  if(int rc = check_tile(w, tile)) return rc;
+  if(tile < 0) return fail(RK_EINVAL, "inject needs one tile");
+  if(s < 0 || s >= w->d.n_species) return fail(RK_EINVAL, "invalid species");
+  if(int rc = check_particles(w)) return rc;
+  if(int rc = launch_inject(w, tile, s, n, pos, vel, id, S(stream))) return rc;
+  uint32_t ovf = 0;
+  if(int rc = read_counter(w, C_OVF, ovf, S(stream))) return rc;
+  if(ovf) return fail(RK_EOVERFLOW, "particle bins full");
+  return RK_OK;
+}
+
+int rk_pending_overflow(rk_world* w, int64_t* n_pending, void* stream)
+{
+  if(!w || !n_pending) return fail(RK_EINVAL, "null argument");
+  if(int rc = check_lost(w, S(stream))) return rc;
+  uint32_t v = 0;
+  if(int rc = read_counter(w, C_OVF, v, S(stream))) return rc;
+  *n_pending = v;
+  return RK_OK;
+}
+
+int rk_bin_demand(rk_world* w, int32_t s, uint32_t* demand_dev, void* stream)
+{
+  if(!w || s < 0 || s >= w->d.n_species) return fail(RK_EINVAL, "invalid species");
+  RK_HIP(hipMemcpyAsync(demand_dev, w->sp[s].bin_count, sizeof(uint32_t) * size_t(w->g.nbricks),
+                        hipMemcpyDeviceToDevice, S(stream)));
+  return RK_OK;
+}
+
+int rk_repack(rk_world* w, int32_t s, int64_t new_capacity, float* const pos[3], float* const vel[3], uint64_t* id,
+              int64_t* bin_start, uint32_t* bin_cap, void* stream)
+{
+  if(!w || s < 0 || s >= w->d.n_species) return fail(RK_EINVAL, "invalid species");
+  Species n = w->sp[s];
+  n.capacity = new_capacity;
+  for(int d = 0; d < 3; ++d) {
+    n.pos[d] = pos[d];
+    n.vel[d] = vel[d];
+  }
+  n.id = id;
+  n.bin_start = bin_start;
+  n.bin_cap = bin_cap;
+  if(int rc = launch_repack(w, s, n, S(stream))) return rc;
+  w->sp[s] = n;
+  return RK_OK;
+}
+
+int rk_retry_append(rk_world* w, void* stream)
+{
+  if(!w) return fail(RK_EINVAL, "null world");
+  if(int rc = launch_retry(w, S(stream))) return rc;
+  uint32_t ovf = 0;
+  if(int rc = read_counter(w, C_OVF, ovf, S(stream))) return rc;
+  if(ovf) return fail(RK_EOVERFLOW, "particle bins still full after repack");
+  return RK_OK;
+}
+
+int rk_tile_particle_count(rk_world* w, int32_t tile, int32_t s, int64_t* count_out, void* stream)
+{
+  if(int rc = check_tile(w, tile)) return rc;
+  if(s < 0 || s >= w->d.n_species) return fail(RK_EINVAL, "invalid species");
+  const int b0 = tile < 0 ? 0 : tile * w->g.bpt;
+  const int nb = tile < 0 ? w->g.nbricks : w->g.bpt;
+  std::vector<uint32_t> c(static_cast<size_t>(nb));
+  RK_HIP(hipMemcpyAsync(c.data(), w->sp[s].bin_count + b0, sizeof(uint32_t) * size_t(nb), hipMemcpyDeviceToHost,
+                        S(stream)));
+  RK_HIP(hipStreamSynchronize(S(stream)));
+  int64_t t = 0;
+  for(auto v : c) t += v;
+  *count_out = t;
+  return RK_OK;
+}
+
+int rk_tile_particles(rk_world* w, int32_t tile, int32_t s, float* const pos[3], float* const vel[3], uint64_t* id,
+                      int64_t capacity, int64_t* count_out, void* stream)
+{
+  if(int rc = check_tile(w, tile)) return rc;
+  if(tile < 0) return fail(RK_EINVAL, "readback needs one tile");
+  if(s < 0 || s >= w->d.n_species) return fail(RK_EINVAL, "invalid species");
+  hipStream_t st = S(stream);
+  const int nb = w->g.bpt;
+  std::vector<uint32_t> c(static_cast<size_t>(nb));
+  RK_HIP(hipMemcpyAsync(c.data(), w->sp[s].bin_count + size_t(tile) * nb, sizeof(uint32_t) * size_t(nb),
+                        hipMemcpyDeviceToHost, st));
+  RK_HIP(hipStreamSynchronize(st));
+  std::vector<int64_t> off(size_t(nb) + 1, 0);
+  for(int b = 0; b < nb; ++b) off[size_t(b) + 1] = off[size_t(b)] + c[size_t(b)];
+  *count_out = off[size_t(nb)];
+  if(off[size_t(nb)] > capacity) return fail(RK_EINVAL, "readback buffer too small");
+  if(off[size_t(nb)] == 0) return RK_OK;
+  int64_t* doff = nullptr;
+  RK_HIP(hipMallocAsync(reinterpret_cast<void**>(&doff), sizeof(int64_t) * off.size(), st));
+  RK_HIP(hipMemcpyAsync(doff, off.data(), sizeof(int64_t) * off.size(), hipMemcpyHostToDevice, st));
+  int rc = launch_gather_tile(w, tile, s, pos, vel, id, doff, st);
+  RK_HIP(hipFreeAsync(doff, st));
+  RK_HIP(hipStreamSynchronize(st));
+  return rc;
+}
+
+int rk_outgoing(rk_world* w, float* const pos[3], float* const vel[3], uint64_t* id, int32_t* src_tile, int32_t* dir,
+                int32_t* species, int64_t capacity, int64_t* count_out, void* stream)
+{
+  if(!w) return fail(RK_EINVAL, "null world");
+  hipStream_t st = S(stream);
+  uint32_t n = 0;
+  if(int rc = read_counter(w, C_MIG0, n, st)) return rc;
+  n = uint32_t(std::min<int64_t>(n, w->mig[0].capacity));
+  *count_out = n;
+  if(int64_t(n) > capacity) return fail(RK_EINVAL, "outgoing buffer too small");
+  if(n == 0) return RK_OK;
+  const auto& m = w->mig[0];
+  for(int d = 0; d < 3; ++d) {
+    RK_HIP(hipMemcpyAsync(pos[d], m.pos[d], sizeof(float) * n, hipMemcpyDeviceToDevice, st));
+    RK_HIP(hipMemcpyAsync(vel[d], m.vel[d], sizeof(float) * n, hipMemcpyDeviceToDevice, st));
+  }
+  RK_HIP(hipMemcpyAsync(id, m.id, sizeof(uint64_t) * n, hipMemcpyDeviceToDevice, st));
+  std::vector<uint32_t> meta(n);
+  RK_HIP(hipMemcpyAsync(meta.data(), m.meta, sizeof(uint32_t) * n, hipMemcpyDeviceToHost, st));
+  RK_HIP(hipStreamSynchronize(st));
+  for(uint32_t r = 0; r < n; ++r) {
+    src_tile[r] = int32_t(meta_bin(meta[r]) / uint32_t(w->g.bpt));
+    dir[r] = meta_dir(meta[r]);
+    species[r] = meta_species(meta[r]);
+  }
+  return RK_OK;
+}
+
+int rk_energies(rk_world* w, int32_t tile, double* out, void* stream)
+{
+  if(int rc = check_tile(w, tile)) return rc;
+  int r0[3], r1[3];
+  interior(w, tile, r0, r1);
+  double fe[2];
+  if(int rc = field_energies(w, r0, r1, fe, S(stream))) return rc;
+  out[0] = fe[0];
+  out[1] = fe[1];
+  if(w->d.n_species > 0 && w->sp[0].bin_start) return kinetic_energies(w, tile, out + 2, S(stream));
+  for(int s = 0; s < w->d.n_species; ++s) out[2 + s] = 0.0;
+  return RK_OK;
+}
+
+int rk_gauss_residual(rk_world* w, int32_t filter_passes, double* out, void* stream)
+{
+  if(int rc = check_tile(w, -1)) return rc;
+  if(int rc = check_particles(w)) return rc;
+  return gauss_residual(w, filter_passes, out, S(stream));
+}
+
+int rk_error_counters(rk_world* w, int64_t* out2, void* stream)
+{
+  if(!w) return fail(RK_EINVAL, "null world");
+  uint32_t a = 0, b = 0;
+  if(int rc = read_counter(w, C_STENCIL, a, S(stream))) return rc;
+  if(int rc = read_counter(w, C_MIGLOST, b, S(stream))) return rc;
+  out2[0] = a;
+  out2[1] = b;
+  return RK_OK;
+}
+
+const void* rk_world_j_pointer(const rk_world* w, int32_t comp) { return w ? w->J[comp] : nullptr; }
+
+}  // extern "C"

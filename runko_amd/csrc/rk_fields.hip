@@ -1,0 +1,395 @@
+// Field kernels on the world lattice: fdtd2 curl updates, E -= J, binomial
+// current filters, periodic ghost copy / fold, FP64 energy partials.
+//
+// Arithmetic follows the reference expression by expression (the library is
+// compiled with -ffp-contract=off, so a*b+c stays two roundings):
+//   push_b_fdtd2  src/runko/emf/yee_lattice_fdtd2.c++:19-58
+//   push_e_fdtd2  src/runko/emf/yee_lattice_fdtd2.c++:68-111
+//   add_current   src/runko/emf/yee_lattice.c++:170-179
+//   binomial2     src/runko/emf/yee_lattice_current_filter_binomial2.c++:23-76
+//   binomial2_unrolled                                       :88-157
+//
+// All kernels are HBM-bound stencils.  Launch shape: 64 lanes along z (the
+// contiguous dimension), 4 rows along y per workgroup, one x plane per
+// grid.z, so every wave reads 256 contiguous bytes per component.
+#include "rk_internal.h"
+
+#include <cmath>
+
+namespace rk {
+
+namespace {
+
+constexpr int BX = 64;
+constexpr int BY = 4;
+
+struct Region {
+  int r0[3];
+  int r1[3];
+};
+
+inline dim3 region_grid(const Region& r)
+{
+  const int ez = r.r1[2] - r.r0[2], ey = r.r1[1] - r.r0[1], ex = r.r1[0] - r.r0[0];
+  return dim3((ez + BX - 1) / BX, (ey + BY - 1) / BY, ex);
+}
+
+__global__ void __launch_bounds__(BX* BY)
+  k_curl_b(Geo g, Region r, float dt, const float* __restrict__ Ex, const float* __restrict__ Ey,
+           const float* __restrict__ Ez, float* __restrict__ Bx, float* __restrict__ By,
+           float* __restrict__ Bz)
+{
+  const int k = r.r0[2] + int(blockIdx.x) * BX + int(threadIdx.x);
+  const int j = r.r0[1] + int(blockIdx.y) * BY + int(threadIdx.y);
+  const int i = r.r0[0] + int(blockIdx.z);
+  if(k >= r.r1[2] || j >= r.r1[1]) return;
+  const uint32_t c = g.at(i, j, k);
+  const uint32_t si = uint32_t(g.L[1]) * uint32_t(g.L[2]), sj = uint32_t(g.L[2]);
+  const float ex = Ex[c], ey = Ey[c], ez = Ez[c];
+  const float DkEy = Ey[c + 1] - ey;
+  const float DjEz = Ez[c + sj] - ez;
+  Bx[c] = Bx[c] + dt * (DkEy - DjEz);
+  const float DiEz = Ez[c + si] - ez;
+  const float DkEx = Ex[c + 1] - ex;
+  By[c] = By[c] + dt * (DiEz - DkEx);
+  const float DjEx = Ex[c + sj] - ex;
+  const float DiEy = Ey[c + si] - ey;
+  Bz[c] = Bz[c] + dt * (DjEx - DiEy);
+}
+
+template<bool ADDJ>
+__global__ void __launch_bounds__(BX* BY)
+  k_curl_e(Geo g, Region r, float dt, const float* __restrict__ Bx, const float* __restrict__ By,
+           const float* __restrict__ Bz, float* __restrict__ Ex, float* __restrict__ Ey,
+           float* __restrict__ Ez, const float* __restrict__ Jx, const float* __restrict__ Jy,
+           const float* __restrict__ Jz)
+{
+  const int k = r.r0[2] + int(blockIdx.x) * BX + int(threadIdx.x);
+  const int j = r.r0[1] + int(blockIdx.y) * BY + int(threadIdx.y);
+  const int i = r.r0[0] + int(blockIdx.z);
+  if(k >= r.r1[2] || j >= r.r1[1]) return;
+  const uint32_t c = g.at(i, j, k);
+  const uint32_t si = uint32_t(g.L[1]) * uint32_t(g.L[2]), sj = uint32_t(g.L[2]);
+  const float bx = Bx[c], by = By[c], bz = Bz[c];
+  const float DkBy = By[c - 1] - by;
+  const float DjBz = Bz[c - sj] - bz;
+  float ex = Ex[c] + dt * (DkBy - DjBz);
+  const float DiBz = Bz[c - si] - bz;
+  const float DkBx = Bx[c - 1] - bx;
+  float ey = Ey[c] + dt * (DiBz - DkBx);
+  const float DjBx = Bx[c - sj] - bx;
+  const float DiBy = By[c - si] - by;
+  float ez = Ez[c] + dt * (DjBx - DiBy);
+  if constexpr(ADDJ) {
+    ex = ex - Jx[c];
+    ey = ey - Jy[c];
+    ez = ez - Jz[c];
+  }
+  Ex[c] = ex;
+  Ey[c] = ey;
+  Ez[c] = ez;
+}
+
+__global__ void __launch_bounds__(BX* BY)
+  k_add_current(Geo g, Region r, float* __restrict__ Ex, float* __restrict__ Ey, float* __restrict__ Ez,
+                const float* __restrict__ Jx, const float* __restrict__ Jy, const float* __restrict__ Jz)
+{
+  const int k = r.r0[2] + int(blockIdx.x) * BX + int(threadIdx.x);
+  const int j = r.r0[1] + int(blockIdx.y) * BY + int(threadIdx.y);
+  const int i = r.r0[0] + int(blockIdx.z);
+  if(k >= r.r1[2] || j >= r.r1[1]) return;
+  const uint32_t c = g.at(i, j, k);
+  Ex[c] = Ex[c] - Jx[c];
+  Ey[c] = Ey[c] - Jy[c];
+  Ez[c] = Ez[c] - Jz[c];
+}
+
+// 27-tap (1,2,1)^3/64, taps in layout_right order, accumulator starting at 0.
+__global__ void __launch_bounds__(BX* BY)
+  k_filter27(Geo g, Region r, const float* __restrict__ S0, const float* __restrict__ S1,
+             const float* __restrict__ S2, float* __restrict__ D0, float* __restrict__ D1,
+             float* __restrict__ D2)
+{
+  const int k = r.r0[2] + int(blockIdx.x) * BX + int(threadIdx.x);
+  const int j = r.r0[1] + int(blockIdx.y) * BY + int(threadIdx.y);
+  const int i = r.r0[0] + int(blockIdx.z);
+  if(k >= r.r1[2] || j >= r.r1[1]) return;
+  const float w[3] = {1.0f, 2.0f, 1.0f};
+  float a0 = 0.0f, a1 = 0.0f, a2 = 0.0f;
+#pragma unroll
+  for(int a = 0; a < 3; ++a)
+#pragma unroll
+    for(int b = 0; b < 3; ++b)
+#pragma unroll
+      for(int cc = 0; cc < 3; ++cc) {
+        // C3[a][b][c] = w[a]*w[b]*w[c]/64, exact in float
+        const float C = (w[a] * w[b] * w[cc]) * (1.0f / 64.0f);
+        const uint32_t q = g.at(i + a - 1, j + b - 1, k + cc - 1);
+        a0 = a0 + C * S0[q];
+        a1 = a1 + C * S1[q];
+        a2 = a2 + C * S2[q];
+      }
+  const uint32_t c = g.at(i, j, k);
+  D0[c] = a0;
+  D1[c] = a1;
+  D2[c] = a2;
+}
+
+// separable z, y, x passes (0.25, 0.5, 0.25), each output evaluated with the
+// reference's intermediate values recomputed in the same order
+__device__ inline float sep_z(const float* __restrict__ S, uint32_t q)
+{
+  return 0.25f * S[q - 1] + 0.5f * S[q] + 0.25f * S[q + 1];
+}
+
+__global__ void __launch_bounds__(BX* BY)
+  k_filter_sep(Geo g, Region r, const float* __restrict__ S0, const float* __restrict__ S1,
+               const float* __restrict__ S2, float* __restrict__ D0, float* __restrict__ D1,
+               float* __restrict__ D2)
+{
+  const int k = r.r0[2] + int(blockIdx.x) * BX + int(threadIdx.x);
+  const int j = r.r0[1] + int(blockIdx.y) * BY + int(threadIdx.y);
+  const int i = r.r0[0] + int(blockIdx.z);
+  if(k >= r.r1[2] || j >= r.r1[1]) return;
+  const float* S[3] = {S0, S1, S2};
+  float* D[3] = {D0, D1, D2};
+#pragma unroll
+  for(int c = 0; c < 3; ++c) {
+    float t2[3];
+#pragma unroll
+    for(int a = 0; a < 3; ++a) {
+      const float y0 = sep_z(S[c], g.at(i + a - 1, j - 1, k));
+      const float y1 = sep_z(S[c], g.at(i + a - 1, j, k));
+      const float y2 = sep_z(S[c], g.at(i + a - 1, j + 1, k));
+      t2[a] = 0.25f * y0 + 0.5f * y1 + 0.25f * y2;
+    }
+    D[c][g.at(i, j, k)] = 0.25f * t2[0] + 0.5f * t2[1] + 0.25f * t2[2];
+  }
+}
+
+__device__ inline int wrap_idx(int c, int N) { return c < HALO ? c + N : (c >= N + HALO ? c - N : c); }
+
+// periodic ghost copy: every ghost node takes the value of its interior image
+__global__ void __launch_bounds__(256) k_ghost_copy(Geo g, float* F0, float* F1, float* F2)
+{
+  const int j = int(blockIdx.x);
+  const int i = int(blockIdx.y);
+  const bool gi = (i < HALO || i >= g.N[0] + HALO);
+  const bool gj = (j < HALO || j >= g.N[1] + HALO);
+  const int Lz = g.L[2];
+  const int si = wrap_idx(i, g.N[0]), sj = wrap_idx(j, g.N[1]);
+  if(gi || gj) {
+    for(int k = int(threadIdx.x); k < Lz; k += int(blockDim.x)) {
+      const uint32_t d = g.at(i, j, k), s = g.at(si, sj, wrap_idx(k, g.N[2]));
+      F0[d] = F0[s];
+      F1[d] = F1[s];
+      F2[d] = F2[s];
+    }
+  } else {
+    const int t = int(threadIdx.x);
+    if(t < 2 * HALO) {
+      const int k = t < HALO ? t : g.N[2] + t;
+      const uint32_t d = g.at(i, j, k), s = g.at(i, j, wrap_idx(k, g.N[2]));
+      F0[d] = F0[s];
+      F1[d] = F1[s];
+      F2[d] = F2[s];
+    }
+  }
+}
+
+// images of an interior coordinate c in the ghost shell: c-N (c >= N) and c+N (c < 2*HALO)
+__device__ inline int images(int c, int N, int out[3])
+{
+  int m = 0;
+  out[m++] = c;
+  if(c >= N) out[m++] = c - N;
+  if(c < 2 * HALO) out[m++] = c + N;
+  return m;
+}
+
+// periodic ghost fold (J_exchange): interior node += all of its ghost images
+__global__ void __launch_bounds__(256) k_ghost_fold(Geo g, float* F0, float* F1, float* F2)
+{
+  const int j = HALO + int(blockIdx.x);
+  const int i = HALO + int(blockIdx.y);
+  int ii[3], jj[3], kk[3];
+  const int mi = images(i, g.N[0], ii);
+  const int mj = images(j, g.N[1], jj);
+  auto fold = [&](int k) {
+    const int mk = images(k, g.N[2], kk);
+    if(mi * mj * mk == 1) return;
+    const uint32_t c = g.at(i, j, k);
+    float s0 = F0[c], s1 = F1[c], s2 = F2[c];
+    for(int a = 0; a < mi; ++a)
+      for(int b = 0; b < mj; ++b)
+        for(int e = 0; e < mk; ++e) {
+          if(a == 0 && b == 0 && e == 0) continue;
+          const uint32_t q = g.at(ii[a], jj[b], kk[e]);
+          s0 = s0 + F0[q];
+          s1 = s1 + F1[q];
+          s2 = s2 + F2[q];
+        }
+    F0[c] = s0;
+    F1[c] = s1;
+    F2[c] = s2;
+  };
+  if(mi * mj > 1) {
+    for(int k = HALO + int(threadIdx.x); k < g.N[2] + HALO; k += int(blockDim.x)) fold(k);
+  } else {
+    const int t = int(threadIdx.x);
+    if(t < 2 * HALO) {
+      const int k = t < HALO ? HALO + t : g.N[2] + t - HALO;
+      if(k >= HALO && k < g.N[2] + HALO && (t < HALO || k >= HALO + HALO)) fold(k);
+    }
+  }
+}
+
+__global__ void __launch_bounds__(256) k_fill(float* F0, float* F1, float* F2, int64_t n, float v)
+{
+  for(int64_t q = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; q < n; q += int64_t(gridDim.x) * blockDim.x) {
+    F0[q] = v;
+    F1[q] = v;
+    F2[q] = v;
+  }
+}
+
+// FP64 partial sums of (Fx^2 + Fy^2) + Fz^2 (float per node, as the reference)
+__global__ void __launch_bounds__(256)
+  k_energy(Geo g, Region r, const float* Ex, const float* Ey, const float* Ez, const float* Bx,
+           const float* By, const float* Bz, double* partial)
+{
+  const int i = r.r0[0] + int(blockIdx.y);
+  double se = 0.0, sb = 0.0;
+  const int ey = r.r1[1] - r.r0[1], ez = r.r1[2] - r.r0[2];
+  for(int q = int(threadIdx.x); q < ey * ez; q += int(blockDim.x)) {
+    const int j = r.r0[1] + q / ez, k = r.r0[2] + q % ez;
+    const uint32_t c = g.at(i, j, k);
+    const float e2 = Ex[c] * Ex[c] + Ey[c] * Ey[c] + Ez[c] * Ez[c];
+    const float b2 = Bx[c] * Bx[c] + By[c] * By[c] + Bz[c] * Bz[c];
+    se += double(e2);
+    sb += double(b2);
+  }
+  __shared__ double red[2][256];
+  red[0][threadIdx.x] = se;
+  red[1][threadIdx.x] = sb;
+  __syncthreads();
+  for(int o = 128; o > 0; o >>= 1) {
+    if(int(threadIdx.x) < o) {
+      red[0][threadIdx.x] += red[0][threadIdx.x + o];
+      red[1][threadIdx.x] += red[1][threadIdx.x + o];
+    }
+    __syncthreads();
+  }
+  if(threadIdx.x == 0) {
+    partial[2 * blockIdx.y] = red[0][0];
+    partial[2 * blockIdx.y + 1] = red[1][0];
+  }
+}
+
+Region make_region(const int r0[3], const int r1[3])
+{
+  Region r;
+  for(int d = 0; d < 3; ++d) {
+    r.r0[d] = r0[d];
+    r.r1[d] = r1[d];
+  }
+  return r;
+}
+
+bool empty(const Region& r) { return r.r1[0] <= r.r0[0] || r.r1[1] <= r.r0[1] || r.r1[2] <= r.r0[2]; }
+
+}  // namespace
+
+int launch_curl_b(rk_world* w, const int r0[3], const int r1[3], float dt, hipStream_t s)
+{
+  const Region r = make_region(r0, r1);
+  if(empty(r)) return RK_OK;
+  hipLaunchKernelGGL(k_curl_b, region_grid(r), dim3(BX, BY), 0, s, w->g, r, dt, w->E[0], w->E[1], w->E[2],
+                     w->B[0], w->B[1], w->B[2]);
+  RK_HIP(hipGetLastError());
+  return RK_OK;
+}
+
+int launch_curl_e(rk_world* w, const int r0[3], const int r1[3], float dt, bool add_j, hipStream_t s)
+{
+  const Region r = make_region(r0, r1);
+  if(empty(r)) return RK_OK;
+  if(add_j)
+    hipLaunchKernelGGL(k_curl_e<true>, region_grid(r), dim3(BX, BY), 0, s, w->g, r, dt, w->B[0], w->B[1],
+                       w->B[2], w->E[0], w->E[1], w->E[2], w->J[0], w->J[1], w->J[2]);
+  else
+    hipLaunchKernelGGL(k_curl_e<false>, region_grid(r), dim3(BX, BY), 0, s, w->g, r, dt, w->B[0], w->B[1],
+                       w->B[2], w->E[0], w->E[1], w->E[2], w->J[0], w->J[1], w->J[2]);
+  RK_HIP(hipGetLastError());
+  return RK_OK;
+}
+
+int launch_add_current(rk_world* w, const int r0[3], const int r1[3], hipStream_t s)
+{
+  const Region r = make_region(r0, r1);
+  if(empty(r)) return RK_OK;
+  hipLaunchKernelGGL(k_add_current, region_grid(r), dim3(BX, BY), 0, s, w->g, r, w->E[0], w->E[1], w->E[2],
+                     w->J[0], w->J[1], w->J[2]);
+  RK_HIP(hipGetLastError());
+  return RK_OK;
+}
+
+int launch_filter(rk_world* w, const int r0[3], const int r1[3], hipStream_t s)
+{
+  const Region r = make_region(r0, r1);
+  if(empty(r)) return RK_OK;
+  if(w->d.current_filter == 0)
+    hipLaunchKernelGGL(k_filter27, region_grid(r), dim3(BX, BY), 0, s, w->g, r, w->J[0], w->J[1], w->J[2],
+                       w->Jf[0], w->Jf[1], w->Jf[2]);
+  else
+    hipLaunchKernelGGL(k_filter_sep, region_grid(r), dim3(BX, BY), 0, s, w->g, r, w->J[0], w->J[1], w->J[2],
+                       w->Jf[0], w->Jf[1], w->Jf[2]);
+  RK_HIP(hipGetLastError());
+  return RK_OK;
+}
+
+int launch_ghost_copy(rk_world* w, float* const F[3], hipStream_t s)
+{
+  hipLaunchKernelGGL(k_ghost_copy, dim3(w->g.L[1], w->g.L[0]), dim3(64), 0, s, w->g, F[0], F[1], F[2]);
+  RK_HIP(hipGetLastError());
+  return RK_OK;
+}
+
+int launch_ghost_fold(rk_world* w, float* const F[3], hipStream_t s)
+{
+  hipLaunchKernelGGL(k_ghost_fold, dim3(w->g.N[1], w->g.N[0]), dim3(64), 0, s, w->g, F[0], F[1], F[2]);
+  RK_HIP(hipGetLastError());
+  return RK_OK;
+}
+
+int launch_fill(float* const F[3], int64_t n, float v, hipStream_t s)
+{
+  const int64_t blocks = std::min<int64_t>((n + 255) / 256, 8192);
+  hipLaunchKernelGGL(k_fill, dim3(unsigned(blocks)), dim3(256), 0, s, F[0], F[1], F[2], n, v);
+  RK_HIP(hipGetLastError());
+  return RK_OK;
+}
+
+int field_energies(rk_world* w, const int r0[3], const int r1[3], double out[2], hipStream_t s)
+{
+  const Region r = make_region(r0, r1);
+  out[0] = out[1] = 0;
+  if(empty(r)) return RK_OK;
+  const int planes = r.r1[0] - r.r0[0];
+  if(w->red_cap < 2 * planes) return (set_error("reduction scratch too small"), RK_EINVAL);
+  hipLaunchKernelGGL(k_energy, dim3(1, planes), dim3(256), 0, s, w->g, r, w->E[0], w->E[1], w->E[2], w->B[0],
+                     w->B[1], w->B[2], w->red);
+  RK_HIP(hipGetLastError());
+  std::vector<double> h(size_t(2 * planes));
+  RK_HIP(hipMemcpyAsync(h.data(), w->red, sizeof(double) * h.size(), hipMemcpyDeviceToHost, s));
+  RK_HIP(hipStreamSynchronize(s));
+  for(int p = 0; p < planes; ++p) {
+    out[0] += h[size_t(2 * p)];
+    out[1] += h[size_t(2 * p + 1)];
+  }
+  out[0] /= 2.0;
+  out[1] /= 2.0;
+  return RK_OK;
+}
+
+}  // namespace rk

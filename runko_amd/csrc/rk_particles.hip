@@ -1,0 +1,1262 @@
+// Particle kernels: interpolation + Boris push, zigzag current deposit,
+// tile/brick classification with compaction, migration append with the
+// periodic wrap, in-bin cell sort, injection, readback, FP64 kinetic energy,
+// and the fused batched particle step.
+//
+// Reference semantics followed (hel-astro-lab/runko v5.0.0):
+//   interpolation  src/runko/emf/yee_lattice_interpolate_linear_1st.h:29-138
+//   Boris          src/runko/pic/particle_boris.h:15-62
+//   zigzag         src/runko/pic/particle_current_zigzag_1st.c++:225-339
+//   divide         src/runko/pic/particle.c++:198-348
+//   append/wrap    src/runko/pic/particle.h:449-572
+//   sort key       src/runko/pic/tile.c++:417-438
+//   kinetic energy src/runko/pic/particle.c++:351-377
+//
+// Execution model: one workgroup (256 lanes, 4 waves) per particle bin
+// (brick).  The brick's node-centred E/B means (brick + 1 node) are staged in
+// LDS once; every particle of the brick gathers its 8 corners from LDS.  The
+// brick's current is accumulated with LDS float atomics over brick + (1,2)
+// nodes and flushed once to HBM with global float atomics.  A particle whose
+// stencil leaves the staged window (possible only on the plain per-tile path,
+// e.g. two pushes without a pack) takes the same arithmetic through global
+// memory.  Leavers are written to SoA migration records reserved with one
+// atomic per wave (ballot + popcount) and their slots are refilled from the
+// bin tail (swap-remove) so bins stay compact.
+#include "rk_internal.h"
+
+#include <algorithm>
+
+namespace rk {
+
+namespace {
+
+constexpr int WG = 256;
+constexpr int NWAVE = WG / 64;
+
+struct SpParams {
+  float* pos[RK_MAX_SPECIES][3];
+  float* vel[RK_MAX_SPECIES][3];
+  uint64_t* id[RK_MAX_SPECIES];
+  const int64_t* start[RK_MAX_SPECIES];
+  const uint32_t* cap[RK_MAX_SPECIES];
+  uint32_t* count[RK_MAX_SPECIES];
+  float qm[RK_MAX_SPECIES];
+  float charge[RK_MAX_SPECIES];
+  int nsp;
+};
+
+struct FieldPtrs {
+  const float* E[3];
+  const float* B[3];
+  float* J[3];
+};
+
+struct MigPtrs {
+  float* pos[2][3];
+  float* vel[2][3];
+  uint64_t* id[2];
+  uint32_t* meta[2];
+  int64_t cap[2];
+};
+
+SpParams make_sp(const rk_world* w)
+{
+  SpParams p {};
+  p.nsp = w->d.n_species;
+  for(int s = 0; s < p.nsp; ++s) {
+    for(int d = 0; d < 3; ++d) {
+      p.pos[s][d] = w->sp[s].pos[d];
+      p.vel[s][d] = w->sp[s].vel[d];
+    }
+    p.id[s] = w->sp[s].id;
+    p.start[s] = w->sp[s].bin_start;
+    p.cap[s] = w->sp[s].bin_cap;
+    p.count[s] = w->sp[s].bin_count;
+    p.qm[s] = w->sp[s].qm;
+    p.charge[s] = w->sp[s].charge;
+  }
+  return p;
+}
+
+FieldPtrs make_fields(const rk_world* w)
+{
+  FieldPtrs f {};
+  for(int d = 0; d < 3; ++d) {
+    f.E[d] = w->E[d];
+    f.B[d] = w->B[d];
+    f.J[d] = w->J[d];
+  }
+  return f;
+}
+
+MigPtrs make_mig(const rk_world* w)
+{
+  MigPtrs m {};
+  for(int s = 0; s < 2; ++s) {
+    for(int d = 0; d < 3; ++d) {
+      m.pos[s][d] = w->mig[s].pos[d];
+      m.vel[s][d] = w->mig[s].vel[d];
+    }
+    m.id[s] = w->mig[s].id;
+    m.meta[s] = w->mig[s].meta;
+    m.cap[s] = w->mig[s].capacity;
+  }
+  return m;
+}
+
+// ---------------------------------------------------------------------------
+// brick geometry
+// ---------------------------------------------------------------------------
+struct Brick {
+  int bin;      // brick index (bin index inside a species pool)
+  int tile;
+  int tc[3];    // tile coords in the world
+  int lb[3];    // brick coords in the tile
+  int toffh[3]; // world haloed index of the tile's haloed index 0
+  int c0[3];    // world haloed index of the brick's first cell
+  float tmin[3], tmax[3], origo[3];
+};
+
+__device__ inline Brick brick_of(const Geo& g, int b)
+{
+  Brick B;
+  B.bin = b;
+  g.brick_coords(b, B.tile, B.lb);
+  g.tile_coords(B.tile, B.tc);
+#pragma unroll
+  for(int d = 0; d < 3; ++d) {
+    B.toffh[d] = B.tc[d] * g.n[d];
+    B.c0[d] = B.toffh[d] + HALO + B.lb[d] * g.br[d];
+    B.tmin[d] = g.tile_min(B.tc[d], d);
+    B.tmax[d] = float((g.toff[d] + B.tc[d] + 1) * g.n[d]);
+    // static_cast<float>(mins) - halo_size, src/runko/pic/tile.c++:329-332
+    B.origo[d] = B.tmin[d] - 3.0f;
+  }
+  return B;
+}
+
+// brick (inside tile with coords tc) of a position; cells clamped to the tile
+__device__ inline int brick_in_tile(const Geo& g, int tile, const float origo[3], const float p[3])
+{
+  int lb[3];
+#pragma unroll
+  for(int d = 0; d < 3; ++d) {
+    int c = int(floorf(p[d] - origo[d])) - HALO;
+    c = c < 0 ? 0 : (c >= g.n[d] ? g.n[d] - 1 : c);
+    lb[d] = c / g.br[d];
+  }
+  return tile * g.bpt + (lb[0] * g.nb[1] + lb[1]) * g.nb[2] + lb[2];
+}
+
+// XCD-aware block -> brick map: consecutive bricks (which share halo lines in
+// L2) go to the same XCD (blocks are dealt round-robin over 8 XCDs).  The grid
+// has 8*ceil(n/8) blocks; blocks mapped past n exit.
+__device__ inline int xcd_brick(int blk, int nblk)
+{
+  const int per = (nblk + 7) / 8;
+  const int b = (blk % 8) * per + blk / 8;
+  return b;
+}
+
+// ---------------------------------------------------------------------------
+// node-centred field means, interpolation_linear_1st :91-123
+// mean(a,b) = (a+b)/2 ; mean(a,b,c,d) = (a+(b+(c+d)))/4  (right fold)
+// ---------------------------------------------------------------------------
+struct Means {
+  float ex, ey, ez, bx, by, bz;
+};
+
+__device__ inline Means node_means(const Geo& g, const FieldPtrs& f, int I, int J, int K)
+{
+  const uint32_t c = g.at(I, J, K);
+  const uint32_t si = uint32_t(g.L[1]) * uint32_t(g.L[2]), sj = uint32_t(g.L[2]);
+  Means m;
+  m.ex = (f.E[0][c - si] + f.E[0][c]) * 0.5f;
+  m.ey = (f.E[1][c - sj] + f.E[1][c]) * 0.5f;
+  m.ez = (f.E[2][c - 1] + f.E[2][c]) * 0.5f;
+  m.bx = (f.B[0][c] + (f.B[0][c - sj] + (f.B[0][c - 1] + f.B[0][c - sj - 1]))) * 0.25f;
+  m.by = (f.B[1][c] + (f.B[1][c - si] + (f.B[1][c - 1] + f.B[1][c - si - 1]))) * 0.25f;
+  m.bz = (f.B[2][c] + (f.B[2][c - si] + (f.B[2][c - sj] + f.B[2][c - si - sj]))) * 0.25f;
+  return m;
+}
+
+__device__ inline float lerpf(float x, float A, float B) { return (1.0f - x) * A + x * B; }
+
+// lerp3D: x first, then y, then z; v[ic*4 + jc*2 + kc]
+__device__ inline float lerp3(const float d[3], const float v[8])
+{
+  const float c00 = lerpf(d[0], v[0], v[4]);
+  const float c01 = lerpf(d[0], v[1], v[5]);
+  const float c10 = lerpf(d[0], v[2], v[6]);
+  const float c11 = lerpf(d[0], v[3], v[7]);
+  const float c0 = lerpf(d[1], c00, c10);
+  const float c1 = lerpf(d[1], c01, c11);
+  return lerpf(d[2], c0, c1);
+}
+
+// LDS window of means: M[d] nodes starting at world haloed index m0[d]
+struct MeanWin {
+  int m0[3];
+  int M[3];
+  __device__ int vol() const { return M[0] * M[1] * M[2]; }
+};
+
+__device__ inline void stage_means(const Geo& g, const FieldPtrs& f, const MeanWin& W, float* lds)
+{
+  const int V = W.vol();
+  for(int q = int(threadIdx.x); q < V; q += WG) {
+    const int lk = q % W.M[2], lj = (q / W.M[2]) % W.M[1], li = q / (W.M[2] * W.M[1]);
+    const Means m = node_means(g, f, W.m0[0] + li, W.m0[1] + lj, W.m0[2] + lk);
+    lds[0 * V + q] = m.ex;
+    lds[1 * V + q] = m.ey;
+    lds[2 * V + q] = m.ez;
+    lds[3 * V + q] = m.bx;
+    lds[4 * V + q] = m.by;
+    lds[5 * V + q] = m.bz;
+  }
+}
+
+// Interpolated E and B at a particle.  I,J,K: world haloed index of the
+// particle's cell; d: offsets inside the cell.
+__device__ inline void gather(const Geo& g, const FieldPtrs& f, const MeanWin& W, const float* lds, int I, int J,
+                              int K, const float d[3], float E[3], float B[3], uint32_t* err)
+{
+  const int li = I - W.m0[0], lj = J - W.m0[1], lk = K - W.m0[2];
+  float v[6][8];
+  if(li >= 0 && lj >= 0 && lk >= 0 && li + 1 < W.M[0] && lj + 1 < W.M[1] && lk + 1 < W.M[2]) {
+    const int V = W.vol();
+#pragma unroll
+    for(int c = 0; c < 8; ++c) {
+      const int q = ((li + (c >> 2)) * W.M[1] + (lj + ((c >> 1) & 1))) * W.M[2] + (lk + (c & 1));
+#pragma unroll
+      for(int comp = 0; comp < 6; ++comp) v[comp][c] = lds[comp * V + q];
+    }
+  } else {
+    // outside the staged window: same arithmetic from HBM (clamped to the lattice)
+    int I0 = I, J0 = J, K0 = K;
+    if(I0 < 1 || J0 < 1 || K0 < 1 || I0 + 1 > g.L[0] - 1 || J0 + 1 > g.L[1] - 1 || K0 + 1 > g.L[2] - 1) {
+      atomicAdd(err, 1u);
+      I0 = min(max(I0, 1), g.L[0] - 2);
+      J0 = min(max(J0, 1), g.L[1] - 2);
+      K0 = min(max(K0, 1), g.L[2] - 2);
+    }
+#pragma unroll
+    for(int c = 0; c < 8; ++c) {
+      const Means m = node_means(g, f, I0 + (c >> 2), J0 + ((c >> 1) & 1), K0 + (c & 1));
+      v[0][c] = m.ex;
+      v[1][c] = m.ey;
+      v[2][c] = m.ez;
+      v[3][c] = m.bx;
+      v[4][c] = m.by;
+      v[5][c] = m.bz;
+    }
+  }
+  E[0] = lerp3(d, v[0]);
+  E[1] = lerp3(d, v[1]);
+  E[2] = lerp3(d, v[2]);
+  B[0] = lerp3(d, v[3]);
+  B[1] = lerp3(d, v[4]);
+  B[2] = lerp3(d, v[5]);
+}
+
+__device__ inline float dot3(const float a[3], const float b[3])
+{
+  float r = a[0] * b[0];
+  r = r + a[1] * b[1];
+  r = r + a[2] * b[2];
+  return r;
+}
+
+// particle_boris.h:39-59, expression order as written there
+__device__ inline void boris(float cfl, float qm, const float E[3], const float B[3], float pos[3], float vel[3])
+{
+  const float hq = 0.5f * qm;
+  float u0[3], E0[3];
+#pragma unroll
+  for(int c = 0; c < 3; ++c) {
+    E0[c] = E[c] * hq;
+    u0[c] = vel[c] * cfl + E0[c];
+  }
+  const float ginv = cfl / sqrtf(cfl * cfl + dot3(u0, u0));
+  const float bfac = hq * ginv;
+  float B0[3];
+#pragma unroll
+  for(int c = 0; c < 3; ++c) B0[c] = (B[c] * bfac) / cfl;
+  const float f = 2.0f / (1.0f + dot3(B0, B0));
+  float u1[3];
+  u1[0] = (u0[0] + (u0[1] * B0[2] - u0[2] * B0[1])) * f;
+  u1[1] = (u0[1] + (-u0[0] * B0[2] + u0[2] * B0[0])) * f;
+  u1[2] = (u0[2] + (u0[0] * B0[1] - u0[1] * B0[0])) * f;
+  float u2[3];
+  u2[0] = (u0[0] + (u1[1] * B0[2] - u1[2] * B0[1])) + E0[0];
+  u2[1] = (u0[1] + (-u1[0] * B0[2] + u1[2] * B0[0])) + E0[1];
+  u2[2] = (u0[2] + (u1[0] * B0[1] - u1[1] * B0[0])) + E0[2];
+#pragma unroll
+  for(int c = 0; c < 3; ++c) vel[c] = u2[c] / cfl;
+  const float ginv2 = cfl / sqrtf(cfl * cfl + dot3(u2, u2));
+#pragma unroll
+  for(int c = 0; c < 3; ++c) pos[c] = pos[c] + (vel[c] * ginv2) * cfl;
+}
+
+// ---------------------------------------------------------------------------
+// zigzag deposit.  x2: particle position relative to the lattice origo of the
+// tile that owns it after migration.  Store(i, j, k, mask, jx, jy, jz) receives
+// tile-haloed node indices; mask bit c set when component c is structurally
+// non-zero.  particle_current_zigzag_1st.c++:249-336
+// ---------------------------------------------------------------------------
+struct Zig {
+  int i1[3], i2[3];
+  float Fx1, Fy1, Fz1, Fx2, Fy2, Fz2;
+  float Wx1, Wy1, Wz1, Wx2, Wy2, Wz2;
+};
+
+__device__ inline Zig zigzag_setup(const float x2[3], const float u[3], float charge, float cfl)
+{
+  Zig z;
+  const float invgam = 1.0f / sqrtf(1.0f + dot3(u, u));
+  const float s = cfl * invgam;
+  float x1[3], fi1[3], fi2[3], xr[3], F1[3], F2[3], W1[3], W2[3];
+#pragma unroll
+  for(int d = 0; d < 3; ++d) {
+    x1[d] = x2[d] - u[d] * s;
+    fi1[d] = floorf(x1[d]);
+    fi2[d] = floorf(x2[d]);
+    const float a = (fi1[d] < fi2[d] ? fi1[d] : fi2[d]) + 1.0f;
+    const float b1 = (fi1[d] > fi2[d] ? fi1[d] : fi2[d]);
+    const float b2 = 0.5f * (x1[d] + x2[d]);
+    const float b = (b1 > b2 ? b1 : b2);
+    xr[d] = (a < b ? a : b);
+    F1[d] = (xr[d] - x1[d]) * charge;
+    F2[d] = (x2[d] - xr[d]) * charge;
+    W1[d] = (x1[d] + xr[d]) * 0.5f - fi1[d];
+    W2[d] = (x2[d] + xr[d]) * 0.5f - fi2[d];
+    z.i1[d] = int(fi1[d]);
+    z.i2[d] = int(fi2[d]);
+  }
+  z.Fx1 = F1[0]; z.Fy1 = F1[1]; z.Fz1 = F1[2];
+  z.Fx2 = F2[0]; z.Fy2 = F2[1]; z.Fz2 = F2[2];
+  z.Wx1 = W1[0]; z.Wy1 = W1[1]; z.Wz1 = W1[2];
+  z.Wx2 = W2[0]; z.Wy2 = W2[1]; z.Wz2 = W2[2];
+  return z;
+}
+
+template<class Store>
+__device__ inline void zigzag_emit(const Zig& z, Store&& st)
+{
+  const float one = 1.0f;
+  const int* a = z.i1;
+  const int* b = z.i2;
+  st(a[0], a[1], a[2], 7, z.Fx1 * (one - z.Wy1) * (one - z.Wz1), z.Fy1 * (one - z.Wx1) * (one - z.Wz1),
+     z.Fz1 * (one - z.Wx1) * (one - z.Wy1));
+  st(b[0], b[1], b[2], 7, z.Fx2 * (one - z.Wy2) * (one - z.Wz2), z.Fy2 * (one - z.Wx2) * (one - z.Wz2),
+     z.Fz2 * (one - z.Wx2) * (one - z.Wy2));
+  st(a[0] + 1, a[1], a[2], 6, 0.0f, z.Fy1 * z.Wx1 * (one - z.Wz1), z.Fz1 * z.Wx1 * (one - z.Wy1));
+  st(b[0] + 1, b[1], b[2], 6, 0.0f, z.Fy2 * z.Wx2 * (one - z.Wz2), z.Fz2 * z.Wx2 * (one - z.Wy2));
+  st(a[0], a[1] + 1, a[2], 5, z.Fx1 * z.Wy1 * (one - z.Wz1), 0.0f, z.Fz1 * (one - z.Wx1) * z.Wy1);
+  st(b[0], b[1] + 1, b[2], 5, z.Fx2 * z.Wy2 * (one - z.Wz2), 0.0f, z.Fz2 * (one - z.Wx2) * z.Wy2);
+  st(a[0], a[1], a[2] + 1, 3, z.Fx1 * (one - z.Wy1) * z.Wz1, z.Fy1 * (one - z.Wx1) * z.Wz1, 0.0f);
+  st(b[0], b[1], b[2] + 1, 3, z.Fx2 * (one - z.Wy2) * z.Wz2, z.Fy2 * (one - z.Wx2) * z.Wz2, 0.0f);
+  st(a[0], a[1] + 1, a[2] + 1, 1, z.Fx1 * z.Wy1 * z.Wz1, 0.0f, 0.0f);
+  st(b[0], b[1] + 1, b[2] + 1, 1, z.Fx2 * z.Wy2 * z.Wz2, 0.0f, 0.0f);
+  st(a[0] + 1, a[1], a[2] + 1, 2, 0.0f, z.Fy1 * z.Wx1 * z.Wz1, 0.0f);
+  st(b[0] + 1, b[1], b[2] + 1, 2, 0.0f, z.Fy2 * z.Wx2 * z.Wz2, 0.0f);
+  st(a[0] + 1, a[1] + 1, a[2], 4, 0.0f, 0.0f, z.Fz1 * z.Wx1 * z.Wy1);
+  st(b[0] + 1, b[1] + 1, b[2], 4, 0.0f, 0.0f, z.Fz2 * z.Wx2 * z.Wy2);
+}
+
+// LDS current window: D[d] nodes starting at world haloed index d0[d]
+struct CurWin {
+  int d0[3];
+  int D[3];
+  __device__ int vol() const { return D[0] * D[1] * D[2]; }
+};
+
+__device__ inline void lds_add(float* p, float v)
+{
+  __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+// Deposit one particle.  toffh: world haloed index of the owning tile's
+// haloed 0; shift: periodic image shift (world cells) applied to fit the window.
+__device__ inline void deposit_particle(const Geo& g, const FieldPtrs& f, const CurWin& C, float* ldsJ, const Zig& z,
+                                        const int toffh[3], uint32_t* err)
+{
+  int base1[3], base2[3];
+  bool in_win = true;
+#pragma unroll
+  for(int d = 0; d < 3; ++d) {
+    int l1 = toffh[d] + z.i1[d] - C.d0[d];
+    int l2 = toffh[d] + z.i2[d] - C.d0[d];
+    if(g.periodic) {
+      // a particle that wrapped around the world is deposited at the periodic
+      // image nearest to this brick (its ghost image is folded later)
+      if(l2 < -1) {
+        l1 += g.N[d];
+        l2 += g.N[d];
+      } else if(l2 > C.D[d]) {
+        l1 -= g.N[d];
+        l2 -= g.N[d];
+      }
+    }
+    base1[d] = l1;
+    base2[d] = l2;
+    in_win = in_win && l1 >= 0 && l2 >= 0 && l1 + 1 < C.D[d] && l2 + 1 < C.D[d];
+  }
+  if(in_win) {
+    const int V = C.vol();
+    // translate tile-haloed node indices into LDS offsets
+    const int o1[3] = {base1[0] - z.i1[0], base1[1] - z.i1[1], base1[2] - z.i1[2]};
+    (void)base2;
+    zigzag_emit(z, [&](int i, int j, int k, int mask, float jx, float jy, float jz) {
+      const int q = ((i + o1[0]) * C.D[1] + (j + o1[1])) * C.D[2] + (k + o1[2]);
+      if(mask & 1) lds_add(&ldsJ[q], jx);
+      if(mask & 2) lds_add(&ldsJ[V + q], jy);
+      if(mask & 4) lds_add(&ldsJ[2 * V + q], jz);
+    });
+  } else {
+    // straight to HBM with global atomics (same contributions)
+    zigzag_emit(z, [&](int i, int j, int k, int mask, float jx, float jy, float jz) {
+      int I = toffh[0] + i, J = toffh[1] + j, K = toffh[2] + k;
+      if(I < 0 || J < 0 || K < 0 || I >= g.L[0] || J >= g.L[1] || K >= g.L[2]) {
+        atomicAdd(err, 1u);
+        return;
+      }
+      const uint32_t q = g.at(I, J, K);
+      if(mask & 1) unsafeAtomicAdd(&f.J[0][q], jx);
+      if(mask & 2) unsafeAtomicAdd(&f.J[1][q], jy);
+      if(mask & 4) unsafeAtomicAdd(&f.J[2][q], jz);
+    });
+  }
+}
+
+__device__ inline void flush_current(const Geo& g, const FieldPtrs& f, const CurWin& C, const float* ldsJ)
+{
+  const int V = C.vol();
+  for(int q = int(threadIdx.x); q < V; q += WG) {
+    const int lk = q % C.D[2], lj = (q / C.D[2]) % C.D[1], li = q / (C.D[2] * C.D[1]);
+    const uint32_t c = g.at(C.d0[0] + li, C.d0[1] + lj, C.d0[2] + lk);
+#pragma unroll
+    for(int comp = 0; comp < 3; ++comp) {
+      const float v = ldsJ[comp * V + q];
+      if(v != 0.0f) unsafeAtomicAdd(&f.J[comp][c], v);
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// block-wide helpers
+// ---------------------------------------------------------------------------
+__device__ inline int lane_id() { return int(threadIdx.x) & 63; }
+__device__ inline int wave_id() { return int(threadIdx.x) >> 6; }
+
+__device__ inline uint64_t lanemask_lt() { return (uint64_t(1) << lane_id()) - 1; }
+
+// exclusive block scan of a 0/1 flag; returns rank, writes total
+__device__ inline int block_rank(bool flag, int* wcnt, int& total)
+{
+  const uint64_t bal = __ballot(flag);
+  if(lane_id() == 0) wcnt[wave_id()] = __popcll(bal);
+  __syncthreads();
+  int off = 0, tot = 0;
+#pragma unroll
+  for(int wv = 0; wv < NWAVE; ++wv) {
+    const int c = wcnt[wv];
+    off += (wv < wave_id()) ? c : 0;
+    tot += c;
+  }
+  __syncthreads();
+  total = tot;
+  return off + __popcll(bal & lanemask_lt());
+}
+
+// reserve one slot per flagged lane in a counter with a single atomic per wave
+__device__ inline uint32_t wave_reserve(bool flag, uint32_t* counter)
+{
+  const uint64_t bal = __ballot(flag);
+  if(bal == 0) return 0;
+  const int leader = __ffsll((unsigned long long)bal) - 1;
+  uint32_t base = 0;
+  if(lane_id() == leader) base = atomicAdd(counter, uint32_t(__popcll(bal)));
+  base = __shfl(base, leader);
+  return base + uint32_t(__popcll(bal & lanemask_lt()));
+}
+
+__device__ inline void write_record(const MigPtrs& m, int set, uint32_t slot, const float p[3], const float v[3],
+                                    uint64_t id, uint32_t meta)
+{
+#pragma unroll
+  for(int d = 0; d < 3; ++d) {
+    m.pos[set][d][slot] = p[d];
+    m.vel[set][d][slot] = v[d];
+  }
+  m.id[set][slot] = id;
+  m.meta[set][slot] = meta;
+}
+
+// LDS for bin compaction
+struct CompactLds {
+  uint32_t holes[MAX_HOLES];
+  uint32_t tailbits[MAX_HOLES / 32];
+  int wcnt[NWAVE];
+  int nholes;
+};
+
+// Swap-remove compaction of one bin after the leavers' slots were recorded in
+// L.holes (ascending) and their ids set to DEAD_ID.
+__device__ void compact_bin(const SpParams& sp, int s, int64_t start, int count, int nholes, CompactLds& L,
+                            uint32_t* counters)
+{
+  const int newc = count - nholes;
+  if(nholes == 0) return;
+  float* P[3] = {sp.pos[s][0], sp.pos[s][1], sp.pos[s][2]};
+  float* U[3] = {sp.vel[s][0], sp.vel[s][1], sp.vel[s][2]};
+  uint64_t* ID = sp.id[s];
+  if(nholes <= MAX_HOLES) {
+    // tail bitmap of holes at index >= newc
+    for(int q = int(threadIdx.x); q < (nholes + 31) / 32; q += WG) L.tailbits[q] = 0;
+    __syncthreads();
+    for(int q = int(threadIdx.x); q < nholes; q += WG) {
+      const uint32_t h = L.holes[q];
+      if(int(h) >= newc) atomicOr(&L.tailbits[(int(h) - newc) >> 5], 1u << ((int(h) - newc) & 31));
+    }
+    __syncthreads();
+    int moved = 0;
+    for(int base = 0; base < nholes; base += WG) {
+      const int t = base + int(threadIdx.x);
+      bool stay = false;
+      if(t < nholes) stay = !((L.tailbits[t >> 5] >> (t & 31)) & 1u);
+      int tot;
+      const int r = block_rank(stay, L.wcnt, tot);
+      if(stay) {
+        const int64_t src = start + newc + t;
+        const int64_t dst = start + int64_t(L.holes[moved + r]);
+#pragma unroll
+        for(int d = 0; d < 3; ++d) {
+          P[d][dst] = P[d][src];
+          U[d][dst] = U[d][src];
+        }
+        ID[dst] = ID[src];
+      }
+      moved += tot;
+    }
+  } else {
+    // slow path: order-preserving in-place compaction over DEAD_ID marks
+    if(threadIdx.x == 0) atomicAdd(&counters[C_SLOWCOMPACT], 1u);
+    int wr = 0;
+    for(int base = 0; base < count; base += WG) {
+      const int t = base + int(threadIdx.x);
+      bool alive = false;
+      float p[3], v[3];
+      uint64_t id = 0;
+      if(t < count) {
+        id = ID[start + t];
+        alive = id != DEAD_ID;
+        if(alive) {
+#pragma unroll
+          for(int d = 0; d < 3; ++d) {
+            p[d] = P[d][start + t];
+            v[d] = U[d][start + t];
+          }
+        }
+      }
+      int tot;
+      const int r = block_rank(alive, L.wcnt, tot);
+      if(alive) {
+        const int64_t dst = start + wr + r;
+#pragma unroll
+        for(int d = 0; d < 3; ++d) {
+          P[d][dst] = p[d];
+          U[d][dst] = v[d];
+        }
+        ID[dst] = id;
+      }
+      wr += tot;
+      __syncthreads();
+    }
+  }
+}
+
+__device__ inline int tile_dir(const float p[3], const float lo[3], const float hi[3])
+{
+  int d[3];
+#pragma unroll
+  for(int a = 0; a < 3; ++a) d[a] = int(p[a] >= lo[a]) - int(p[a] < hi[a]);
+  return 9 * (d[0] + 1) + 3 * (d[1] + 1) + (d[2] + 1);
+}
+
+__device__ inline float wrap_coord(float x, float a, float b)
+{
+  // (x < 0 ? max : min) + fmod(x, L), particle.h:523-549
+  const float Ld = b - a;
+  return (x < 0 ? b : a) + fmodf(x, Ld);
+}
+
+__device__ inline int bin_total(const SpParams& sp, int b)
+{
+  int t = 0;
+  for(int s = 0; s < sp.nsp; ++s) t += int(sp.count[s][b]);
+  return t;
+}
+
+// ---------------------------------------------------------------------------
+// kernels
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(WG)
+  k_push(Geo g, SpParams sp, FieldPtrs f, int bin0, int nbins, uint32_t* counters)
+{
+  extern __shared__ float lds[];
+  const int b = bin0 + xcd_brick(int(blockIdx.x), nbins);
+  if(b >= bin0 + nbins) return;
+  if(bin_total(sp, b) == 0) return;
+  const Brick Bk = brick_of(g, b);
+  MeanWin W;
+#pragma unroll
+  for(int d = 0; d < 3; ++d) {
+    W.m0[d] = Bk.c0[d];
+    W.M[d] = g.br[d] + 2;
+  }
+  stage_means(g, f, W, lds);
+  __syncthreads();
+  for(int s = 0; s < sp.nsp; ++s) {
+    const int64_t start = sp.start[s][b];
+    const int count = int(sp.count[s][b]);
+    const float qm = sp.qm[s];
+    for(int t = int(threadIdx.x); t < count; t += WG) {
+      const int64_t q = start + t;
+      float pos[3] = {sp.pos[s][0][q], sp.pos[s][1][q], sp.pos[s][2][q]};
+      float vel[3] = {sp.vel[s][0][q], sp.vel[s][1][q], sp.vel[s][2][q]};
+      float pr[3], dd[3];
+      int ci[3];
+#pragma unroll
+      for(int d = 0; d < 3; ++d) {
+        pr[d] = pos[d] - Bk.origo[d];
+        ci[d] = int(uint32_t(pr[d]));
+        dd[d] = pr[d] - float(uint32_t(ci[d]));
+      }
+      float E[3], Bf[3];
+      gather(g, f, W, lds, Bk.toffh[0] + ci[0], Bk.toffh[1] + ci[1], Bk.toffh[2] + ci[2], dd, E, Bf,
+             &counters[C_STENCIL]);
+      boris(g.cfl, qm, E, Bf, pos, vel);
+#pragma unroll
+      for(int d = 0; d < 3; ++d) {
+        sp.pos[s][d][q] = pos[d];
+        sp.vel[s][d][q] = vel[d];
+      }
+    }
+  }
+}
+
+__global__ void __launch_bounds__(WG)
+  k_deposit(Geo g, SpParams sp, FieldPtrs f, int bin0, int nbins, uint32_t* counters)
+{
+  extern __shared__ float lds[];
+  const int b = bin0 + xcd_brick(int(blockIdx.x), nbins);
+  if(b >= bin0 + nbins) return;
+  if(bin_total(sp, b) == 0) return;
+  const Brick Bk = brick_of(g, b);
+  CurWin C;
+#pragma unroll
+  for(int d = 0; d < 3; ++d) {
+    C.d0[d] = Bk.c0[d] - 1;
+    C.D[d] = g.br[d] + 3;
+  }
+  const int V = C.vol();
+  for(int q = int(threadIdx.x); q < 3 * V; q += WG) lds[q] = 0.0f;
+  __syncthreads();
+  for(int s = 0; s < sp.nsp; ++s) {
+    const int64_t start = sp.start[s][b];
+    const int count = int(sp.count[s][b]);
+    for(int t = int(threadIdx.x); t < count; t += WG) {
+      const int64_t q = start + t;
+      const float u[3] = {sp.vel[s][0][q], sp.vel[s][1][q], sp.vel[s][2][q]};
+      const float x2[3] = {sp.pos[s][0][q] - Bk.origo[0], sp.pos[s][1][q] - Bk.origo[1],
+                           sp.pos[s][2][q] - Bk.origo[2]};
+      const Zig z = zigzag_setup(x2, u, sp.charge[s], g.cfl);
+      deposit_particle(g, f, C, lds, z, Bk.toffh, &counters[C_STENCIL]);
+    }
+  }
+  __syncthreads();
+  flush_current(g, f, C, lds);
+}
+
+// pack_outgoing: tile leavers -> record set 0, brick changes -> set 1
+__global__ void __launch_bounds__(WG)
+  k_pack(Geo g, SpParams sp, MigPtrs m, int bin0, int nbins, uint32_t* counters)
+{
+  __shared__ CompactLds L;
+  const int b = bin0 + int(blockIdx.x);
+  if(b >= bin0 + nbins) return;
+  const Brick Bk = brick_of(g, b);
+  for(int s = 0; s < sp.nsp; ++s) {
+    const int64_t start = sp.start[s][b];
+    const int count = int(sp.count[s][b]);
+    if(count == 0) continue;
+    int nholes = 0;
+    for(int base = 0; base < count; base += WG) {
+      const int t = base + int(threadIdx.x);
+      bool lv0 = false, lv1 = false;
+      float p[3], v[3];
+      uint64_t id = 0;
+      int dir = 13;
+      if(t < count) {
+        const int64_t q = start + t;
+#pragma unroll
+        for(int d = 0; d < 3; ++d) {
+          p[d] = sp.pos[s][d][q];
+          v[d] = sp.vel[s][d][q];
+        }
+        dir = tile_dir(p, Bk.tmin, Bk.tmax);
+        if(dir != 13) lv0 = true;
+        else lv1 = brick_in_tile(g, Bk.tile, Bk.origo, p) != b;
+        if(lv0 || lv1) id = sp.id[s][q];
+      }
+      uint32_t slot0 = wave_reserve(lv0, &counters[C_MIG0]);
+      uint32_t slot1 = wave_reserve(lv1, &counters[C_MIG1]);
+      if(lv0 && int64_t(slot0) >= m.cap[0]) {
+        lv0 = false;
+        atomicAdd(&counters[C_MIGLOST], 1u);
+      }
+      if(lv1 && int64_t(slot1) >= m.cap[1]) {
+        lv1 = false;
+        atomicAdd(&counters[C_MIGLOST], 1u);
+      }
+      if(lv0) write_record(m, 0, slot0, p, v, id, meta_pack(s, 0, dir, uint32_t(b)));
+      if(lv1) write_record(m, 1, slot1, p, v, id, meta_pack(s, 1, 13, uint32_t(b)));
+      const bool hole = lv0 || lv1;
+      if(hole) sp.id[s][start + t] = DEAD_ID;
+      int tot;
+      const int r = block_rank(hole, L.wcnt, tot);
+      if(hole && nholes + r < MAX_HOLES) L.holes[nholes + r] = uint32_t(t);
+      nholes += tot;
+    }
+    __syncthreads();
+    compact_bin(sp, s, start, count, nholes, L, counters);
+    __syncthreads();
+    if(threadIdx.x == 0) sp.count[s][b] = uint32_t(count - nholes);
+    __syncthreads();
+  }
+}
+
+// append migration records of one set to their destination bins
+__global__ void __launch_bounds__(WG)
+  k_append(Geo g, SpParams sp, MigPtrs m, int set, uint32_t* counters, uint32_t* ovf, int64_t ovf_cap,
+           const uint32_t* list, int list_n)
+{
+  // list == nullptr: records [0, counters[set]) ; else: entries of list (set << 31 | index)
+  // the counter can exceed the buffer when reservations were refused (those
+  // particles stayed in their bins); never read past the record buffer
+  const int n = list ? list_n : int(min(int64_t(counters[set == 0 ? C_MIG0 : C_MIG1]), m.cap[set]));
+  for(int e = int(blockIdx.x * blockDim.x + threadIdx.x); e < n; e += int(gridDim.x * blockDim.x)) {
+    int st = set;
+    uint32_t r = uint32_t(e);
+    if(list) {
+      st = int(list[e] >> 31);
+      r = list[e] & 0x7fffffffu;
+    }
+    const uint32_t meta = m.meta[st][r];
+    const int s = meta_species(meta);
+    const int kind = meta_kind(meta);
+    const int srcb = int(meta_bin(meta));
+    int tile = srcb / g.bpt;
+    float p[3] = {m.pos[st][0][r], m.pos[st][1][r], m.pos[st][2][r]};
+    if(kind == 0) {
+      int tc[3];
+      g.tile_coords(tile, tc);
+      const int dir = meta_dir(meta);
+      const int dd[3] = {dir / 9 - 1, (dir / 3) % 3 - 1, dir % 3 - 1};
+#pragma unroll
+      for(int d = 0; d < 3; ++d) {
+        tc[d] = (tc[d] + dd[d] + g.nt[d]) % g.nt[d];
+        p[d] = wrap_coord(p[d], g.wmin[d], g.wmax[d]);
+      }
+      tile = g.tile_lin(tc[0], tc[1], tc[2]);
+    }
+    int tc[3];
+    g.tile_coords(tile, tc);
+    float origo[3];
+#pragma unroll
+    for(int d = 0; d < 3; ++d) origo[d] = g.tile_min(tc[d], d) - 3.0f;
+    const int db = brick_in_tile(g, tile, origo, p);
+    const uint32_t slot = atomicAdd(&sp.count[s][db], 1u);
+    if(slot < sp.cap[s][db]) {
+      const int64_t q = sp.start[s][db] + slot;
+#pragma unroll
+      for(int d = 0; d < 3; ++d) {
+        sp.pos[s][d][q] = p[d];
+        sp.vel[s][d][q] = m.vel[st][d][r];
+      }
+      sp.id[s][q] = m.id[st][r];
+    } else {
+      // keep the record (its position already wrapped) for the retry after a repack
+#pragma unroll
+      for(int d = 0; d < 3; ++d) m.pos[st][d][r] = p[d];
+      m.meta[st][r] = meta_pack(s, 1, 13, uint32_t(db));
+      const uint32_t o = atomicAdd(&counters[C_OVF], 1u);
+      if(int64_t(o) < ovf_cap) ovf[o] = (uint32_t(st) << 31) | r;
+      else atomicAdd(&counters[C_MIGLOST], 1u);
+    }
+  }
+}
+
+__global__ void k_reset(uint32_t* counters, int which)
+{
+  if(threadIdx.x == 0) counters[which] = 0;
+}
+
+// Fused batched step of one brick: gather + Boris + classification + zigzag
+// deposit (in the post-migration frame) + record write + compaction.
+__global__ void __launch_bounds__(WG)
+  k_fused(Geo g, SpParams sp, FieldPtrs f, MigPtrs m, int nbins, uint32_t* counters)
+{
+  extern __shared__ float lds[];
+  __shared__ CompactLds L;
+  const int b = xcd_brick(int(blockIdx.x), nbins);
+  if(b >= nbins) return;
+  if(bin_total(sp, b) == 0) return;
+  const Brick Bk = brick_of(g, b);
+  MeanWin W;
+  CurWin C;
+#pragma unroll
+  for(int d = 0; d < 3; ++d) {
+    W.m0[d] = Bk.c0[d];
+    W.M[d] = g.br[d] + 2;
+    C.d0[d] = Bk.c0[d] - 1;
+    C.D[d] = g.br[d] + 3;
+  }
+  float* ldsM = lds;
+  float* ldsJ = lds + 6 * W.vol();
+  const int V = C.vol();
+  for(int q = int(threadIdx.x); q < 3 * V; q += WG) ldsJ[q] = 0.0f;
+  stage_means(g, f, W, ldsM);
+  __syncthreads();
+  for(int s = 0; s < sp.nsp; ++s) {
+    const int64_t start = sp.start[s][b];
+    const int count = int(sp.count[s][b]);
+    if(count == 0) continue;
+    const float qm = sp.qm[s];
+    const float charge = sp.charge[s];
+    int nholes = 0;
+    for(int base = 0; base < count; base += WG) {
+      const int t = base + int(threadIdx.x);
+      bool lv0 = false, lv1 = false;
+      float pos[3], vel[3];
+      uint64_t id = 0;
+      int dir = 13;
+      int64_t q = start + t;
+      if(t < count) {
+#pragma unroll
+        for(int d = 0; d < 3; ++d) {
+          pos[d] = sp.pos[s][d][q];
+          vel[d] = sp.vel[s][d][q];
+        }
+        float pr[3], dd[3];
+        int ci[3];
+#pragma unroll
+        for(int d = 0; d < 3; ++d) {
+          pr[d] = pos[d] - Bk.origo[d];
+          ci[d] = int(uint32_t(pr[d]));
+          dd[d] = pr[d] - float(uint32_t(ci[d]));
+        }
+        float E[3], Bf[3];
+        gather(g, f, W, ldsM, Bk.toffh[0] + ci[0], Bk.toffh[1] + ci[1], Bk.toffh[2] + ci[2], dd, E, Bf,
+               &counters[C_STENCIL]);
+        boris(g.cfl, qm, E, Bf, pos, vel);
+        // classification against the tile (pack_outgoing) and the brick
+        dir = tile_dir(pos, Bk.tmin, Bk.tmax);
+        int dtile = Bk.tile;
+        int dtoffh[3] = {Bk.toffh[0], Bk.toffh[1], Bk.toffh[2]};
+        float dorigo[3] = {Bk.origo[0], Bk.origo[1], Bk.origo[2]};
+        float pw[3] = {pos[0], pos[1], pos[2]};
+        if(dir != 13) {
+          lv0 = true;
+          const int dv[3] = {dir / 9 - 1, (dir / 3) % 3 - 1, dir % 3 - 1};
+          int tc[3];
+#pragma unroll
+          for(int d = 0; d < 3; ++d) {
+            tc[d] = (Bk.tc[d] + dv[d] + g.nt[d]) % g.nt[d];
+            pw[d] = wrap_coord(pos[d], g.wmin[d], g.wmax[d]);
+            dtoffh[d] = tc[d] * g.n[d];
+            dorigo[d] = g.tile_min(tc[d], d) - 3.0f;
+          }
+          dtile = g.tile_lin(tc[0], tc[1], tc[2]);
+        } else {
+          lv1 = brick_in_tile(g, Bk.tile, Bk.origo, pos) != b;
+        }
+        (void)dtile;
+        // zigzag deposit in the frame of the tile that owns the particle after migration
+        const float x2[3] = {pw[0] - dorigo[0], pw[1] - dorigo[1], pw[2] - dorigo[2]};
+        const Zig z = zigzag_setup(x2, vel, charge, g.cfl);
+        deposit_particle(g, f, C, ldsJ, z, dtoffh, &counters[C_STENCIL]);
+        // records keep the unwrapped position; the append applies the wrap
+        if(lv0 || lv1) id = sp.id[s][q];
+      }
+      uint32_t slot0 = wave_reserve(lv0, &counters[C_MIG0]);
+      uint32_t slot1 = wave_reserve(lv1, &counters[C_MIG1]);
+      if(lv0 && int64_t(slot0) >= m.cap[0]) {
+        lv0 = false;
+        atomicAdd(&counters[C_MIGLOST], 1u);
+      }
+      if(lv1 && int64_t(slot1) >= m.cap[1]) {
+        lv1 = false;
+        atomicAdd(&counters[C_MIGLOST], 1u);
+      }
+      const bool hole = lv0 || lv1;
+      if(t < count) {
+        if(lv0) write_record(m, 0, slot0, pos, vel, id, meta_pack(s, 0, dir, uint32_t(b)));
+        if(lv1) write_record(m, 1, slot1, pos, vel, id, meta_pack(s, 1, 13, uint32_t(b)));
+        if(hole) {
+          sp.id[s][q] = DEAD_ID;
+        } else {
+#pragma unroll
+          for(int d = 0; d < 3; ++d) {
+            sp.pos[s][d][q] = pos[d];
+            sp.vel[s][d][q] = vel[d];
+          }
+        }
+      }
+      int tot;
+      const int r = block_rank(hole, L.wcnt, tot);
+      if(hole && nholes + r < MAX_HOLES) L.holes[nholes + r] = uint32_t(t);
+      nholes += tot;
+    }
+    __syncthreads();
+    compact_bin(sp, s, start, count, nholes, L, counters);
+    __syncthreads();
+    if(threadIdx.x == 0) sp.count[s][b] = uint32_t(count - nholes);
+    __syncthreads();
+  }
+  flush_current(g, f, C, ldsJ);
+}
+
+// in-bin counting sort by brick-local cell (layout_right), out-of-brick last
+__global__ void __launch_bounds__(WG)
+  k_sort(Geo g, SpParams sp, int bin0, int nbins, float* stage, int64_t stage_stride)
+{
+  extern __shared__ uint32_t cnt[];  // nbuck + 1
+  const int cells = g.br[0] * g.br[1] * g.br[2];
+  const int nbuck = cells + 1;
+  float* P = stage + int64_t(blockIdx.x) * stage_stride * 8;
+  for(int bi = int(blockIdx.x); bi < nbins; bi += int(gridDim.x)) {
+    const int b = bin0 + bi;
+    const Brick Bk = brick_of(g, b);
+    for(int s = 0; s < sp.nsp; ++s) {
+      const int64_t start = sp.start[s][b];
+      const int count = int(sp.count[s][b]);
+      if(count < 2) continue;
+      for(int q = int(threadIdx.x); q <= nbuck; q += WG) cnt[q] = 0;
+      __syncthreads();
+      auto key = [&](int64_t q) {
+        int c[3];
+        bool in = true;
+#pragma unroll
+        for(int d = 0; d < 3; ++d) {
+          c[d] = int(floorf(sp.pos[s][d][q] - Bk.origo[d])) - HALO - Bk.lb[d] * g.br[d];
+          in = in && c[d] >= 0 && c[d] < g.br[d];
+        }
+        return in ? (c[0] * g.br[1] + c[1]) * g.br[2] + c[2] : cells;
+      };
+      for(int t = int(threadIdx.x); t < count; t += WG) atomicAdd(&cnt[key(start + t)], 1u);
+      __syncthreads();
+      if(threadIdx.x == 0) {
+        uint32_t acc = 0;
+        for(int q = 0; q < nbuck; ++q) {
+          const uint32_t c = cnt[q];
+          cnt[q] = acc;
+          acc += c;
+        }
+      }
+      __syncthreads();
+      for(int t = int(threadIdx.x); t < count; t += WG) {
+        const int64_t q = start + t;
+        const uint32_t dst = atomicAdd(&cnt[key(q)], 1u);
+#pragma unroll
+        for(int d = 0; d < 3; ++d) {
+          P[int64_t(d) * stage_stride + dst] = sp.pos[s][d][q];
+          P[int64_t(3 + d) * stage_stride + dst] = sp.vel[s][d][q];
+        }
+        reinterpret_cast<uint64_t*>(P + 6 * stage_stride)[dst] = sp.id[s][q];
+      }
+      __syncthreads();
+      for(int t = int(threadIdx.x); t < count; t += WG) {
+        const int64_t q = start + t;
+#pragma unroll
+        for(int d = 0; d < 3; ++d) {
+          sp.pos[s][d][q] = P[int64_t(d) * stage_stride + t];
+          sp.vel[s][d][q] = P[int64_t(3 + d) * stage_stride + t];
+        }
+        sp.id[s][q] = reinterpret_cast<const uint64_t*>(P + 6 * stage_stride)[t];
+      }
+      __syncthreads();
+    }
+  }
+}
+
+// injection: input particles -> record set 1 (kind 1: in-tile, brick by position)
+__global__ void __launch_bounds__(WG)
+  k_inject_records(MigPtrs m, int64_t n, const float* px, const float* py, const float* pz, const float* vx,
+                   const float* vy, const float* vz, const uint64_t* id, uint32_t meta, uint32_t* counters)
+{
+  for(int64_t e = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; e < n; e += int64_t(gridDim.x) * blockDim.x) {
+    const int64_t r = e;
+    m.pos[1][0][r] = px[e];
+    m.pos[1][1][r] = py[e];
+    m.pos[1][2][r] = pz[e];
+    m.vel[1][0][r] = vx[e];
+    m.vel[1][1][r] = vy[e];
+    m.vel[1][2][r] = vz[e];
+    m.id[1][r] = id[e];
+    m.meta[1][r] = meta;
+  }
+  if(blockIdx.x == 0 && threadIdx.x == 0) counters[C_MIG1] = uint32_t(n);
+}
+
+__global__ void __launch_bounds__(WG)
+  k_repack(const float* const* __restrict__ src, float* const* __restrict__ dst, const uint64_t* sid, uint64_t* did,
+           const int64_t* ostart, const uint32_t* ocap, uint32_t* count, const int64_t* nstart, int nbins)
+{
+  for(int b = int(blockIdx.x); b < nbins; b += int(gridDim.x)) {
+    const uint32_t c = min(count[b], ocap[b]);
+    const int64_t so = ostart[b], no = nstart[b];
+    for(uint32_t t = threadIdx.x; t < c; t += WG) {
+#pragma unroll
+      for(int d = 0; d < 6; ++d) dst[d][no + t] = src[d][so + t];
+      did[no + t] = sid[so + t];
+    }
+    __syncthreads();
+    if(threadIdx.x == 0) count[b] = c;
+  }
+}
+
+__global__ void __launch_bounds__(WG)
+  k_gather_tile(SpParams sp, int s, int bin0, int nbins, const int64_t* offs, float* ox, float* oy, float* oz,
+                float* ux, float* uy, float* uz, uint64_t* oid)
+{
+  const int bi = int(blockIdx.x);
+  if(bi >= nbins) return;
+  const int b = bin0 + bi;
+  const int64_t start = sp.start[s][b];
+  const int count = int(sp.count[s][b]);
+  const int64_t o = offs[bi];
+  for(int t = int(threadIdx.x); t < count; t += WG) {
+    ox[o + t] = sp.pos[s][0][start + t];
+    oy[o + t] = sp.pos[s][1][start + t];
+    oz[o + t] = sp.pos[s][2][start + t];
+    ux[o + t] = sp.vel[s][0][start + t];
+    uy[o + t] = sp.vel[s][1][start + t];
+    uz[o + t] = sp.vel[s][2][start + t];
+    oid[o + t] = sp.id[s][start + t];
+  }
+}
+
+// sum over alive particles of (sqrt(1 + u.u) - 1) in float, accumulated in FP64
+__global__ void __launch_bounds__(WG) k_kinetic(SpParams sp, int s, int bin0, int nbins, double* partial)
+{
+  double acc = 0.0;
+  for(int bi = int(blockIdx.x); bi < nbins; bi += int(gridDim.x)) {
+    const int b = bin0 + bi;
+    const int64_t start = sp.start[s][b];
+    const int count = int(sp.count[s][b]);
+    for(int t = int(threadIdx.x); t < count; t += WG) {
+      const float u[3] = {sp.vel[s][0][start + t], sp.vel[s][1][start + t], sp.vel[s][2][start + t]};
+      const float e = sqrtf(1.0f + dot3(u, u)) - 1.0f;
+      acc += double(e);
+    }
+  }
+  __shared__ double red[WG];
+  red[threadIdx.x] = acc;
+  __syncthreads();
+  for(int o = WG / 2; o > 0; o >>= 1) {
+    if(int(threadIdx.x) < o) red[threadIdx.x] += red[threadIdx.x + o];
+    __syncthreads();
+  }
+  if(threadIdx.x == 0) partial[blockIdx.x] = red[0];
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+namespace {
+void bin_range(const rk_world* w, int tile, int& b0, int& nb)
+{
+  if(tile < 0) {
+    b0 = 0;
+    nb = w->g.nbricks;
+  } else {
+    b0 = tile * w->g.bpt;
+    nb = w->g.bpt;
+  }
+}
+
+// grid for xcd_brick(): 8 * ceil(n/8) blocks so the permutation covers [0, n)
+unsigned xcd_grid(int n) { return unsigned((n + 7) / 8 * 8); }
+
+size_t means_lds(const Geo& g) { return sizeof(float) * 6 * size_t(g.br[0] + 2) * (g.br[1] + 2) * (g.br[2] + 2); }
+size_t cur_lds(const Geo& g) { return sizeof(float) * 3 * size_t(g.br[0] + 3) * (g.br[1] + 3) * (g.br[2] + 3); }
+}  // namespace
+
+int launch_push(rk_world* w, int tile, hipStream_t s)
+{
+  int b0, nb;
+  bin_range(w, tile, b0, nb);
+  hipLaunchKernelGGL(k_push, dim3(xcd_grid(nb)), dim3(WG), means_lds(w->g), s, w->g, make_sp(w), make_fields(w), b0,
+                     nb, w->counters);
+  RK_HIP(hipGetLastError());
+  return RK_OK;
+}
+
+int launch_deposit(rk_world* w, int tile, hipStream_t s)
+{
+  int b0, nb;
+  bin_range(w, tile, b0, nb);
+  hipLaunchKernelGGL(k_deposit, dim3(xcd_grid(nb)), dim3(WG), cur_lds(w->g), s, w->g, make_sp(w), make_fields(w),
+                     b0, nb, w->counters);
+  RK_HIP(hipGetLastError());
+  return RK_OK;
+}
+
+int launch_pack(rk_world* w, int tile, hipStream_t s)
+{
+  int b0, nb;
+  bin_range(w, tile, b0, nb);
+  hipLaunchKernelGGL(k_pack, dim3(unsigned(nb)), dim3(WG), 0, s, w->g, make_sp(w), make_mig(w), b0, nb, w->counters);
+  RK_HIP(hipGetLastError());
+  return RK_OK;
+}
+
+int launch_append(rk_world* w, int set, hipStream_t s)
+{
+  const uint32_t* none = nullptr;
+  hipLaunchKernelGGL(k_append, dim3(1024), dim3(WG), 0, s, w->g, make_sp(w), make_mig(w), set, w->counters,
+                     w->ovf_list, w->ovf_cap, none, 0);
+  RK_HIP(hipGetLastError());
+  return RK_OK;
+}
+
+int launch_reset_counter(rk_world* w, int which, hipStream_t s)
+{
+  hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, s, w->counters, which);
+  RK_HIP(hipGetLastError());
+  return RK_OK;
+}
+
+int launch_fused(rk_world* w, hipStream_t s)
+{
+  const size_t dyn = means_lds(w->g) + cur_lds(w->g);
+  hipLaunchKernelGGL(k_fused, dim3(xcd_grid(w->g.nbricks)), dim3(WG), dyn, s, w->g, make_sp(w), make_fields(w),
+                     make_mig(w), w->g.nbricks, w->counters);
+  RK_HIP(hipGetLastError());
+  return RK_OK;
+}
+
+int launch_sort(rk_world* w, int tile, hipStream_t s)
+{
+  int b0, nb;
+  bin_range(w, tile, b0, nb);
+  uint32_t maxcap = 0;
+  {
+    std::vector<uint32_t> caps(size_t(w->g.nbricks));
+    for(int sp = 0; sp < w->d.n_species; ++sp) {
+      RK_HIP(hipMemcpyAsync(caps.data(), w->sp[sp].bin_cap, sizeof(uint32_t) * caps.size(), hipMemcpyDeviceToHost, s));
+      RK_HIP(hipStreamSynchronize(s));
+      for(int b = b0; b < b0 + nb; ++b) maxcap = std::max(maxcap, caps[size_t(b)]);
+    }
+  }
+  if(maxcap == 0) return RK_OK;
+  const int grid = std::min(nb, 512);
+  const int64_t stride = (int64_t(maxcap) + 63) / 64 * 64;
+  float* stage = nullptr;
+  RK_HIP(hipMallocAsync(reinterpret_cast<void**>(&stage), sizeof(float) * 8 * size_t(stride) * size_t(grid), s));
+  const int cells = w->g.br[0] * w->g.br[1] * w->g.br[2];
+  hipLaunchKernelGGL(k_sort, dim3(unsigned(grid)), dim3(WG), sizeof(uint32_t) * size_t(cells + 2), s, w->g, make_sp(w),
+                     b0, nb, stage, stride);
+  RK_HIP(hipGetLastError());
+  RK_HIP(hipFreeAsync(stage, s));
+  return RK_OK;
+}
+
+int launch_inject(rk_world* w, int tile, int s, int64_t n, const float* const pos[3], const float* const vel[3],
+                  const uint64_t* id, hipStream_t st)
+{
+  if(n <= 0) return RK_OK;
+  if(n > w->mig[1].capacity) return (set_error("inject: record set 1 smaller than the batch"), RK_EOVERFLOW);
+  const unsigned blocks = unsigned(std::min<int64_t>((n + WG - 1) / WG, 4096));
+  hipLaunchKernelGGL(k_inject_records, dim3(blocks), dim3(WG), 0, st, make_mig(w), n, pos[0], pos[1], pos[2], vel[0],
+                     vel[1], vel[2], id, meta_pack(s, 1, 13, uint32_t(tile * w->g.bpt)), w->counters);
+  RK_HIP(hipGetLastError());
+  int rc = launch_append(w, 1, st);
+  if(rc) return rc;
+  return launch_reset_counter(w, C_MIG1, st);
+}
+
+int launch_retry(rk_world* w, hipStream_t st)
+{
+  // move the overflow list aside and re-append its records
+  uint32_t n = 0;
+  RK_HIP(hipMemcpyAsync(&n, &w->counters[C_OVF], sizeof(uint32_t), hipMemcpyDeviceToHost, st));
+  RK_HIP(hipStreamSynchronize(st));
+  if(n == 0) return RK_OK;
+  uint32_t* list = nullptr;
+  RK_HIP(hipMallocAsync(reinterpret_cast<void**>(&list), sizeof(uint32_t) * n, st));
+  RK_HIP(hipMemcpyAsync(list, w->ovf_list, sizeof(uint32_t) * n, hipMemcpyDeviceToDevice, st));
+  int rc = launch_reset_counter(w, C_OVF, st);
+  if(rc) return rc;
+  hipLaunchKernelGGL(k_append, dim3(1024), dim3(WG), 0, st, w->g, make_sp(w), make_mig(w), 0, w->counters, w->ovf_list,
+                     w->ovf_cap, list, int(n));
+  RK_HIP(hipGetLastError());
+  RK_HIP(hipFreeAsync(list, st));
+  return RK_OK;
+}
+
+int launch_repack(rk_world* w, int s, const Species& nsp, hipStream_t st)
+{
+  const Species& o = w->sp[s];
+  float* src[6] = {o.pos[0], o.pos[1], o.pos[2], o.vel[0], o.vel[1], o.vel[2]};
+  float* dst[6] = {nsp.pos[0], nsp.pos[1], nsp.pos[2], nsp.vel[0], nsp.vel[1], nsp.vel[2]};
+  float** dsrc = nullptr;
+  float** ddst = nullptr;
+  RK_HIP(hipMallocAsync(reinterpret_cast<void**>(&dsrc), sizeof(src), st));
+  RK_HIP(hipMallocAsync(reinterpret_cast<void**>(&ddst), sizeof(dst), st));
+  RK_HIP(hipMemcpyAsync(dsrc, src, sizeof(src), hipMemcpyHostToDevice, st));
+  RK_HIP(hipMemcpyAsync(ddst, dst, sizeof(dst), hipMemcpyHostToDevice, st));
+  const int grid = std::min(w->g.nbricks, 4096);
+  hipLaunchKernelGGL(k_repack, dim3(unsigned(grid)), dim3(WG), 0, st, dsrc, ddst, o.id, nsp.id, o.bin_start, o.bin_cap,
+                     o.bin_count, nsp.bin_start, w->g.nbricks);
+  RK_HIP(hipGetLastError());
+  RK_HIP(hipStreamSynchronize(st));
+  RK_HIP(hipFreeAsync(dsrc, st));
+  RK_HIP(hipFreeAsync(ddst, st));
+  return RK_OK;
+}
+
+int launch_gather_tile(rk_world* w, int tile, int s, float* const pos[3], float* const vel[3], uint64_t* id,
+                       const int64_t* offs, hipStream_t st)
+{
+  const int b0 = tile * w->g.bpt, nb = w->g.bpt;
+  hipLaunchKernelGGL(k_gather_tile, dim3(unsigned(nb)), dim3(WG), 0, st, make_sp(w), s, b0, nb, offs, pos[0], pos[1],
+                     pos[2], vel[0], vel[1], vel[2], id);
+  RK_HIP(hipGetLastError());
+  return RK_OK;
+}
+
+int kinetic_energies(rk_world* w, int tile, double* out, hipStream_t st)
+{
+  int b0, nb;
+  bin_range(w, tile, b0, nb);
+  const int grid = std::min(nb, 1024);
+  if(w->red_cap < grid) return (set_error("reduction scratch too small"), RK_EINVAL);
+  std::vector<double> h(static_cast<size_t>(grid));
+  for(int s = 0; s < w->d.n_species; ++s) {
+    hipLaunchKernelGGL(k_kinetic, dim3(unsigned(grid)), dim3(WG), 0, st, make_sp(w), s, b0, nb, w->red);
+    RK_HIP(hipGetLastError());
+    RK_HIP(hipMemcpyAsync(h.data(), w->red, sizeof(double) * h.size(), hipMemcpyDeviceToHost, st));
+    RK_HIP(hipStreamSynchronize(st));
+    double acc = 0;
+    for(double v : h) acc += v;
+    out[s] = acc;
+  }
+  return RK_OK;
+}
+
+}  // namespace rk

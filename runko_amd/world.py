@@ -1,0 +1,311 @@
+"""World: the device state behind one tile grid (or one stand-alone tile).
+
+A world owns, as torch tensors on the GPU:
+  * F: float32 [4, 3, Lx, Ly, Lz] -- E, B, J and the filter target Jf on one
+    haloed lattice of L = N+6 nodes per dimension (N = tiles * cells),
+  * per species a brick-binned SoA pool (pos[3], vel[3] float32, id uint64)
+    with bin_start (int64), bin_cap and bin_count (uint32 stored in int32),
+  * two migration record sets (tile leavers, in-tile brick changes),
+and a handle to the C-ABI world that borrows them.  Bins grow on demand: an
+append that finds its bin full leaves the record pending, the world repacks
+that species' pool with larger bins and re-appends.
+"""
+
+import ctypes
+import math
+import os
+
+import numpy as np
+import torch
+
+from . import _lib as L
+
+HALO = 3
+SLACK = 1.25
+PAD = 64
+
+
+def choose_brick(n_cells):
+    """Largest divisor <= 8 per dimension (LDS window of one workgroup)."""
+    env = os.environ.get("RUNKO_AMD_BRICK")
+    if env:
+        b = tuple(int(v) for v in env.split(","))
+        if all(n % v == 0 for n, v in zip(n_cells, b)):
+            return b
+    out = []
+    for n in n_cells:
+        for d in range(8, 0, -1):
+            if n % d == 0:
+                out.append(d)
+                break
+    return tuple(out)
+
+
+def _stream():
+    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def _require_gpu():
+    if not torch.cuda.is_available():
+        raise RuntimeError("runko_amd needs a ROCm GPU (no CPU fallback)")
+
+
+class _Pool:
+    def __init__(self, nb, cap_per_bin, dev):
+        caps = np.asarray(cap_per_bin, dtype=np.int64)
+        self.set_layout(caps, dev)
+
+    def set_layout(self, caps, dev):
+        starts = np.zeros_like(caps)
+        starts[1:] = np.cumsum(caps)[:-1]
+        self.capacity = int(caps.sum())
+        n = max(self.capacity, 1)
+        self.pos = torch.empty((3, n), dtype=torch.float32, device=dev)
+        self.vel = torch.empty((3, n), dtype=torch.float32, device=dev)
+        self.id = torch.empty((n,), dtype=torch.int64, device=dev)
+        self.start = torch.from_numpy(starts).to(dev)
+        self.cap = torch.from_numpy(caps.astype(np.int32)).to(dev)
+
+
+class World:
+    def __init__(self, n_cells, n_tiles, tile_offset, grid_tiles, periodic, cfl, charges, masses,
+                 current_filter="binomial2", brick=None, device="cuda"):
+        _require_gpu()
+        self.dev = torch.device(device)
+        self.n = tuple(int(v) for v in n_cells)
+        self.nt = tuple(int(v) for v in n_tiles)
+        self.toff = tuple(int(v) for v in tile_offset)
+        self.gt = tuple(int(v) for v in grid_tiles)
+        self.periodic = bool(periodic)
+        self.cfl = float(cfl)
+        self.charges = [float(q) for q in charges]
+        self.masses = [float(m) for m in masses]
+        self.nsp = len(self.charges)
+        self.brick = tuple(brick) if brick else choose_brick(self.n)
+        self.N = tuple(a * b for a, b in zip(self.n, self.nt))
+        self.Lh = tuple(v + 2 * HALO for v in self.N)
+        d = L.WorldDesc()
+        for a in range(3):
+            d.n_cells[a] = self.n[a]
+            d.n_tiles[a] = self.nt[a]
+            d.tile_offset[a] = self.toff[a]
+            d.grid_tiles[a] = self.gt[a]
+            d.brick[a] = self.brick[a]
+        d.periodic = 1 if periodic else 0
+        d.n_species = self.nsp
+        d.current_filter = {"binomial2": 0, "binomial2_unrolled": 1}[current_filter or "binomial2"]
+        d.cfl = self.cfl
+        for s in range(self.nsp):
+            d.charge[s] = self.charges[s]
+            d.mass[s] = self.masses[s]
+        h = ctypes.c_void_p()
+        L.check(L.lib().rk_world_create(ctypes.byref(h), ctypes.byref(d)))
+        self.h = h
+        self.F = torch.zeros((4, 3) + self.Lh, dtype=torch.float32, device=self.dev)
+        L.check(L.lib().rk_world_set_fields(self.h, L.p3(self.F[0]), L.p3(self.F[1]), L.p3(self.F[2]),
+                                            L.p3(self.F[3])))
+        self.nbricks = L.lib().rk_world_num_bricks(self.h)
+        self.bpt = self.nbricks // (self.nt[0] * self.nt[1] * self.nt[2])
+        self.count = [torch.zeros((self.nbricks,), dtype=torch.int32, device=self.dev) for _ in range(self.nsp)]
+        self.pools = [_Pool(self.nbricks, np.zeros(self.nbricks, dtype=np.int64), self.dev) for _ in range(self.nsp)]
+        for s in range(self.nsp):
+            self._bind_species(s)
+        self.mig = [None, None]
+        self._mig_cap = [0, 0]
+        self.ensure_migration(1 << 12)
+
+    # ------------------------------------------------------------------ plumbing
+    def __del__(self):
+        h = getattr(self, "h", None)
+        if h:
+            try:
+                torch.cuda.synchronize()
+                L.lib().rk_world_destroy(h)
+            except Exception:
+                pass
+            self.h = None
+
+    def _bind_species(self, s):
+        p = self.pools[s]
+        L.check(L.lib().rk_world_set_species(self.h, s, p.capacity, L.p3(p.pos), L.p3(p.vel), p.id.data_ptr(),
+                                             p.start.data_ptr(), p.cap.data_ptr(), self.count[s].data_ptr()))
+
+    def ensure_migration(self, cap):
+        for st in range(2):
+            if self._mig_cap[st] >= cap:
+                continue
+            c = int(cap)
+            rec = dict(pos=torch.empty((3, c), dtype=torch.float32, device=self.dev),
+                       vel=torch.empty((3, c), dtype=torch.float32, device=self.dev),
+                       id=torch.empty((c,), dtype=torch.int64, device=self.dev),
+                       meta=torch.empty((c,), dtype=torch.int32, device=self.dev))
+            torch.cuda.synchronize()
+            L.check(L.lib().rk_world_set_migration(self.h, st, c, L.p3(rec["pos"]), L.p3(rec["vel"]),
+                                                   rec["id"].data_ptr(), rec["meta"].data_ptr()))
+            self.mig[st] = rec
+            self._mig_cap[st] = c
+
+    @property
+    def ntiles(self):
+        return self.nt[0] * self.nt[1] * self.nt[2]
+
+    def tile_coords(self, t):
+        return (t // (self.nt[1] * self.nt[2]), (t // self.nt[2]) % self.nt[1], t % self.nt[2])
+
+    def tile_interior(self, t):
+        c = self.tile_coords(t)
+        return tuple(slice(HALO + c[a] * self.n[a], HALO + (c[a] + 1) * self.n[a]) for a in range(3))
+
+    def tile_window(self, t):
+        c = self.tile_coords(t)
+        return tuple(slice(c[a] * self.n[a], c[a] * self.n[a] + self.n[a] + 2 * HALO) for a in range(3))
+
+    @property
+    def E(self):
+        return self.F[0]
+
+    @property
+    def B(self):
+        return self.F[1]
+
+    @property
+    def J(self):
+        """The buffer currently holding J (filter passes swap F[2] and F[3])."""
+        p = L.lib().rk_world_j_pointer(self.h, 0)
+        return self.F[2] if p == self.F[2][0].data_ptr() else self.F[3]
+
+    # ------------------------------------------------------------------ methods
+    def call(self, name, tile):
+        f = getattr(L.lib(), name)
+        rc = f(self.h, int(tile), _stream())
+        if rc == L.RK_EOVERFLOW:
+            self.settle_overflow()
+            return
+        L.check(rc)
+
+    def comm_local(self, mode):
+        rc = L.lib().rk_comm_local(self.h, int(mode), _stream())
+        if rc == L.RK_EOVERFLOW:
+            self.settle_overflow()
+            return
+        L.check(rc)
+
+    def particle_step_fused(self):
+        L.check(L.lib().rk_particle_step_fused(self.h, _stream()))
+
+    def pending_overflow(self):
+        n = ctypes.c_int64(0)
+        L.check(L.lib().rk_pending_overflow(self.h, ctypes.byref(n), _stream()))
+        return n.value
+
+    def settle_overflow(self):
+        """Repack every species whose bins overflowed, then re-append the
+        pending records.  Synchronous."""
+        for _ in range(4):
+            if self.pending_overflow() == 0:
+                return
+            for s in range(self.nsp):
+                demand = self.count[s].cpu().numpy().view(np.uint32).astype(np.int64)
+                caps = self.pools[s].cap.cpu().numpy().view(np.uint32).astype(np.int64)
+                if np.all(demand <= caps):
+                    continue
+                newcaps = np.maximum(caps, np.ceil(demand * SLACK).astype(np.int64) + PAD)
+                newcaps = (newcaps + 31) // 32 * 32
+                self._repack(s, newcaps)
+            L.check(L.lib().rk_retry_append(self.h, _stream()))
+        if self.pending_overflow():
+            raise RuntimeError("runko_amd: particle bins still overflowing after repack")
+
+    def _repack(self, s, newcaps):
+        newp = _Pool(self.nbricks, newcaps, self.dev)
+        L.check(L.lib().rk_repack(self.h, s, newp.capacity, L.p3(newp.pos), L.p3(newp.vel), newp.id.data_ptr(),
+                                  newp.start.data_ptr(), newp.cap.data_ptr(), _stream()))
+        torch.cuda.synchronize()
+        self.pools[s] = newp
+
+    def presize(self, s, demand):
+        """Lay out species s with bins sized for `demand` particles per bin
+        (only while the species is empty)."""
+        if int(self.count[s].sum().item()) != 0:
+            raise RuntimeError("presize on a non-empty species")
+        caps = np.ceil(np.asarray(demand, dtype=np.float64) * SLACK).astype(np.int64)
+        caps = np.where(caps > 0, caps + PAD, 0)
+        caps = (caps + 31) // 32 * 32
+        self.pools[s] = _Pool(self.nbricks, caps, self.dev)
+        self._bind_species(s)
+
+    def bricks_of(self, t, pos):
+        """Brick index (world bin) of positions pos [3, n] (float32 tensor) in tile t."""
+        c = self.tile_coords(t)
+        lb = []
+        for a in range(3):
+            origo = np.float32((self.toff[a] + c[a]) * self.n[a]) - np.float32(3.0)
+            cell = torch.floor(pos[a] - origo).to(torch.int64) - HALO
+            cell = cell.clamp(0, self.n[a] - 1)
+            lb.append(cell // self.brick[a])
+        nbd = [self.n[a] // self.brick[a] for a in range(3)]
+        return t * self.bpt + (lb[0] * nbd[1] + lb[1]) * nbd[2] + lb[2]
+
+    def inject(self, t, s, pos, vel, ids):
+        """pos, vel: float32 device tensors [3, n]; ids: int64 device tensor [n]."""
+        n = int(ids.numel())
+        if n == 0:
+            return
+        self.ensure_migration(n)
+        pos = pos.contiguous()
+        vel = vel.contiguous()
+        ids = ids.contiguous()
+        rc = L.lib().rk_inject(self.h, int(t), int(s), n, L.p3(pos), L.p3(vel), ids.data_ptr(), _stream())
+        if rc == L.RK_EOVERFLOW:
+            self.settle_overflow()
+        else:
+            L.check(rc)
+
+    # ------------------------------------------------------------------ readback
+    def tile_count(self, t, s):
+        n = ctypes.c_int64(0)
+        L.check(L.lib().rk_tile_particle_count(self.h, int(t), int(s), ctypes.byref(n), _stream()))
+        return n.value
+
+    def tile_particles(self, t, s):
+        n = self.tile_count(t, s)
+        pos = torch.empty((3, max(n, 1)), dtype=torch.float32, device=self.dev)
+        vel = torch.empty((3, max(n, 1)), dtype=torch.float32, device=self.dev)
+        ids = torch.empty((max(n, 1),), dtype=torch.int64, device=self.dev)
+        got = ctypes.c_int64(0)
+        L.check(L.lib().rk_tile_particles(self.h, int(t), int(s), L.p3(pos), L.p3(vel), ids.data_ptr(), max(n, 1),
+                                          ctypes.byref(got), _stream()))
+        return pos[:, :n], vel[:, :n], ids[:n]
+
+    def outgoing(self):
+        cap = self._mig_cap[0]
+        pos = torch.empty((3, cap), dtype=torch.float32, device=self.dev)
+        vel = torch.empty((3, cap), dtype=torch.float32, device=self.dev)
+        ids = torch.empty((cap,), dtype=torch.int64, device=self.dev)
+        src = np.empty(cap, dtype=np.int32)
+        dirs = np.empty(cap, dtype=np.int32)
+        sp = np.empty(cap, dtype=np.int32)
+        n = ctypes.c_int64(0)
+        L.check(L.lib().rk_outgoing(self.h, L.p3(pos), L.p3(vel), ids.data_ptr(), src.ctypes.data,
+                                    dirs.ctypes.data, sp.ctypes.data, cap, ctypes.byref(n), _stream()))
+        k = n.value
+        return (pos[:, :k].cpu().numpy(), vel[:, :k].cpu().numpy(), ids[:k].cpu().numpy().view(np.uint64),
+                src[:k].copy(), dirs[:k].copy(), sp[:k].copy())
+
+    def energies(self, tile=-1):
+        out = np.zeros(2 + self.nsp, dtype=np.float64)
+        L.check(L.lib().rk_energies(self.h, int(tile), out.ctypes.data, _stream()))
+        return out
+
+    def gauss_residual(self, filter_passes):
+        out = np.zeros(3, dtype=np.float64)
+        L.check(L.lib().rk_gauss_residual(self.h, int(filter_passes), out.ctypes.data, _stream()))
+        return out
+
+    def error_counters(self):
+        out = np.zeros(2, dtype=np.int64)
+        L.check(L.lib().rk_error_counters(self.h, out.ctypes.data, _stream()))
+        return out
+
+    def total_particles(self):
+        return sum(int(c.sum().item()) for c in self.count)

@@ -1,0 +1,157 @@
+"""Builders of identical inputs for the GPU path (runko_amd) and the oracle."""
+
+import numpy as np
+
+import runko_amd as runko
+
+CFL = 0.45
+
+
+def make_config(nt, n, charges, masses, filt="binomial2", cfl=CFL, execution="plain"):
+    c = runko.Configuration(None)
+    c.n_tiles = list(nt)
+    c.n_cells_per_tile = list(n)
+    c.cfl = cfl
+    c.field_propagator = "fdtd2"
+    c.particle_pusher = "boris"
+    c.field_interpolator = "linear_1st"
+    c.current_depositer = "zigzag_1st_atomic"
+    c.current_filter = filt
+    c.tile_partitioning = "hilbert_curve"
+    c.n_laps = 10
+    c.execution = execution
+    for s, (q, m) in enumerate(zip(charges, masses)):
+        setattr(c, f"q{s}", q)
+        setattr(c, f"m{s}", m)
+    return c
+
+
+def random_inputs(nt, n, nsp, seed, ppc=4, field_amp=0.05, umax=1.5):
+    """Per tile: fields [9, n] (J zero) and per species particle states [k, 6]."""
+    rng = np.random.default_rng(seed)
+    tiles = {}
+    for i in range(nt[0]):
+        for j in range(nt[1]):
+            for k in range(nt[2]):
+                F = np.zeros((9,) + tuple(n))
+                F[0:6] = field_amp * rng.standard_normal((6,) + tuple(n))
+                parts = []
+                cells = np.prod(n)
+                lo = np.array([i * n[0], j * n[1], k * n[2]], dtype=np.float64)
+                base = np.stack(np.meshgrid(*(np.arange(v, dtype=np.float64) for v in n), indexing="ij"), -1)
+                base = base.reshape(-1, 3)
+                for s in range(nsp):
+                    pos = np.repeat(base, ppc, 0) + rng.random((cells * ppc, 3)) + lo
+                    u = (rng.random((cells * ppc, 3)) * 2 - 1) * umax
+                    parts.append(np.concatenate([pos, u], 1))
+                tiles[(i, j, k)] = (F, parts)
+    return tiles
+
+
+def build_gpu(nt, n, charges, masses, inputs, filt="binomial2", execution="plain", comm_prelude=True):
+    conf = make_config(nt, n, charges, masses, filt=filt, execution=execution)
+    grid = runko.TileGrid(conf)
+    for idx in grid.local_tile_indices():
+        F, parts = inputs[idx]
+        tile = runko.pic.threeD.Tile(idx, conf)
+        fs = [lambda x, y, z, a=F[c]: a for c in range(9)]
+        tile.batch_set_EBJ(*fs)
+        for s, st in enumerate(parts):
+            tile.batch_inject_to_cells(
+                s, lambda x, y, z, st=st: runko.pic.threeD.ParticleStateBatch(pos=(st[:, 0], st[:, 1], st[:, 2]),
+                                                                            vel=(st[:, 3], st[:, 4], st[:, 5])))
+        grid.add_tile(tile, idx)
+    sim = grid.configure_simulation(conf)
+    if comm_prelude:
+        def sync(x):
+            x.comm_local(runko.tools.comm_mode.emf_E)
+            x.comm_local(runko.tools.comm_mode.emf_B)
+        sim.prelude(sync)
+    return conf, grid, sim
+
+
+def build_oracle(orc, nt, n, charges, masses, inputs, prec=32, filt="binomial2", comm_prelude=True):
+    w = orc.OracleWorld(nt, n, CFL, charges, masses, prec=prec, current_filter=filt)
+    for idx, (F, parts) in inputs.items():
+        t = w.tile_index(*idx)
+        w.set_fields(t, F)
+        for s, st in enumerate(parts):
+            w.add_particles(t, s, st)
+    if comm_prelude:
+        w.comm_local("emf_E")
+        w.comm_local("emf_B")
+    return w
+
+
+def gpu_particles(grid, s):
+    """(tile index per particle, states [k,6] float64 of the float32 values, ids) sorted by id."""
+    ts, sts, ids = [], [], []
+    for t, tile in enumerate(grid.tiles()):
+        x, y, z = tile.get_positions(s)
+        u, v, w = tile.get_velocities(s)
+        i = tile.get_ids(s)
+        ts.append(np.full(len(i), t))
+        sts.append(np.stack([x, y, z, u, v, w], 1).astype(np.float64))
+        ids.append(i)
+    ts, sts, ids = np.concatenate(ts), np.concatenate(sts), np.concatenate(ids)
+    o = np.argsort(ids, kind="stable")
+    return ts[o], sts[o], ids[o]
+
+
+def oracle_particles(w, s):
+    ts, sts, ids = w.all_particles(s)
+    o = np.argsort(ids, kind="stable")
+    return ts[o], sts[o], ids[o]
+
+
+def gpu_fields(grid):
+    """[9, Nx, Ny, Nz] interior fields assembled from the tiles."""
+    tiles = grid.tiles()
+    n = tiles[0].n
+    nt = grid.n_tiles
+    out = np.zeros((9,) + tuple(a * b for a, b in zip(nt, n)))
+    for t, tile in enumerate(tiles):
+        i, j, k = tile.index
+        (E, B, J) = tile.get_EBJ()
+        arr = np.stack(list(E) + list(B) + list(J))
+        out[:, i * n[0]:(i + 1) * n[0], j * n[1]:(j + 1) * n[1], k * n[2]:(k + 1) * n[2]] = arr
+    return out
+
+
+def canonical_lap(sim, runko_mod=runko, io=False, sort_every=5):
+    """projects/pic-turbulence/pic.py:187-221 without snapshot IO."""
+    cm = runko_mod.tools.comm_mode
+
+    def step(x):
+        x.grid_push_half_b()
+        x.comm_external(cm.emf_B)
+        x.comm_local(cm.emf_B)
+        x.prtcl_push()
+        x.prtcl_pack_outgoing()
+        x.comm_external(cm.pic_particle)
+        x.comm_local(cm.pic_particle)
+        if sort_every and sim.lap % sort_every == 0:
+            x.prtcl_sort()
+        x.prtcl_deposit_current()
+        x.comm_external(cm.emf_J)
+        x.comm_local(cm.emf_J_exchange)
+        x.comm_external(cm.emf_J)
+        x.comm_local(cm.emf_J)
+        x.grid_filter_current()
+        x.comm_external(cm.emf_J)
+        x.comm_local(cm.emf_J)
+        x.grid_filter_current()
+        x.grid_filter_current()
+        x.grid_push_half_b()
+        x.comm_external(cm.emf_B)
+        x.comm_local(cm.emf_B)
+        x.grid_push_e()
+        x.grid_add_current()
+        x.comm_external(cm.emf_E)
+        x.comm_local(cm.emf_E)
+        if io:
+            x.io_average_kinetic_energy()
+            x.io_average_B_energy_density()
+            x.io_average_E_energy_density()
+
+    return step

@@ -1,0 +1,92 @@
+"""Host-side logic of the runko-compatible API that needs no GPU: the
+configuration object, MethodWrapper dispatch, timers, the thermal sampler,
+and brick selection."""
+
+import numpy as np
+import pytest
+from scipy.special import kn
+
+from runko_amd.configuration import Configuration
+from runko_amd.method_wrapper import MethodWrapper
+from runko_amd.sampling import sample_boosted_juttner_synge
+from runko_amd.timer import Timer, timer_statistics
+from runko_amd.tools import comm_mode
+from runko_amd.world import choose_brick
+
+
+def test_configuration_ini(tmp_path):
+    p = tmp_path / "c.ini"
+    p.write_text("[grid]\nn_tiles = [2, 3, 4]\ncfl = 0.45 # comment\n[particles]\nq0 = -1.0\nname = 'x'\n")
+    c = Configuration(str(p))
+    assert c.n_tiles == [2, 3, 4] and c.cfl == 0.45 and c.q0 == -1.0 and c.name == "x"
+    assert c.does_not_exist is None
+    with pytest.raises(AttributeError):
+        c.__missing_dunder__
+    assert Configuration(None).anything is None
+    with pytest.raises(ValueError):
+        Configuration(str(tmp_path / "nope.ini"))
+
+
+def test_method_wrapper_dispatch_order_and_binding():
+    log = []
+    w = MethodWrapper(lambda n, *a: log.append(("a", n, a)), pre=lambda n, *a: log.append(("pre", n)),
+                      post=lambda n, *a: log.append(("post", n)))
+    w.foo(1, 2)
+    assert log == [("pre", "foo"), ("a", "foo", (1, 2)), ("post", "foo")]
+    strict = MethodWrapper(lambda n: None)
+    with pytest.raises(TypeError):
+        strict.bar(3)
+    with pytest.raises(AttributeError):
+        strict.__dunder__
+
+
+def test_timer_name_rules():
+    t = Timer()
+    t.start("a")
+    t.start("a")
+    t.stop("a")
+    t.stop("a")
+    assert set(t.get_elapsed_times()) == {"a", "a-1"}
+    with pytest.raises(RuntimeError):
+        t.stop("b")
+    st = timer_statistics([t, t])
+    assert st["a"].count == 2
+
+
+def test_comm_mode_values():
+    # src/runko/communication_common.h:30-38
+    assert [int(m) for m in (comm_mode.emf_J, comm_mode.emf_E, comm_mode.emf_B, comm_mode.pic_particle,
+                             comm_mode.pic_particle_extra, comm_mode.emf_J_exchange)] == [0, 1, 2, 3, 4, 6]
+
+
+@pytest.mark.parametrize("theta", [0.3, 1.0, 0.05])
+def test_juttner_synge_mean_gamma(theta):
+    # <gamma> of the Juttner-Synge distribution = K1(1/t)/K2(1/t) + 3t  (t > 0.2, Sobol branch)
+    # below 0.2 the reference draws |u| from a Rayleigh law with sigma = sqrt(2 theta)
+    # (runko/sample_thermal_distributions.py:41-55), so <u^2> = 2 sigma^2 = 4 theta
+    gen = np.random.default_rng(123)
+    ux, uy, uz = sample_boosted_juttner_synge(200000, theta, beta=0, gen=gen)
+    u2 = ux**2 + uy**2 + uz**2
+    if theta > 0.2:
+        mean_g = np.mean(np.sqrt(1 + u2))
+        expected = kn(1, 1 / theta) / kn(2, 1 / theta) + 3 * theta
+        assert mean_g == pytest.approx(expected, rel=5e-3)
+    else:
+        assert np.mean(u2) == pytest.approx(4 * theta, rel=1e-2)
+    assert abs(np.mean(ux)) < 0.01 * np.sqrt(np.mean(u2))
+
+
+def test_boosted_drift_and_errors():
+    gen = np.random.default_rng(5)
+    ux, uy, uz = sample_boosted_juttner_synge(100000, 1e-4, Gamma=10.0, direction="-x", gen=gen)
+    assert np.mean(ux) == pytest.approx(-np.sqrt(99.0), rel=1e-3)
+    with pytest.raises(RuntimeError):
+        sample_boosted_juttner_synge(10, 0.3, beta=0.1, Gamma=2.0)
+    with pytest.raises(RuntimeError):
+        sample_boosted_juttner_synge(10, 0.3, beta=0.1, direction="w")
+
+
+def test_brick_choice():
+    assert choose_brick((16, 16, 16)) == (8, 8, 8)
+    assert choose_brick((10, 11, 13)) == (5, 1, 1)
+    assert choose_brick((32, 6, 12)) == (8, 6, 6)
