@@ -1,0 +1,155 @@
+"""GPU-side initial conditions for large worlds.
+
+Host-side injection (Tile.batch_inject_to_cells with a Python generator per
+tile) is the reference's way in (src/runko/pic/tile.c++:207-322); at 4.3e9
+particles it would dominate the run, so this module fills a grid world's
+pools directly on the device:
+  * positions uniform in every cell (ppc per species), species 1 on top of
+    species 0 as in projects/pic-turbulence/pic.py:141-156,
+  * momenta from the Juttner-Synge distribution (Sobol rejection for
+    theta > 0.2, Rayleigh magnitude below, isotropic direction), the same
+    algorithm as runko/sample_thermal_distributions.py:7-127, drawn with
+    torch's Philox generator (statistically, not bitwise, equal),
+  * ids (tile_tag << 40) | ordinal, ordinals counted brick by brick,
+  * a guide field B0 along z plus n x n divergence-free sinusoidal dBx, dBy
+    modes with seeded random phases, normalised to rms(dB) = dB_over_B0 * B0.
+"""
+
+import math
+
+import numpy as np
+import torch
+from scipy.special import kn
+
+from .world import HALO, World
+
+
+def plasma_parameters(cfl, c_omp, ppc, theta, sigma):
+    """Charges and guide field of projects/pic-turbulence/pic.py:44-68
+    (m0 = m1 = 1 mass-to-charge ratios, oppc = 2 ppc)."""
+    oppc = 2 * ppc
+    omp = cfl / c_omp
+    q0 = -(omp**2) / (0.5 * oppc * (1.0 + 1.0))
+    m0hat = abs(q0)
+    B0 = math.sqrt((1.0 + 1.5 * theta) * oppc * m0hat * cfl**2 * sigma)
+    return q0, -q0, B0
+
+
+def juttner_momenta(n, theta, gen, device):
+    """n samples (ux, uy, uz) float32 of the isotropic Juttner-Synge law."""
+    if theta > 0.2:
+        eff = kn(2, 1 / theta) / (2 * theta**2)
+        parts, have = [], 0
+        while have < n:
+            m = int(1.05 * (n - have) / eff) + 4096
+            x = torch.rand((4, m), generator=gen, device=device, dtype=torch.float32).clamp_min_(1e-30)
+            p3 = x[0] * x[1] * x[2]
+            u = -theta * torch.log(p3)
+            eta = -theta * torch.log(p3 * x[3])
+            keep = (eta * eta - u * u) > 1.0
+            parts.append(u[keep])
+            have += int(parts[-1].numel())
+            del x, p3, eta, keep
+        u = torch.cat(parts)[:n]
+    else:
+        vth = math.sqrt(2 * theta)
+        u = vth * torch.sqrt(-2.0 * torch.log(torch.rand(n, generator=gen, device=device).clamp_min_(1e-30)))
+    X1 = torch.rand(n, generator=gen, device=device)
+    X2 = torch.rand(n, generator=gen, device=device)
+    s = 2 * u * torch.sqrt(X1 * (1 - X1))
+    return torch.stack([u * (2 * X1 - 1), s * torch.cos(2 * math.pi * X2), s * torch.sin(2 * math.pi * X2)])
+
+
+def fill_uniform_plasma(world, ppc, theta, seed=1, chunk_bins=4096):
+    """Every cell gets ppc particles of every species (all species at the same
+    positions), bins laid out for exactly that occupancy plus slack."""
+    dev = world.dev
+    br = world.brick
+    cells = br[0] * br[1] * br[2]
+    per_bin = cells * ppc
+    for s in range(world.nsp):
+        world.presize(s, np.full(world.nbricks, per_bin, dtype=np.int64))
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(int(seed))
+    nbd = [world.n[a] // br[a] for a in range(3)]
+    cl = torch.stack(torch.meshgrid(*(torch.arange(v, device=dev) for v in br), indexing="ij"), -1).reshape(-1, 3)
+    for b0 in range(0, world.nbricks, chunk_bins):
+        b1 = min(world.nbricks, b0 + chunk_bins)
+        b = torch.arange(b0, b1, device=dev)
+        tile = b // world.bpt
+        lb = b % world.bpt
+        lbc = torch.stack([lb // (nbd[1] * nbd[2]), (lb // nbd[2]) % nbd[1], lb % nbd[2]], -1)
+        tc = torch.stack([tile // (world.nt[1] * world.nt[2]), (tile // world.nt[2]) % world.nt[1],
+                          tile % world.nt[2]], -1)
+        origin = (tc + torch.tensor(world.toff, device=dev)) * torch.tensor(world.n, device=dev) + \
+            lbc * torch.tensor(br, device=dev)
+        nb = b1 - b0
+        base = (origin[:, None, None, :] + cl[None, :, None, :]).expand(nb, cells, ppc, 3).to(torch.float64)
+        pos = (base + torch.rand((nb, cells, ppc, 3), generator=gen, device=dev, dtype=torch.float64))
+        pos = pos.to(torch.float32).reshape(nb, per_bin, 3)
+        ordinal = (lb[:, None] * per_bin + torch.arange(per_bin, device=dev)[None, :])
+        ids = (tile[:, None] << 40) | ordinal
+        for s in range(world.nsp):
+            pool = world.pools[s]
+            cap = int(pool.cap[0].item())
+            u = juttner_momenta(nb * per_bin, theta, gen, dev).reshape(3, nb, per_bin)
+            for d in range(3):
+                pool.pos[d][: world.nbricks * cap].view(world.nbricks, cap)[b0:b1, :per_bin] = pos[:, :, d]
+                pool.vel[d][: world.nbricks * cap].view(world.nbricks, cap)[b0:b1, :per_bin] = u[d]
+            pool.id[: world.nbricks * cap].view(world.nbricks, cap)[b0:b1, :per_bin] = ids
+    for s in range(world.nsp):
+        world.count[s].fill_(per_bin)
+    torch.cuda.synchronize()
+
+
+def set_turbulent_field(world, B0, n_modes=8, dB_over_B0=1.0, seed=2):
+    """Bz = B0 everywhere; dBx, dBy from A_z = sum_{m,n=1..n_modes} a_mn
+    sin(m k x + n k y + phi_mn) (k = 2 pi / L), so div dB = 0; every mode has
+    the same amplitude and the total is scaled to rms(dB) = dB_over_B0 * B0."""
+    dev = world.dev
+    N = world.N
+    rng = np.random.default_rng(seed)
+    phi = 2 * np.pi * rng.random((n_modes, n_modes))
+    kx, ky = 2 * np.pi / N[0], 2 * np.pi / N[1]
+    ii = torch.arange(N[0], device=dev, dtype=torch.float64)
+    jj = torch.arange(N[1], device=dev, dtype=torch.float64)
+
+    def dB(xoff, yoff):
+        X = (ii + xoff)[:, None]
+        Y = (jj + yoff)[None, :]
+        bx = torch.zeros((N[0], N[1]), device=dev, dtype=torch.float64)
+        by = torch.zeros_like(bx)
+        for m in range(1, n_modes + 1):
+            for n in range(1, n_modes + 1):
+                kk = math.sqrt((m * kx) ** 2 + (n * ky) ** 2)
+                c = torch.cos(m * kx * X + n * ky * Y + float(phi[m - 1, n - 1]))
+                bx += (n * ky / kk) * c
+                by -= (m * kx / kk) * c
+        return bx, by
+
+    bx, _ = dB(0.0, 0.5)  # Bx at (i, j+1/2, k+1/2)
+    _, by = dB(0.5, 0.0)  # By at (i+1/2, j, k+1/2)
+    rms = math.sqrt(float((bx**2 + by**2).mean()))
+    scale = dB_over_B0 * B0 / rms
+    inner = (slice(HALO, HALO + N[0]), slice(HALO, HALO + N[1]), slice(HALO, HALO + N[2]))
+    world.F.zero_()
+    world.F[1][0][inner] = (scale * bx).to(torch.float32)[:, :, None]
+    world.F[1][1][inner] = (scale * by).to(torch.float32)[:, :, None]
+    world.F[1][2][inner] = float(B0)
+    torch.cuda.synchronize()
+
+
+def turbulence_world(N=512, tile=32, ppc=16, theta=0.3, sigma=10.0, cfl=0.45, c_omp=10.0, n_modes=8,
+                     dB_over_B0=1.0, seed=1, brick=None, mig_fraction=0.08):
+    """Decaying-turbulence box: N^3 cells in tile^3 tiles, periodic."""
+    nt = N // tile
+    q0, q1, B0 = plasma_parameters(cfl, c_omp, ppc, theta, sigma)
+    w = World((tile,) * 3, (nt,) * 3, (0, 0, 0), (nt,) * 3, periodic=True, cfl=cfl, charges=[q0, q1],
+              masses=[1.0, 1.0], current_filter="binomial2", brick=brick)
+    set_turbulent_field(w, B0, n_modes=n_modes, dB_over_B0=dB_over_B0, seed=seed + 1)
+    fill_uniform_plasma(w, ppc, theta, seed=seed)
+    w.ensure_migration(max(1 << 16, int(mig_fraction * w.total_particles())))
+    w.comm_local(1)
+    w.comm_local(2)
+    torch.cuda.synchronize()
+    return w, dict(q0=q0, q1=q1, B0=B0)
