@@ -1,0 +1,242 @@
+#!/usr/bin/env python
+"""Benchmark of the PIC time step on one MI355X.
+
+Headline (`value`): particle-steps/s of the full PIC step in the batched form
+(one launch per method over every tile, fused particle phase) on the 3D
+decaying-turbulence workload: 512^3 cells in 32^3-cell tiles, 16
+particles/cell/species (4.29e9 particles), theta = 0.3, sigma = 10, guide
+field + 8x8 sinusoidal dB modes, FP32.  Inputs are generated on the GPU and
+resident in HBM before the timed region.  Each step = half-B push, B ghost
+sync, fused interpolation + Boris + migration + zigzag deposit, J fold and
+sync, 3 binomial passes, half-B push, E push with E -= J, E sync.
+
+Also reported: the plain per-tile form (one call per tile per method through
+the runko-compatible Simulation) vs the batched form on the 64^3 workload,
+the roofline of the dominant kernel from HIP events around its launches, and
+the oracle (FP32 C++/OpenMP restatement) timed on the host cores.
+
+usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
+"""
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "particle-steps/s/GPU full PIC step, batched vs plain; adjoint/forward time"
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def run_turbulence(args):
+    import torch
+
+    from runko_amd import _lib
+    from runko_amd.init import turbulence_world
+    from runko_amd.stepper import BatchedLap
+
+    N, tile, ppc = args.cells, args.tile, args.ppc
+    t0 = time.time()
+    w, info = turbulence_world(N=N, tile=tile, ppc=ppc, theta=0.3, sigma=10.0, cfl=0.45, c_omp=10.0, n_modes=8,
+                               dB_over_B0=1.0, seed=1, mig_fraction=args.mig_fraction)
+    npart = w.total_particles()
+    ncell = N ** 3
+    log(f"[bench] turbulence {N}^3, tiles {tile}^3, {npart} particles, brick {w.brick}, setup {time.time() - t0:.1f}s")
+    lap = BatchedLap(w)
+    for i in range(args.warmup):
+        lap()
+        lap.settle()
+        log(f"[bench] warmup step {i} done")
+    torch.cuda.synchronize()
+    _lib.check(_lib.lib().rk_profile_enable(w.h, 1))
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    t_start = time.perf_counter()
+    ev0.record()
+    for i in range(args.steps):
+        lap()
+        lap.settle()
+    ev1.record()
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t_start
+    _lib.check(_lib.lib().rk_profile_enable(w.h, 0))
+    import numpy as np
+    ms = np.zeros(9)
+    cnt = np.zeros(9, dtype=np.int64)
+    _lib.check(_lib.lib().rk_profile_read(w.h, ms.ctypes.data, cnt.ctypes.data))
+    prof = {name: {"ms_total": float(ms[i]), "launches": int(cnt[i])} for i, name in enumerate(_lib.PROF_CATEGORIES)}
+    assert w.total_particles() == npart, "particle count changed"
+    err = w.error_counters()
+    fused_ms = ms[0] / max(cnt[0], 1)
+    alg_bytes = 48.0 * npart + 36.0 * ncell  # per fused launch: pos/u r+w, E/B staging read, J write
+    achieved = alg_bytes / (fused_ms * 1e-3) / 1e9
+    field_ms = float(ms[5] + ms[6] + ms[7])
+    return dict(
+        npart=npart, ncell=ncell, steps=args.steps, wall=wall, gpu_ms=ev0.elapsed_time(ev1),
+        ms_per_step=1e3 * wall / args.steps, prof=prof, fused_ms=float(fused_ms), alg_bytes=alg_bytes,
+        achieved_gbs=achieved, field_ms_per_step=field_ms / args.steps,
+        cell_updates_per_s=ncell * args.steps / (field_ms * 1e-3) if field_ms > 0 else None,
+        error_counters=[int(v) for v in err], brick=list(w.brick), B0=info["B0"])
+
+
+def config0_inputs(seed=42):
+    """64^3 cells, 4^3 tiles of 16^3, 8 ppc/species, theta 0.3, sigma 1, E = 0,
+    B = B0 z (projects/pic-turbulence/pic.py conventions)."""
+    import numpy as np
+
+    from runko_amd.init import plasma_parameters
+    from runko_amd.sampling import sample_boosted_juttner_synge
+
+    nt, n, ppc, theta = (4, 4, 4), (16, 16, 16), 8, 0.3
+    q0, q1, B0 = plasma_parameters(0.45, 10.0, ppc, theta, 1.0)
+    rng = np.random.default_rng(seed)
+    tiles = {}
+    cells = np.stack(np.meshgrid(*(np.arange(v, dtype=np.float64) for v in n), indexing="ij"), -1).reshape(-1, 3)
+    for i in range(nt[0]):
+        for j in range(nt[1]):
+            for k in range(nt[2]):
+                lo = np.array([i * n[0], j * n[1], k * n[2]], dtype=np.float64)
+                pos = np.repeat(cells, ppc, 0) + lo + rng.random((len(cells) * ppc, 3))
+                parts = []
+                for s in range(2):
+                    ux, uy, uz = sample_boosted_juttner_synge(len(pos), theta, beta=0, gen=rng)
+                    parts.append(np.concatenate([pos, np.stack([ux, uy, uz], 1)], 1))
+                F = np.zeros((9,) + n)
+                F[5] = B0
+                tiles[(i, j, k)] = (F, parts)
+    return nt, n, [q0, q1], [1.0, 1.0], tiles
+
+
+def run_config0_gpu(inputs, laps, execution):
+    import torch
+
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from gpu_common import build_gpu, canonical_lap
+
+    nt, n, Q, M, tiles = inputs
+    conf, grid, sim = build_gpu(nt, n, Q, M, tiles, execution=execution)
+    npart = grid.world.total_particles()
+    step = canonical_lap(sim)
+    sim.for_one_lap(step)  # warm-up (includes the lap-0 sort)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(laps):
+        sim.for_one_lap(step)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    return dict(particle_steps_per_s=npart * laps / dt, ms_per_step=1e3 * dt / laps, laps=laps, npart=npart)
+
+
+def run_cpu_baseline(inputs, seconds):
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+
+    nt, n, Q, M, tiles = inputs
+    w = oracle.OracleWorld(nt, n, 0.45, Q, M, prec=32)
+    for idx, (F, parts) in tiles.items():
+        t = w.tile_index(*idx)
+        w.set_fields(t, F)
+        for s, st in enumerate(parts):
+            w.add_particles(t, s, st)
+    w.comm_local("emf_E")
+    w.comm_local("emf_B")
+    npart = sum(w.num_alive(t, s) for t in range(w.ntiles) for s in range(2))
+    w.lap(0)  # warm-up
+    laps, t0 = 0, time.perf_counter()
+    while True:
+        w.lap(laps + 1)
+        laps += 1
+        if time.perf_counter() - t0 > seconds or laps >= 200:
+            break
+    dt = time.perf_counter() - t0
+    return dict(value=npart * laps / dt, unit="particle-steps/s", cores=oracle.max_threads(), kind="port",
+                sample=f"oracle FP32 C++/OpenMP restatement, 64^3 cells in 4^3 tiles, {npart} particles, "
+                       f"{laps} full laps ({dt:.1f} s), same step order as the GPU run")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--cells", type=int, default=512)
+    ap.add_argument("--tile", type=int, default=32)
+    ap.add_argument("--ppc", type=int, default=16)
+    ap.add_argument("--mig-fraction", type=float, default=0.08)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-plain", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        sys.exit("bench.py measures one GPU: every box here has a single MI355X, and this build has no "
+                 "multi-GPU path (--gpus must be 1)")
+
+    res = run_turbulence(args)
+    log(f"[bench] batched turbulence: {res['ms_per_step']:.2f} ms/step, fused kernel {res['fused_ms']:.2f} ms, "
+        f"{res['achieved_gbs']:.0f} GB/s algorithmic")
+    value = res["npart"] * res["steps"] / res["wall"]
+
+    extra = {}
+    inputs = None
+    if not args.no_plain or not args.no_cpu_baseline:
+        inputs = config0_inputs()
+    if not args.no_plain:
+        b = run_config0_gpu(inputs, laps=20, execution="batched")
+        p = run_config0_gpu(inputs, laps=5, execution="plain")
+        extra["batched_vs_plain_64cubed"] = dict(batched=b, plain_per_tile=p,
+                                                 ratio=b["particle_steps_per_s"] / p["particle_steps_per_s"])
+        log(f"[bench] 64^3: batched {b['particle_steps_per_s']:.3e}, plain {p['particle_steps_per_s']:.3e} p-s/s")
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = run_cpu_baseline(inputs, args.cpu_seconds)
+        log(f"[bench] cpu baseline {cpu['value']:.3e} particle-steps/s on {cpu['cores']} threads")
+
+    line = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": res["ms_per_step"],
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded on-GPU initial conditions)",
+        "config": {
+            "workload": f"3D decaying turbulence, {args.cells}^3 cells in {args.tile}^3-cell tiles, "
+                        f"{args.ppc} particles/cell/species, theta 0.3, sigma 10, 8x8 dB modes, batched form",
+            "particles": res["npart"],
+            "cells": res["ncell"],
+            "brick": res["brick"],
+        },
+        "roofline": {
+            "bound": "hbm",
+            "kernel": "k_fused (interpolation + Boris + migration classify + zigzag deposit)",
+            "achieved": res["achieved_gbs"],
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": res["achieved_gbs"] / HBM_PEAK_GBS,
+            "traffic": None,
+            "algorithmic_bytes_per_launch": res["alg_bytes"],
+            "avg_launch_ms": res["fused_ms"],
+        },
+        "cpu_baseline": cpu,
+        "fdtd_cell_updates_per_s": res["cell_updates_per_s"],
+        "kernel_ms": res["prof"],
+        "error_counters": res["error_counters"],
+    }
+    line.update(extra)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

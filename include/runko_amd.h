@@ -127,6 +127,20 @@ int rk_sort_particles(rk_world* w, int32_t tile, void* stream);
  * tile.  No counterpart in the reference (its driver issues those four). */
 int rk_particle_step_fused(rk_world* w, void* stream);
 
+/* E += cfl curl B followed by E -= J in one pass (same two roundings per
+ * node as rk_push_e then rk_add_current).  Batched form of that pair. */
+int rk_push_e_add_current(rk_world* w, int32_t tile, void* stream);
+
+/* Kernel timing: when enabled, every launch is bracketed by HIP events on its
+ * stream, accumulated per category; rk_profile_read (synchronous) returns and
+ * clears the totals.  Categories: 0 fused particle step kernel, 1 push,
+ * 2 deposit, 3 pack, 4 migration append, 5 fdtd curl (+E -= J), 6 filter,
+ * 7 ghost copy/fold, 8 other (fills, counters, sort).  Not for use while a
+ * stream is being captured into a graph. */
+#define RK_PROF_NCAT 9
+int rk_profile_enable(rk_world* w, int32_t enable);
+int rk_profile_read(rk_world* w, double* ms, int64_t* launches);
+
 /* corgi local communication for one comm_mode (runko/simulation.py:292-293).
  * Fields: periodic ghost copy (E, B, J) or ghost fold-add (J_exchange).
  * pic_particle: append every pending tile-leaver record to its destination

@@ -71,6 +71,9 @@ def lib():
         f = getattr(L, name)
         f.argtypes = [vp, i32, vp]
     L.rk_particle_step_fused.argtypes = [vp, vp]
+    L.rk_push_e_add_current.argtypes = [vp, i32, vp]
+    L.rk_profile_enable.argtypes = [vp, i32]
+    L.rk_profile_read.argtypes = [vp, vp, vp]
     L.rk_comm_local.argtypes = [vp, i32, vp]
     L.rk_inject.argtypes = [vp, i32, i32, i64, P3, P3, vp, vp]
     L.rk_pending_overflow.argtypes = [vp, ctypes.POINTER(i64), vp]
@@ -104,5 +107,8 @@ EXPORTED = [
     "rk_push_particles", "rk_deposit_current", "rk_pack_outgoing_particles", "rk_sort_particles",
     "rk_particle_step_fused", "rk_comm_local", "rk_inject", "rk_pending_overflow", "rk_bin_demand",
     "rk_repack", "rk_retry_append", "rk_tile_particle_count", "rk_tile_particles", "rk_outgoing",
-    "rk_energies", "rk_gauss_residual", "rk_error_counters",
+    "rk_energies", "rk_gauss_residual", "rk_error_counters", "rk_push_e_add_current", "rk_profile_enable",
+    "rk_profile_read",
 ]
+
+PROF_CATEGORIES = ["fused_particle_step", "push", "deposit", "pack", "append", "fdtd", "filter", "ghost", "other"]

@@ -186,6 +186,10 @@ int rk_world_create(rk_world** out, const rk_world_desc* d)
 int rk_world_destroy(rk_world* w)
 {
   if(!w) return RK_OK;
+  for(auto& r : w->prof) {
+    (void)hipEventDestroy(r.a);
+    (void)hipEventDestroy(r.b);
+  }
   (void)hipFree(w->counters);
   (void)hipFree(w->red);
   (void)hipFree(w->ovf_list);
@@ -255,6 +259,7 @@ int rk_push_half_b(rk_world* w, int32_t tile, void* stream)
   if(int rc = check_tile(w, tile)) return rc;
   int r0[3], r1[3];
   interior(w, tile, r0, r1);
+  ProfScope ps(w, 5, S(stream));
   return launch_curl_b(w, r0, r1, static_cast<float>(w->d.cfl / 2), S(stream));
 }
 
@@ -263,7 +268,18 @@ int rk_push_e(rk_world* w, int32_t tile, void* stream)
   if(int rc = check_tile(w, tile)) return rc;
   int r0[3], r1[3];
   interior(w, tile, r0, r1);
+  ProfScope ps(w, 5, S(stream));
   return launch_curl_e(w, r0, r1, static_cast<float>(w->d.cfl), false, S(stream));
+}
+
+int rk_push_e_add_current(rk_world* w, int32_t tile, void* stream)
+{
+  if(int rc = check_tile(w, tile)) return rc;
+  close_j(w);
+  int r0[3], r1[3];
+  interior(w, tile, r0, r1);
+  ProfScope ps(w, 5, S(stream));
+  return launch_curl_e(w, r0, r1, static_cast<float>(w->d.cfl), true, S(stream));
 }
 
 int rk_add_current(rk_world* w, int32_t tile, void* stream)
@@ -272,6 +288,7 @@ int rk_add_current(rk_world* w, int32_t tile, void* stream)
   close_j(w);
   int r0[3], r1[3];
   interior(w, tile, r0, r1);
+  ProfScope ps(w, 5, S(stream));
   return launch_add_current(w, r0, r1, S(stream));
 }
 
@@ -287,7 +304,10 @@ int rk_filter_current(rk_world* w, int32_t tile, void* stream)
   if(tile < 0) {
     if(w->filter_open)
       return fail(RK_ESTATE, "filter_current: a per-tile filter pass is open; finish it on every tile first");
-    if(int rc = launch_filter(w, r0, r1, S(stream))) return rc;
+    {
+      ProfScope ps(w, 6, S(stream));
+      if(int rc = launch_filter(w, r0, r1, S(stream))) return rc;
+    }
     swap();
     return RK_OK;
   }
@@ -300,7 +320,10 @@ int rk_filter_current(rk_world* w, int32_t tile, void* stream)
     return fail(RK_ESTATE,
                 "filter_current: tile filtered twice in one pass (every tile of the grid must be filtered once "
                 "per pass)");
-  if(int rc = launch_filter(w, r0, r1, S(stream))) return rc;
+  {
+    ProfScope ps(w, 6, S(stream));
+    if(int rc = launch_filter(w, r0, r1, S(stream))) return rc;
+  }
   w->filter_done[size_t(tile)] = 1;
   if(++w->filter_done_count == w->g.ntiles()) {
     swap();
@@ -313,6 +336,7 @@ int rk_push_particles(rk_world* w, int32_t tile, void* stream)
 {
   if(int rc = check_tile(w, tile)) return rc;
   if(int rc = check_particles(w)) return rc;
+  ProfScope ps(w, 1, S(stream));
   return launch_push(w, tile, S(stream));
 }
 
@@ -323,9 +347,11 @@ int rk_deposit_current(rk_world* w, int32_t tile, void* stream)
   if(w->filter_open) return fail(RK_ESTATE, "deposit_current during an open filter pass");
   if(!w->j_accumulating) {
     // clear_current + zeroed generated J: J = 0 on the whole lattice (tile.c++:369-398)
+    ProfScope ps(w, 8, S(stream));
     if(int rc = launch_fill(w->J, w->g.nodes(), 0.0f, S(stream))) return rc;
     w->j_accumulating = true;
   }
+  ProfScope ps(w, 2, S(stream));
   return launch_deposit(w, tile, S(stream));
 }
 
@@ -335,10 +361,16 @@ int rk_pack_outgoing_particles(rk_world* w, int32_t tile, void* stream)
   if(int rc = check_particles(w)) return rc;
   if(!w->mig[0].meta || !w->mig[1].meta) return fail(RK_EINVAL, "migration buffers not set");
   hipStream_t s = S(stream);
-  if(int rc = launch_pack(w, tile, s)) return rc;
+  {
+    ProfScope ps(w, 3, s);
+    if(int rc = launch_pack(w, tile, s)) return rc;
+  }
   // brick changes inside the tile are settled at once
-  if(int rc = launch_append(w, 1, s)) return rc;
-  if(int rc = launch_reset_counter(w, C_MIG1, s)) return rc;
+  {
+    ProfScope ps(w, 4, s);
+    if(int rc = launch_append(w, 1, s)) return rc;
+    if(int rc = launch_reset_counter(w, C_MIG1, s)) return rc;
+  }
   if(int rc = check_lost(w, s)) return rc;
   uint32_t ovf = 0;
   if(int rc = read_counter(w, C_OVF, ovf, s)) return rc;
@@ -350,6 +382,7 @@ int rk_sort_particles(rk_world* w, int32_t tile, void* stream)
 {
   if(int rc = check_tile(w, tile)) return rc;
   if(int rc = check_particles(w)) return rc;
+  ProfScope ps(w, 8, S(stream));
   return launch_sort(w, tile, S(stream));
 }
 
@@ -360,9 +393,16 @@ int rk_particle_step_fused(rk_world* w, void* stream)
   if(!w->g.periodic) return fail(RK_EINVAL, "the fused particle step needs a periodic tile grid");
   if(w->filter_open) return fail(RK_ESTATE, "particle step during an open filter pass");
   hipStream_t s = S(stream);
-  if(int rc = launch_fill(w->J, w->g.nodes(), 0.0f, s)) return rc;
+  {
+    ProfScope ps(w, 8, s);
+    if(int rc = launch_fill(w->J, w->g.nodes(), 0.0f, s)) return rc;
+  }
   w->j_accumulating = true;
-  if(int rc = launch_fused(w, s)) return rc;
+  {
+    ProfScope ps(w, 0, s);
+    if(int rc = launch_fused(w, s)) return rc;
+  }
+  ProfScope ps(w, 4, s);
   if(int rc = launch_append(w, 0, s)) return rc;
   if(int rc = launch_append(w, 1, s)) return rc;
   if(int rc = launch_reset_counter(w, C_MIG0, s)) return rc;
@@ -374,6 +414,7 @@ int rk_comm_local(rk_world* w, int32_t mode, void* stream)
   if(int rc = check_tile(w, -1)) return rc;
   if(!w->g.periodic) return fail(RK_EINVAL, "comm_local needs a tile grid (stand-alone tile)");
   hipStream_t s = S(stream);
+  ProfScope ps(w, mode == RK_COMM_PIC_PARTICLE ? 4 : 7, s);
   switch(mode) {
     case RK_COMM_EMF_E: return launch_ghost_copy(w, w->E, s);
     case RK_COMM_EMF_B: return launch_ghost_copy(w, w->B, s);
@@ -554,6 +595,35 @@ int rk_error_counters(rk_world* w, int64_t* out2, void* stream)
   if(int rc = read_counter(w, C_MIGLOST, b, S(stream))) return rc;
   out2[0] = a;
   out2[1] = b;
+  return RK_OK;
+}
+
+int rk_profile_enable(rk_world* w, int32_t enable)
+{
+  if(!w) return fail(RK_EINVAL, "null world");
+  w->prof_on = enable != 0;
+  return RK_OK;
+}
+
+int rk_profile_read(rk_world* w, double* ms, int64_t* launches)
+{
+  if(!w) return fail(RK_EINVAL, "null world");
+  for(auto& r : w->prof) {
+    RK_HIP(hipEventSynchronize(r.b));
+    float t = 0.0f;
+    RK_HIP(hipEventElapsedTime(&t, r.a, r.b));
+    w->prof_ms[r.cat] += double(t);
+    w->prof_n[r.cat] += 1;
+    (void)hipEventDestroy(r.a);
+    (void)hipEventDestroy(r.b);
+  }
+  w->prof.clear();
+  for(int c = 0; c < RK_PROF_NCAT; ++c) {
+    if(ms) ms[c] = w->prof_ms[c];
+    if(launches) launches[c] = w->prof_n[c];
+    w->prof_ms[c] = 0;
+    w->prof_n[c] = 0;
+  }
   return RK_OK;
 }
 

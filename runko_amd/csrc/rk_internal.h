@@ -132,6 +132,15 @@ struct rk_world {
   double* red = nullptr;          // reduction partials
   int64_t red_cap = 0;
   uint32_t* host_counters = nullptr;  // pinned mirror
+  // kernel timing (rk_profile_*)
+  bool prof_on = false;
+  struct ProfRec {
+    int cat;
+    hipEvent_t a, b;
+  };
+  std::vector<ProfRec> prof;
+  double prof_ms[RK_PROF_NCAT] = {};
+  int64_t prof_n[RK_PROF_NCAT] = {};
   // host-side call-sequence state
   bool j_accumulating = false;  // a deposit generation is open (J holds partial sums)
   bool filter_open = false;
@@ -142,6 +151,30 @@ struct rk_world {
 namespace rk {
 
 void set_error(const std::string& msg);
+
+// Brackets the launches issued in its lifetime with events when timing is on.
+struct ProfScope {
+  rk_world* w;
+  int cat;
+  hipStream_t s;
+  hipEvent_t a = nullptr;
+  ProfScope(rk_world* w_, int cat_, hipStream_t s_) : w(w_), cat(cat_), s(s_)
+  {
+    if(w->prof_on && hipEventCreate(&a) == hipSuccess) (void)hipEventRecord(a, s);
+    else a = nullptr;
+  }
+  ~ProfScope()
+  {
+    if(!a) return;
+    hipEvent_t b = nullptr;
+    if(hipEventCreate(&b) == hipSuccess) {
+      (void)hipEventRecord(b, s);
+      w->prof.push_back({cat, a, b});
+    } else {
+      (void)hipEventDestroy(a);
+    }
+  }
+};
 int hip_fail(hipError_t e, const char* where);
 
 #define RK_HIP(call)                                       \

@@ -1,0 +1,36 @@
+"""The canonical PIC lap of projects/pic-turbulence/pic.py:187-221 in batched
+form on one world: one launch per method over every tile, the particle phase
+(push, pack_outgoing, comm pic_particle, sort, deposit_current) as the fused
+particle step, E += cfl curl B and E -= J in one pass.  It issues the same
+kernels as Simulation in "batched" mode, without the lap-function dispatch.
+"""
+
+from .tools import comm_mode
+
+
+class BatchedLap:
+    def __init__(self, world, filter_passes=3):
+        self.w = world
+        self.filter_passes = filter_passes
+
+    def __call__(self):
+        w = self.w
+        w.call("rk_push_half_b", -1)
+        w.comm_local(int(comm_mode.emf_B))
+        w.particle_step_fused()
+        w.comm_local(int(comm_mode.emf_J_exchange))
+        w.comm_local(int(comm_mode.emf_J))
+        for p in range(self.filter_passes):
+            w.call("rk_filter_current", -1)
+            if p == 0 and self.filter_passes > 1:
+                w.comm_local(int(comm_mode.emf_J))
+        w.call("rk_push_half_b", -1)
+        w.comm_local(int(comm_mode.emf_B))
+        w.call("rk_push_e_add_current", -1)
+        w.comm_local(int(comm_mode.emf_E))
+
+    def settle(self):
+        """Synchronous check after a lap: bins that overflowed are repacked and
+        their pending particles appended (none are lost)."""
+        if self.w.pending_overflow():
+            self.w.settle_overflow()
