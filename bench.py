@@ -169,7 +169,7 @@ def main():
     ap.add_argument("--cells", type=int, default=512)
     ap.add_argument("--tile", type=int, default=32)
     ap.add_argument("--ppc", type=int, default=16)
-    ap.add_argument("--mig-fraction", type=float, default=0.08)
+    ap.add_argument("--mig-fraction", type=float, default=0.01)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-plain", action="store_true")

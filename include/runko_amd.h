@@ -108,6 +108,20 @@ int rk_world_set_species(rk_world* w, int32_t s, int64_t capacity, float* const 
 int rk_world_set_migration(rk_world* w, int32_t set, int64_t capacity, float* const pos[3],
                            float* const vel[3], uint64_t* id, uint32_t* meta);
 
+/* Borrow the per-brick mailboxes of the fused particle step: per_brick records
+ * for every brick (same SoA record layout as above).  A brick writes its
+ * leavers at consecutive offsets of its own mailbox, so the fused step needs
+ * no global atomics to place them; leavers past per_brick spill into the
+ * record sets. */
+int rk_world_set_mailbox(rk_world* w, int64_t per_brick, float* const pos[3], float* const vel[3],
+                         uint64_t* id, uint32_t* meta);
+
+/* Borrow the current staging buffer of the fused particle step: one slot of
+ * stride floats per brick (stride >= 3 * (br+3)^3 for brick shape br), where
+ * every brick leaves its current window; a gather pass then sums the windows
+ * into J.  Required by rk_particle_step_fused. */
+int rk_world_set_current_staging(rk_world* w, float* buf, int64_t stride);
+
 /* ---- emf::Tile methods (src/runko/emf/tile.c++:358-426) ---- */
 int rk_push_half_b(rk_world* w, int32_t tile, void* stream);   /* B += cfl/2 curl E, fdtd2 */
 int rk_push_e(rk_world* w, int32_t tile, void* stream);        /* E += cfl curl B, fdtd2 */

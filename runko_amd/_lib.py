@@ -66,6 +66,8 @@ def lib():
     L.rk_world_j_pointer.restype = vp
     L.rk_world_set_species.argtypes = [vp, i32, i64, P3, P3, vp, vp, vp, vp]
     L.rk_world_set_migration.argtypes = [vp, i32, i64, P3, P3, vp, vp]
+    L.rk_world_set_mailbox.argtypes = [vp, i64, P3, P3, vp, vp]
+    L.rk_world_set_current_staging.argtypes = [vp, vp, i64]
     for name in ["rk_push_half_b", "rk_push_e", "rk_add_current", "rk_filter_current", "rk_push_particles",
                  "rk_deposit_current", "rk_pack_outgoing_particles", "rk_sort_particles"]:
         f = getattr(L, name)
@@ -108,7 +110,7 @@ EXPORTED = [
     "rk_particle_step_fused", "rk_comm_local", "rk_inject", "rk_pending_overflow", "rk_bin_demand",
     "rk_repack", "rk_retry_append", "rk_tile_particle_count", "rk_tile_particles", "rk_outgoing",
     "rk_energies", "rk_gauss_residual", "rk_error_counters", "rk_push_e_add_current", "rk_profile_enable",
-    "rk_profile_read",
+    "rk_profile_read", "rk_world_set_mailbox", "rk_world_set_current_staging",
 ]
 
 PROF_CATEGORIES = ["fused_particle_step", "push", "deposit", "pack", "append", "fdtd", "filter", "ghost", "other"]

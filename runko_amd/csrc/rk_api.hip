@@ -7,6 +7,8 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <new>
 #include <string>
 
@@ -121,6 +123,25 @@ const char* rk_last_error(void) { return rk::g_err.c_str(); }
 
 int rk_version(void) { return 1; }
 
+// x/cfl through a reciprocal and an FMA correction is used only where it was
+// checked bit-identical to the division for every float input (cached per cfl)
+static int fast_division_ok(float cfl, float rcp, int& ok)
+{
+  static std::mutex mu;
+  static std::map<uint32_t, int> cache;
+  uint32_t key;
+  std::memcpy(&key, &cfl, sizeof key);
+  std::lock_guard<std::mutex> lk(mu);
+  auto it = cache.find(key);
+  if(it == cache.end()) {
+    uint64_t bad = 0;
+    if(int rc = verify_fast_division(cfl, rcp, &bad)) return rc;
+    it = cache.emplace(key, bad == 0 ? 1 : 0).first;
+  }
+  ok = it->second;
+  return RK_OK;
+}
+
 int rk_world_create(rk_world** out, const rk_world_desc* d)
 {
   if(!out || !d) return fail(RK_EINVAL, "null argument");
@@ -156,6 +177,15 @@ int rk_world_create(rk_world** out, const rk_world_desc* d)
   g.nbricks = nb * g.ntiles();
   g.periodic = d->periodic;
   g.cfl = float(d->cfl);
+  g.rcp_cfl = 1.0f / g.cfl;
+  g.fast_div = 0;
+  for(int a = 0; a < 3; ++a) {
+    if(g.n[a] >= (1 << 16)) {
+      delete w;
+      return fail(RK_EINVAL, "at most 65535 cells per tile and dimension");
+    }
+    g.br_magic[a] = g.br[a] > 1 ? uint32_t(((uint64_t(1) << 32) + uint64_t(g.br[a]) - 1) / uint64_t(g.br[a])) : 0u;
+  }
   if(int64_t(g.L[0]) * g.L[1] * g.L[2] >= (int64_t(1) << 32)) {
     delete w;
     return fail(RK_EINVAL, "world lattice too large for 32-bit node indices");
@@ -179,6 +209,10 @@ int rk_world_create(rk_world** out, const rk_world_desc* d)
     return hip_fail(e, "rk_world_create");
   }
   w->filter_done.assign(size_t(g.ntiles()), 0);
+  if(int rc = fast_division_ok(g.cfl, g.rcp_cfl, g.fast_div)) {
+    rk_world_destroy(w);
+    return rc;
+  }
   *out = w;
   return RK_OK;
 }
@@ -193,6 +227,7 @@ int rk_world_destroy(rk_world* w)
   (void)hipFree(w->counters);
   (void)hipFree(w->red);
   (void)hipFree(w->ovf_list);
+  (void)hipFree(w->mbox_count);
   delete w;
   return RK_OK;
 }
@@ -235,7 +270,7 @@ int rk_world_set_migration(rk_world* w, int32_t set, int64_t capacity, float* co
                            uint64_t* id, uint32_t* meta)
 {
   if(!w || set < 0 || set > 1) return fail(RK_EINVAL, "invalid record set");
-  if(capacity >= (int64_t(1) << 31)) return fail(RK_EINVAL, "record set capacity must be < 2^31");
+  if(capacity >= (int64_t(1) << 30)) return fail(RK_EINVAL, "record set capacity must be < 2^30");
   auto& m = w->mig[set];
   m.capacity = capacity;
   for(int d = 0; d < 3; ++d) {
@@ -244,11 +279,50 @@ int rk_world_set_migration(rk_world* w, int32_t set, int64_t capacity, float* co
   }
   m.id = id;
   m.meta = meta;
-  const int64_t need = w->mig[0].capacity + w->mig[1].capacity;
+  const int64_t need = w->mig[0].capacity + w->mig[1].capacity + w->mbox.capacity;
   if(need > w->ovf_cap) {
     (void)hipFree(w->ovf_list);
     w->ovf_list = nullptr;
     RK_HIP(hipMalloc(reinterpret_cast<void**>(&w->ovf_list), sizeof(uint32_t) * size_t(std::max<int64_t>(need, 1))));
+    w->ovf_cap = need;
+  }
+  return RK_OK;
+}
+
+int rk_world_set_current_staging(rk_world* w, float* buf, int64_t stride)
+{
+  if(!w) return fail(RK_EINVAL, "null world");
+  const int64_t need = 3 * int64_t(w->g.br[0] + 3) * (w->g.br[1] + 3) * (w->g.br[2] + 3);
+  if(!buf || stride < need) return fail(RK_EINVAL, "current staging: null buffer or stride below 3*(br+3)^3");
+  w->jstage = buf;
+  w->jstage_stride = stride;
+  return RK_OK;
+}
+
+int rk_world_set_mailbox(rk_world* w, int64_t per_brick, float* const pos[3], float* const vel[3], uint64_t* id,
+                         uint32_t* meta)
+{
+  if(!w) return fail(RK_EINVAL, "null world");
+  const int64_t cap = per_brick * int64_t(w->g.nbricks);
+  if(per_brick <= 0 || cap >= (int64_t(1) << 30)) return fail(RK_EINVAL, "mailbox capacity must be in (0, 2^30)");
+  auto& m = w->mbox;
+  m.capacity = cap;
+  for(int d = 0; d < 3; ++d) {
+    m.pos[d] = pos[d];
+    m.vel[d] = vel[d];
+  }
+  m.id = id;
+  m.meta = meta;
+  w->mbox_per_brick = per_brick;
+  if(!w->mbox_count) {
+    RK_HIP(hipMalloc(reinterpret_cast<void**>(&w->mbox_count), sizeof(uint32_t) * size_t(w->g.nbricks)));
+    RK_HIP(hipMemset(w->mbox_count, 0, sizeof(uint32_t) * size_t(w->g.nbricks)));
+  }
+  const int64_t need = w->mig[0].capacity + w->mig[1].capacity + w->mbox.capacity;
+  if(need > w->ovf_cap) {
+    (void)hipFree(w->ovf_list);
+    w->ovf_list = nullptr;
+    RK_HIP(hipMalloc(reinterpret_cast<void**>(&w->ovf_list), sizeof(uint32_t) * size_t(need)));
     w->ovf_cap = need;
   }
   return RK_OK;
@@ -391,6 +465,8 @@ int rk_particle_step_fused(rk_world* w, void* stream)
   if(int rc = check_tile(w, -1)) return rc;
   if(int rc = check_particles(w)) return rc;
   if(!w->g.periodic) return fail(RK_EINVAL, "the fused particle step needs a periodic tile grid");
+  if(!w->mbox.meta || !w->mig[0].meta || !w->mig[1].meta || !w->jstage)
+    return fail(RK_EINVAL, "fused particle step: mailbox, migration or current staging buffers not set");
   if(w->filter_open) return fail(RK_ESTATE, "particle step during an open filter pass");
   hipStream_t s = S(stream);
   {
@@ -402,7 +478,12 @@ int rk_particle_step_fused(rk_world* w, void* stream)
     ProfScope ps(w, 0, s);
     if(int rc = launch_fused(w, s)) return rc;
   }
+  {
+    ProfScope ps(w, 2, s);
+    if(int rc = launch_current_gather(w, s)) return rc;
+  }
   ProfScope ps(w, 4, s);
+  if(int rc = launch_append_mailbox(w, s)) return rc;
   if(int rc = launch_append(w, 0, s)) return rc;
   if(int rc = launch_append(w, 1, s)) return rc;
   if(int rc = launch_reset_counter(w, C_MIG0, s)) return rc;

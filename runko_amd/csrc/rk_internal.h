@@ -56,6 +56,19 @@ struct Geo {
   float wmin[3];  // periodic wrap box of the full grid (global coordinates)
   float wmax[3];
   float cfl;
+  float rcp_cfl;      // RN(1/cfl)
+  int fast_div;       // 1: x/cfl evaluated as q = x*r; q += fma(-q, cfl, x)*r (verified
+                      //    bit-identical to IEEE division for all 2^32 float inputs at world creation)
+  uint32_t br_magic[3];  // ceil(2^32 / br) for br > 1: c / br == umulhi(c, magic) for 0 <= c < 2^16
+
+  __host__ __device__ int div_br(int c, int d) const
+  {
+#ifdef __HIP_DEVICE_COMPILE__
+    return br[d] == 1 ? c : int(__umulhi(uint32_t(c), br_magic[d]));
+#else
+    return c / br[d];
+#endif
+  }
 
   __host__ __device__ int64_t nodes() const { return int64_t(L[0]) * L[1] * L[2]; }
   __host__ __device__ uint32_t at(int i, int j, int k) const
@@ -125,9 +138,19 @@ struct rk_world {
   float* Jf[3] {};
   rk::Species sp[RK_MAX_SPECIES];
   rk::MigSet mig[2];
+  // per-brick current windows of the fused step: brick b writes its
+  // 3 x (br+3)^3 partial current at jstage + b*jstage_stride, a gather kernel
+  // sums the overlapping windows into J
+  float* jstage = nullptr;
+  int64_t jstage_stride = 0;
+  // per-brick mailboxes of the fused particle step: brick b owns records
+  // [b*mbox_per_brick, (b+1)*mbox_per_brick) of mbox, mbox_count[b] used
+  rk::MigSet mbox;
+  int64_t mbox_per_brick = 0;
+  uint32_t* mbox_count = nullptr;
   // library scratch
   uint32_t* counters = nullptr;   // rk::C_NUM
-  uint32_t* ovf_list = nullptr;   // overflowed record refs: (set << 31) | index
+  uint32_t* ovf_list = nullptr;   // overflowed record refs: (set << 30) | index, set 2 = mailbox
   int64_t ovf_cap = 0;
   double* red = nullptr;          // reduction partials
   int64_t red_cap = 0;
@@ -197,6 +220,9 @@ int launch_push(rk_world* w, int tile, hipStream_t s);
 int launch_deposit(rk_world* w, int tile, hipStream_t s);
 int launch_pack(rk_world* w, int tile, hipStream_t s);
 int launch_append(rk_world* w, int set, hipStream_t s);
+int launch_append_mailbox(rk_world* w, hipStream_t s);
+int launch_current_gather(rk_world* w, hipStream_t s);
+int verify_fast_division(float cfl, float rcp, uint64_t* mismatches);
 int launch_reset_counter(rk_world* w, int which, hipStream_t s);
 int launch_sort(rk_world* w, int tile, hipStream_t s);
 int launch_fused(rk_world* w, hipStream_t s);

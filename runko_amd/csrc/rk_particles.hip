@@ -51,12 +51,15 @@ struct FieldPtrs {
   float* J[3];
 };
 
+// record sets: 0 tile leavers, 1 brick changes, 2 per-brick mailboxes
 struct MigPtrs {
-  float* pos[2][3];
-  float* vel[2][3];
-  uint64_t* id[2];
-  uint32_t* meta[2];
-  int64_t cap[2];
+  float* pos[3][3];
+  float* vel[3][3];
+  uint64_t* id[3];
+  uint32_t* meta[3];
+  int64_t cap[3];
+  int64_t per_brick;   // mailbox records per brick (set 2)
+  uint32_t* mcount;    // mailbox fill per brick
 };
 
 SpParams make_sp(const rk_world* w)
@@ -92,15 +95,18 @@ FieldPtrs make_fields(const rk_world* w)
 MigPtrs make_mig(const rk_world* w)
 {
   MigPtrs m {};
-  for(int s = 0; s < 2; ++s) {
+  for(int s = 0; s < 3; ++s) {
+    const MigSet& src = s < 2 ? w->mig[s] : w->mbox;
     for(int d = 0; d < 3; ++d) {
-      m.pos[s][d] = w->mig[s].pos[d];
-      m.vel[s][d] = w->mig[s].vel[d];
+      m.pos[s][d] = src.pos[d];
+      m.vel[s][d] = src.vel[d];
     }
-    m.id[s] = w->mig[s].id;
-    m.meta[s] = w->mig[s].meta;
-    m.cap[s] = w->mig[s].capacity;
+    m.id[s] = src.id;
+    m.meta[s] = src.meta;
+    m.cap[s] = src.capacity;
   }
+  m.per_brick = w->mbox_per_brick;
+  m.mcount = w->mbox_count;
   return m;
 }
 
@@ -143,7 +149,7 @@ __device__ inline int brick_in_tile(const Geo& g, int tile, const float origo[3]
   for(int d = 0; d < 3; ++d) {
     int c = int(floorf(p[d] - origo[d])) - HALO;
     c = c < 0 ? 0 : (c >= g.n[d] ? g.n[d] - 1 : c);
-    lb[d] = c / g.br[d];
+    lb[d] = g.div_br(c, d);
   }
   return tile * g.bpt + (lb[0] * g.nb[1] + lb[1]) * g.nb[2] + lb[2];
 }
@@ -267,9 +273,23 @@ __device__ inline float dot3(const float a[3], const float b[3])
   return r;
 }
 
-// particle_boris.h:39-59, expression order as written there
-__device__ inline void boris(float cfl, float qm, const float E[3], const float B[3], float pos[3], float vel[3])
+// x / cfl, correctly rounded: either the division itself or, when the world
+// verified it bit-identical for all 2^32 float inputs x, one multiply by
+// RN(1/cfl) followed by one FMA remainder correction (Markstein's scheme)
+__device__ inline float div_cfl(const Geo& g, float x)
 {
+  if(g.fast_div) {
+    const float q = x * g.rcp_cfl;
+    const float r = fmaf(-q, g.cfl, x);
+    return fmaf(r, g.rcp_cfl, q);
+  }
+  return x / g.cfl;
+}
+
+// particle_boris.h:39-59, expression order as written there
+__device__ inline void boris(const Geo& g, float qm, const float E[3], const float B[3], float pos[3], float vel[3])
+{
+  const float cfl = g.cfl;
   const float hq = 0.5f * qm;
   float u0[3], E0[3];
 #pragma unroll
@@ -281,7 +301,7 @@ __device__ inline void boris(float cfl, float qm, const float E[3], const float 
   const float bfac = hq * ginv;
   float B0[3];
 #pragma unroll
-  for(int c = 0; c < 3; ++c) B0[c] = (B[c] * bfac) / cfl;
+  for(int c = 0; c < 3; ++c) B0[c] = div_cfl(g, B[c] * bfac);
   const float f = 2.0f / (1.0f + dot3(B0, B0));
   float u1[3];
   u1[0] = (u0[0] + (u0[1] * B0[2] - u0[2] * B0[1])) * f;
@@ -292,7 +312,7 @@ __device__ inline void boris(float cfl, float qm, const float E[3], const float 
   u2[1] = (u0[1] + (-u1[0] * B0[2] + u1[2] * B0[0])) + E0[1];
   u2[2] = (u0[2] + (u1[0] * B0[1] - u1[1] * B0[0])) + E0[2];
 #pragma unroll
-  for(int c = 0; c < 3; ++c) vel[c] = u2[c] / cfl;
+  for(int c = 0; c < 3; ++c) vel[c] = div_cfl(g, u2[c]);
   const float ginv2 = cfl / sqrtf(cfl * cfl + dot3(u2, u2));
 #pragma unroll
   for(int c = 0; c < 3; ++c) pos[c] = pos[c] + (vel[c] * ginv2) * cfl;
@@ -371,14 +391,17 @@ struct CurWin {
   __device__ int vol() const { return D[0] * D[1] * D[2]; }
 };
 
-__device__ inline void lds_add(float* p, float v)
+// The brick's current accumulates in FP64 in LDS: on gfx950 ds_add_f64 runs
+// about 9x faster than ds_add_f32, and the FP64 sums are exact to far below
+// float32 rounding; each node is rounded to float once, when it leaves LDS.
+__device__ inline void lds_add(double* p, float v)
 {
-  __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+  __hip_atomic_fetch_add(p, double(v), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
 // Deposit one particle.  toffh: world haloed index of the owning tile's
 // haloed 0; shift: periodic image shift (world cells) applied to fit the window.
-__device__ inline void deposit_particle(const Geo& g, const FieldPtrs& f, const CurWin& C, float* ldsJ, const Zig& z,
+__device__ inline void deposit_particle(const Geo& g, const FieldPtrs& f, const CurWin& C, double* ldsJ, const Zig& z,
                                         const int toffh[3], uint32_t* err)
 {
   int base1[3], base2[3];
@@ -429,18 +452,26 @@ __device__ inline void deposit_particle(const Geo& g, const FieldPtrs& f, const 
   }
 }
 
-__device__ inline void flush_current(const Geo& g, const FieldPtrs& f, const CurWin& C, const float* ldsJ)
+// plain per-tile deposit: add the brick's window into J with global atomics
+__device__ inline void flush_current(const Geo& g, const FieldPtrs& f, const CurWin& C, const double* ldsJ)
 {
   const int V = C.vol();
-  for(int q = int(threadIdx.x); q < V; q += WG) {
+  for(int q = int(threadIdx.x); q < V; q += int(blockDim.x)) {
     const int lk = q % C.D[2], lj = (q / C.D[2]) % C.D[1], li = q / (C.D[2] * C.D[1]);
     const uint32_t c = g.at(C.d0[0] + li, C.d0[1] + lj, C.d0[2] + lk);
 #pragma unroll
     for(int comp = 0; comp < 3; ++comp) {
-      const float v = ldsJ[comp * V + q];
+      const float v = float(ldsJ[comp * V + q]);
       if(v != 0.0f) unsafeAtomicAdd(&f.J[comp][c], v);
     }
   }
+}
+
+// fused step: store the brick's window (rounded to float) to its staging slot
+__device__ inline void stage_current(const CurWin& C, const double* ldsJ, float* stage)
+{
+  const int V3 = 3 * C.vol();
+  for(int q = int(threadIdx.x); q < V3; q += int(blockDim.x)) stage[q] = float(ldsJ[q]);
 }
 
 // ---------------------------------------------------------------------------
@@ -451,15 +482,17 @@ __device__ inline int wave_id() { return int(threadIdx.x) >> 6; }
 
 __device__ inline uint64_t lanemask_lt() { return (uint64_t(1) << lane_id()) - 1; }
 
-// exclusive block scan of a 0/1 flag; returns rank, writes total
+// exclusive block scan of a 0/1 flag over NT lanes; returns rank, writes total
+template<int NT = WG>
 __device__ inline int block_rank(bool flag, int* wcnt, int& total)
 {
+  constexpr int NW = NT / 64;
   const uint64_t bal = __ballot(flag);
   if(lane_id() == 0) wcnt[wave_id()] = __popcll(bal);
   __syncthreads();
   int off = 0, tot = 0;
 #pragma unroll
-  for(int wv = 0; wv < NWAVE; ++wv) {
+  for(int wv = 0; wv < NW; ++wv) {
     const int c = wcnt[wv];
     off += (wv < wave_id()) ? c : 0;
     tot += c;
@@ -494,16 +527,19 @@ __device__ inline void write_record(const MigPtrs& m, int set, uint32_t slot, co
 }
 
 // LDS for bin compaction
-struct CompactLds {
+template<int NT = WG>
+struct CompactLdsT {
   uint32_t holes[MAX_HOLES];
   uint32_t tailbits[MAX_HOLES / 32];
-  int wcnt[NWAVE];
+  int wcnt[NT / 64];
   int nholes;
 };
+using CompactLds = CompactLdsT<WG>;
 
 // Swap-remove compaction of one bin after the leavers' slots were recorded in
 // L.holes (ascending) and their ids set to DEAD_ID.
-__device__ void compact_bin(const SpParams& sp, int s, int64_t start, int count, int nholes, CompactLds& L,
+template<int NT = WG>
+__device__ void compact_bin(const SpParams& sp, int s, int64_t start, int count, int nholes, CompactLdsT<NT>& L,
                             uint32_t* counters)
 {
   const int newc = count - nholes;
@@ -513,20 +549,20 @@ __device__ void compact_bin(const SpParams& sp, int s, int64_t start, int count,
   uint64_t* ID = sp.id[s];
   if(nholes <= MAX_HOLES) {
     // tail bitmap of holes at index >= newc
-    for(int q = int(threadIdx.x); q < (nholes + 31) / 32; q += WG) L.tailbits[q] = 0;
+    for(int q = int(threadIdx.x); q < (nholes + 31) / 32; q += NT) L.tailbits[q] = 0;
     __syncthreads();
-    for(int q = int(threadIdx.x); q < nholes; q += WG) {
+    for(int q = int(threadIdx.x); q < nholes; q += NT) {
       const uint32_t h = L.holes[q];
       if(int(h) >= newc) atomicOr(&L.tailbits[(int(h) - newc) >> 5], 1u << ((int(h) - newc) & 31));
     }
     __syncthreads();
     int moved = 0;
-    for(int base = 0; base < nholes; base += WG) {
+    for(int base = 0; base < nholes; base += NT) {
       const int t = base + int(threadIdx.x);
       bool stay = false;
       if(t < nholes) stay = !((L.tailbits[t >> 5] >> (t & 31)) & 1u);
       int tot;
-      const int r = block_rank(stay, L.wcnt, tot);
+      const int r = block_rank<NT>(stay, L.wcnt, tot);
       if(stay) {
         const int64_t src = start + newc + t;
         const int64_t dst = start + int64_t(L.holes[moved + r]);
@@ -543,7 +579,7 @@ __device__ void compact_bin(const SpParams& sp, int s, int64_t start, int count,
     // slow path: order-preserving in-place compaction over DEAD_ID marks
     if(threadIdx.x == 0) atomicAdd(&counters[C_SLOWCOMPACT], 1u);
     int wr = 0;
-    for(int base = 0; base < count; base += WG) {
+    for(int base = 0; base < count; base += NT) {
       const int t = base + int(threadIdx.x);
       bool alive = false;
       float p[3], v[3];
@@ -560,7 +596,7 @@ __device__ void compact_bin(const SpParams& sp, int s, int64_t start, int count,
         }
       }
       int tot;
-      const int r = block_rank(alive, L.wcnt, tot);
+      const int r = block_rank<NT>(alive, L.wcnt, tot);
       if(alive) {
         const int64_t dst = start + wr + r;
 #pragma unroll
@@ -636,7 +672,7 @@ __global__ void __launch_bounds__(WG)
       float E[3], Bf[3];
       gather(g, f, W, lds, Bk.toffh[0] + ci[0], Bk.toffh[1] + ci[1], Bk.toffh[2] + ci[2], dd, E, Bf,
              &counters[C_STENCIL]);
-      boris(g.cfl, qm, E, Bf, pos, vel);
+      boris(g, qm, E, Bf, pos, vel);
 #pragma unroll
       for(int d = 0; d < 3; ++d) {
         sp.pos[s][d][q] = pos[d];
@@ -649,7 +685,8 @@ __global__ void __launch_bounds__(WG)
 __global__ void __launch_bounds__(WG)
   k_deposit(Geo g, SpParams sp, FieldPtrs f, int bin0, int nbins, uint32_t* counters)
 {
-  extern __shared__ float lds[];
+  extern __shared__ double ldsd[];
+  double* lds = ldsd;
   const int b = bin0 + xcd_brick(int(blockIdx.x), nbins);
   if(b >= bin0 + nbins) return;
   if(bin_total(sp, b) == 0) return;
@@ -661,7 +698,7 @@ __global__ void __launch_bounds__(WG)
     C.D[d] = g.br[d] + 3;
   }
   const int V = C.vol();
-  for(int q = int(threadIdx.x); q < 3 * V; q += WG) lds[q] = 0.0f;
+  for(int q = int(threadIdx.x); q < 3 * V; q += WG) lds[q] = 0.0;
   __syncthreads();
   for(int s = 0; s < sp.nsp; ++s) {
     const int64_t start = sp.start[s][b];
@@ -737,12 +774,82 @@ __global__ void __launch_bounds__(WG)
   }
 }
 
-// append migration records of one set to their destination bins
+// Destination of one migration record.  Tile leavers (kind 0) go to the
+// neighbouring tile in their direction with every coordinate wrapped by the
+// global box; brick changes (kind 1) stay in the source tile.
+struct RecordDest {
+  int s;     // species
+  int db;    // destination bin
+  float p[3];  // position (wrapped for tile leavers)
+};
+
+__device__ inline RecordDest record_dest(const Geo& g, const MigPtrs& m, int st, uint32_t r)
+{
+  RecordDest R;
+  const uint32_t meta = m.meta[st][r];
+  R.s = meta_species(meta);
+  const int kind = meta_kind(meta);
+  const int srcb = int(meta_bin(meta));
+  int tile = srcb / g.bpt;
+#pragma unroll
+  for(int d = 0; d < 3; ++d) R.p[d] = m.pos[st][d][r];
+  if(kind == 0) {
+    int tc[3];
+    g.tile_coords(tile, tc);
+    const int dir = meta_dir(meta);
+    const int dd[3] = {dir / 9 - 1, (dir / 3) % 3 - 1, dir % 3 - 1};
+#pragma unroll
+    for(int d = 0; d < 3; ++d) {
+      tc[d] = (tc[d] + dd[d] + g.nt[d]) % g.nt[d];
+      R.p[d] = wrap_coord(R.p[d], g.wmin[d], g.wmax[d]);
+    }
+    tile = g.tile_lin(tc[0], tc[1], tc[2]);
+  }
+  int tc[3];
+  g.tile_coords(tile, tc);
+  float origo[3];
+#pragma unroll
+  for(int d = 0; d < 3; ++d) origo[d] = g.tile_min(tc[d], d) - 3.0f;
+  R.db = brick_in_tile(g, tile, origo, R.p);
+  return R;
+}
+
+// Store a record at slot `slot` of its destination bin, or, past the bin's
+// capacity, keep it (position wrapped, re-tagged as a brick change into db)
+// on the overflow list for the retry after a repack.
+__device__ inline void place_record(const SpParams& sp, const MigPtrs& m, int st, uint32_t r, const RecordDest& R,
+                                    uint32_t slot, uint32_t* counters, uint32_t* ovf, int64_t ovf_cap)
+{
+  if(slot < sp.cap[R.s][R.db]) {
+    const int64_t q = sp.start[R.s][R.db] + slot;
+#pragma unroll
+    for(int d = 0; d < 3; ++d) {
+      sp.pos[R.s][d][q] = R.p[d];
+      sp.vel[R.s][d][q] = m.vel[st][d][r];
+    }
+    sp.id[R.s][q] = m.id[st][r];
+  } else {
+#pragma unroll
+    for(int d = 0; d < 3; ++d) m.pos[st][d][r] = R.p[d];
+    m.meta[st][r] = meta_pack(R.s, 1, 13, uint32_t(R.db));
+    const uint32_t o = atomicAdd(&counters[C_OVF], 1u);
+    if(int64_t(o) < ovf_cap) ovf[o] = (uint32_t(st) << 30) | r;
+    else atomicAdd(&counters[C_MIGLOST], 1u);
+  }
+}
+
+__device__ inline void append_record(const Geo& g, const SpParams& sp, const MigPtrs& m, int st, uint32_t r,
+                                     uint32_t* counters, uint32_t* ovf, int64_t ovf_cap)
+{
+  const RecordDest R = record_dest(g, m, st, r);
+  place_record(sp, m, st, r, R, atomicAdd(&sp.count[R.s][R.db], 1u), counters, ovf, ovf_cap);
+}
+
+// append the records of set 0 or 1 (list == nullptr) or of an overflow list
 __global__ void __launch_bounds__(WG)
   k_append(Geo g, SpParams sp, MigPtrs m, int set, uint32_t* counters, uint32_t* ovf, int64_t ovf_cap,
            const uint32_t* list, int list_n)
 {
-  // list == nullptr: records [0, counters[set]) ; else: entries of list (set << 31 | index)
   // the counter can exceed the buffer when reservations were refused (those
   // particles stayed in their bins); never read past the record buffer
   const int n = list ? list_n : int(min(int64_t(counters[set == 0 ? C_MIG0 : C_MIG1]), m.cap[set]));
@@ -750,51 +857,77 @@ __global__ void __launch_bounds__(WG)
     int st = set;
     uint32_t r = uint32_t(e);
     if(list) {
-      st = int(list[e] >> 31);
-      r = list[e] & 0x7fffffffu;
+      st = int(list[e] >> 30);
+      r = list[e] & 0x3fffffffu;
     }
-    const uint32_t meta = m.meta[st][r];
-    const int s = meta_species(meta);
-    const int kind = meta_kind(meta);
-    const int srcb = int(meta_bin(meta));
-    int tile = srcb / g.bpt;
-    float p[3] = {m.pos[st][0][r], m.pos[st][1][r], m.pos[st][2][r]};
-    if(kind == 0) {
-      int tc[3];
-      g.tile_coords(tile, tc);
-      const int dir = meta_dir(meta);
-      const int dd[3] = {dir / 9 - 1, (dir / 3) % 3 - 1, dir % 3 - 1};
+    append_record(g, sp, m, st, r, counters, ovf, ovf_cap);
+  }
+}
+
+// Global brick coordinate of bin b along axis d.
+__device__ inline int global_brick(const Geo& g, int b, int d)
+{
+  int tile, lb[3], tc[3];
+  g.brick_coords(b, tile, lb);
+  g.tile_coords(tile, tc);
+  return tc[d] * g.nb[d] + lb[d];
+}
+
+// Append every brick's mailbox, then empty it.  Records of one brick go to
+// the brick itself or its 26 periodic neighbours: per chunk of WG records
+// the slots of each (species, neighbour) pair are ranked with LDS atomics and
+// reserved with one global atomic per pair.
+__global__ void __launch_bounds__(WG)
+  k_append_mailbox(Geo g, SpParams sp, MigPtrs m, int nbins, uint32_t* counters, uint32_t* ovf, int64_t ovf_cap)
+{
+  constexpr int NK = RK_MAX_SPECIES * 27;
+  __shared__ uint32_t cnt[NK];
+  __shared__ uint32_t kbase[NK];
+  __shared__ int kdb[NK];
+  for(int b = int(blockIdx.x); b < nbins; b += int(gridDim.x)) {
+    const int n = int(m.mcount[b]);
+    if(n == 0) continue;  // uniform over the block
+    const uint32_t base = uint32_t(int64_t(b) * m.per_brick);
+    int gsrc[3];
 #pragma unroll
-      for(int d = 0; d < 3; ++d) {
-        tc[d] = (tc[d] + dd[d] + g.nt[d]) % g.nt[d];
-        p[d] = wrap_coord(p[d], g.wmin[d], g.wmax[d]);
+    for(int d = 0; d < 3; ++d) gsrc[d] = global_brick(g, b, d);
+    for(int c0 = 0; c0 < n; c0 += WG) {
+      for(int k = int(threadIdx.x); k < NK; k += WG) cnt[k] = 0;
+      __syncthreads();
+      const int e = c0 + int(threadIdx.x);
+      RecordDest R;
+      int key = -1;
+      uint32_t rank = 0;
+      if(e < n) {
+        R = record_dest(g, m, 2, base + uint32_t(e));
+        int off = 0;
+        bool near = true;
+#pragma unroll
+        for(int d = 0; d < 3; ++d) {
+          const int NB = g.nt[d] * g.nb[d];
+          int df = global_brick(g, R.db, d) - gsrc[d];
+          df = df < 0 ? df + NB : df;
+          const int o = df == 0 ? 0 : (df == 1 ? 1 : (df == NB - 1 ? -1 : 2));
+          near = near && o != 2;
+          off = off * 3 + (o + 1);
+        }
+        if(near) {
+          key = R.s * 27 + off;
+          rank = atomicAdd(&cnt[key], 1u);
+          kdb[key] = R.db;  // every writer of a key stores the same bin
+        }
       }
-      tile = g.tile_lin(tc[0], tc[1], tc[2]);
-    }
-    int tc[3];
-    g.tile_coords(tile, tc);
-    float origo[3];
-#pragma unroll
-    for(int d = 0; d < 3; ++d) origo[d] = g.tile_min(tc[d], d) - 3.0f;
-    const int db = brick_in_tile(g, tile, origo, p);
-    const uint32_t slot = atomicAdd(&sp.count[s][db], 1u);
-    if(slot < sp.cap[s][db]) {
-      const int64_t q = sp.start[s][db] + slot;
-#pragma unroll
-      for(int d = 0; d < 3; ++d) {
-        sp.pos[s][d][q] = p[d];
-        sp.vel[s][d][q] = m.vel[st][d][r];
+      __syncthreads();
+      for(int k = int(threadIdx.x); k < NK; k += WG)
+        if(cnt[k]) kbase[k] = atomicAdd(&sp.count[k / 27][kdb[k]], cnt[k]);
+      __syncthreads();
+      if(e < n) {
+        const uint32_t slot = key >= 0 ? kbase[key] + rank : atomicAdd(&sp.count[R.s][R.db], 1u);
+        place_record(sp, m, 2, base + uint32_t(e), R, slot, counters, ovf, ovf_cap);
       }
-      sp.id[s][q] = m.id[st][r];
-    } else {
-      // keep the record (its position already wrapped) for the retry after a repack
-#pragma unroll
-      for(int d = 0; d < 3; ++d) m.pos[st][d][r] = p[d];
-      m.meta[st][r] = meta_pack(s, 1, 13, uint32_t(db));
-      const uint32_t o = atomicAdd(&counters[C_OVF], 1u);
-      if(int64_t(o) < ovf_cap) ovf[o] = (uint32_t(st) << 31) | r;
-      else atomicAdd(&counters[C_MIGLOST], 1u);
+      __syncthreads();
     }
+    if(threadIdx.x == 0) m.mcount[b] = 0;
   }
 }
 
@@ -804,15 +937,18 @@ __global__ void k_reset(uint32_t* counters, int which)
 }
 
 // Fused batched step of one brick: gather + Boris + classification + zigzag
-// deposit (in the post-migration frame) + record write + compaction.
-__global__ void __launch_bounds__(WG)
-  k_fused(Geo g, SpParams sp, FieldPtrs f, MigPtrs m, int nbins, uint32_t* counters)
+// deposit (in the post-migration frame) + mailbox record + compaction, then
+// the brick's current window goes to its staging slot (no global atomics).
+constexpr int FWG = 512;  // 8 waves per brick
+
+__global__ void __launch_bounds__(FWG) __attribute__((amdgpu_waves_per_eu(4)))
+  k_fused(Geo g, SpParams sp, FieldPtrs f, MigPtrs m, int nbins, uint32_t* counters, float* jstage,
+          int64_t jstage_stride)
 {
-  extern __shared__ float lds[];
-  __shared__ CompactLds L;
+  extern __shared__ double lds_dyn[];
+  __shared__ CompactLdsT<FWG> L;
   const int b = xcd_brick(int(blockIdx.x), nbins);
   if(b >= nbins) return;
-  if(bin_total(sp, b) == 0) return;
   const Brick Bk = brick_of(g, b);
   MeanWin W;
   CurWin C;
@@ -823,26 +959,28 @@ __global__ void __launch_bounds__(WG)
     C.d0[d] = Bk.c0[d] - 1;
     C.D[d] = g.br[d] + 3;
   }
-  float* ldsM = lds;
-  float* ldsJ = lds + 6 * W.vol();
+  double* ldsJ = lds_dyn;
+  float* ldsM = reinterpret_cast<float*>(lds_dyn + 3 * C.vol());
   const int V = C.vol();
-  for(int q = int(threadIdx.x); q < 3 * V; q += WG) ldsJ[q] = 0.0f;
-  stage_means(g, f, W, ldsM);
+  for(int q = int(threadIdx.x); q < 3 * V; q += FWG) ldsJ[q] = 0.0;
+  const bool any = bin_total(sp, b) != 0;
+  if(any) stage_means(g, f, W, ldsM);
   __syncthreads();
-  for(int s = 0; s < sp.nsp; ++s) {
+  int nrec = 0;  // records in this brick's mailbox (uniform over the block)
+  for(int s = 0; any && s < sp.nsp; ++s) {
     const int64_t start = sp.start[s][b];
     const int count = int(sp.count[s][b]);
     if(count == 0) continue;
     const float qm = sp.qm[s];
     const float charge = sp.charge[s];
     int nholes = 0;
-    for(int base = 0; base < count; base += WG) {
+    for(int base = 0; base < count; base += FWG) {
       const int t = base + int(threadIdx.x);
       bool lv0 = false, lv1 = false;
       float pos[3], vel[3];
       uint64_t id = 0;
       int dir = 13;
-      int64_t q = start + t;
+      const int64_t q = start + t;
       if(t < count) {
 #pragma unroll
         for(int d = 0; d < 3; ++d) {
@@ -860,29 +998,26 @@ __global__ void __launch_bounds__(WG)
         float E[3], Bf[3];
         gather(g, f, W, ldsM, Bk.toffh[0] + ci[0], Bk.toffh[1] + ci[1], Bk.toffh[2] + ci[2], dd, E, Bf,
                &counters[C_STENCIL]);
-        boris(g.cfl, qm, E, Bf, pos, vel);
+        boris(g, qm, E, Bf, pos, vel);
         // classification against the tile (pack_outgoing) and the brick
         dir = tile_dir(pos, Bk.tmin, Bk.tmax);
-        int dtile = Bk.tile;
         int dtoffh[3] = {Bk.toffh[0], Bk.toffh[1], Bk.toffh[2]};
         float dorigo[3] = {Bk.origo[0], Bk.origo[1], Bk.origo[2]};
         float pw[3] = {pos[0], pos[1], pos[2]};
         if(dir != 13) {
           lv0 = true;
           const int dv[3] = {dir / 9 - 1, (dir / 3) % 3 - 1, dir % 3 - 1};
-          int tc[3];
 #pragma unroll
           for(int d = 0; d < 3; ++d) {
-            tc[d] = (Bk.tc[d] + dv[d] + g.nt[d]) % g.nt[d];
+            int tc = Bk.tc[d] + dv[d];
+            tc = tc < 0 ? tc + g.nt[d] : (tc >= g.nt[d] ? tc - g.nt[d] : tc);
             pw[d] = wrap_coord(pos[d], g.wmin[d], g.wmax[d]);
-            dtoffh[d] = tc[d] * g.n[d];
-            dorigo[d] = g.tile_min(tc[d], d) - 3.0f;
+            dtoffh[d] = tc * g.n[d];
+            dorigo[d] = g.tile_min(tc, d) - 3.0f;
           }
-          dtile = g.tile_lin(tc[0], tc[1], tc[2]);
         } else {
           lv1 = brick_in_tile(g, Bk.tile, Bk.origo, pos) != b;
         }
-        (void)dtile;
         // zigzag deposit in the frame of the tile that owns the particle after migration
         const float x2[3] = {pw[0] - dorigo[0], pw[1] - dorigo[1], pw[2] - dorigo[2]};
         const Zig z = zigzag_setup(x2, vel, charge, g.cfl);
@@ -890,21 +1025,28 @@ __global__ void __launch_bounds__(WG)
         // records keep the unwrapped position; the append applies the wrap
         if(lv0 || lv1) id = sp.id[s][q];
       }
-      uint32_t slot0 = wave_reserve(lv0, &counters[C_MIG0]);
-      uint32_t slot1 = wave_reserve(lv1, &counters[C_MIG1]);
-      if(lv0 && int64_t(slot0) >= m.cap[0]) {
-        lv0 = false;
-        atomicAdd(&counters[C_MIGLOST], 1u);
-      }
-      if(lv1 && int64_t(slot1) >= m.cap[1]) {
-        lv1 = false;
-        atomicAdd(&counters[C_MIGLOST], 1u);
-      }
+      // leavers take consecutive slots of this brick's mailbox (rank from a
+      // block scan: no global atomics); past its capacity they spill into
+      // the shared record sets
       const bool hole = lv0 || lv1;
+      int tot;
+      const int r = block_rank<FWG>(hole, L.wcnt, tot);
+      const int64_t local = int64_t(nrec) + r;
+      const bool spill = hole && local >= m.per_brick;
+      const uint32_t slot0 = wave_reserve(spill && lv0, &counters[C_MIG0]);
+      const uint32_t slot1 = wave_reserve(spill && lv1, &counters[C_MIG1]);
+      bool kept = false;
+      if(spill && ((lv0 && int64_t(slot0) >= m.cap[0]) || (lv1 && int64_t(slot1) >= m.cap[1]))) {
+        // no room anywhere: the particle stays in this bin (pushed and
+        // deposited); it is classified again next step
+        kept = true;
+        atomicAdd(&counters[C_MIGLOST], 1u);
+      }
       if(t < count) {
-        if(lv0) write_record(m, 0, slot0, pos, vel, id, meta_pack(s, 0, dir, uint32_t(b)));
-        if(lv1) write_record(m, 1, slot1, pos, vel, id, meta_pack(s, 1, 13, uint32_t(b)));
-        if(hole) {
+        const uint32_t meta = lv0 ? meta_pack(s, 0, dir, uint32_t(b)) : meta_pack(s, 1, 13, uint32_t(b));
+        if(hole && !spill) write_record(m, 2, uint32_t(int64_t(b) * m.per_brick + local), pos, vel, id, meta);
+        else if(spill && !kept) write_record(m, lv0 ? 0 : 1, lv0 ? slot0 : slot1, pos, vel, id, meta);
+        if(hole && !kept) {
           sp.id[s][q] = DEAD_ID;
         } else {
 #pragma unroll
@@ -914,18 +1056,74 @@ __global__ void __launch_bounds__(WG)
           }
         }
       }
-      int tot;
-      const int r = block_rank(hole, L.wcnt, tot);
-      if(hole && nholes + r < MAX_HOLES) L.holes[nholes + r] = uint32_t(t);
-      nholes += tot;
+      // holes: every leaver that left the bin; kept particles are not holes
+      const bool hole2 = hole && !kept;
+      int tot2;
+      const int r2 = block_rank<FWG>(hole2, L.wcnt, tot2);
+      if(hole2 && nholes + r2 < MAX_HOLES) L.holes[nholes + r2] = uint32_t(t);
+      nholes += tot2;
+      nrec += (int64_t(nrec) + tot <= m.per_brick) ? tot : int(m.per_brick - nrec);
     }
     __syncthreads();
-    compact_bin(sp, s, start, count, nholes, L, counters);
+    compact_bin<FWG>(sp, s, start, count, nholes, L, counters);
     __syncthreads();
     if(threadIdx.x == 0) sp.count[s][b] = uint32_t(count - nholes);
     __syncthreads();
   }
-  flush_current(g, f, C, ldsJ);
+  if(threadIdx.x == 0) m.mcount[b] = uint32_t(nrec);
+  __syncthreads();
+  stage_current(C, ldsJ, jstage + int64_t(b) * jstage_stride);
+}
+
+// J on every lattice node += sum of the staged windows of the bricks that
+// cover it, in a fixed order.  The window of global brick gb along one axis
+// covers haloed nodes [2 + gb*br, 4 + (gb+1)*br].
+__global__ void __launch_bounds__(128)
+  k_current_gather(Geo g, const float* __restrict__ stage, int64_t stride, float* J0, float* J1, float* J2)
+{
+  const int Jn = int(blockIdx.x), In = int(blockIdx.y);
+  const int D[3] = {g.br[0] + 3, g.br[1] + 3, g.br[2] + 3};
+  const int V = D[0] * D[1] * D[2];
+  const int Nb[3] = {g.nt[0] * g.nb[0], g.nt[1] * g.nb[1], g.nt[2] * g.nb[2]};
+  // candidate bricks along x and y for this row
+  int bx[4], ox[4], nx = 0, by[4], oy[4], ny = 0;
+  for(int gb = min((In - 2) / g.br[0], Nb[0] - 1); In >= 2 && gb >= 0 && nx < 4; --gb) {
+    const int o = In - (2 + gb * g.br[0]);
+    if(o >= D[0]) break;
+    bx[nx] = gb;
+    ox[nx++] = o;
+  }
+  for(int gb = min((Jn - 2) / g.br[1], Nb[1] - 1); Jn >= 2 && gb >= 0 && ny < 4; --gb) {
+    const int o = Jn - (2 + gb * g.br[1]);
+    if(o >= D[1]) break;
+    by[ny] = gb;
+    oy[ny++] = o;
+  }
+  for(int K = int(threadIdx.x); K < g.L[2]; K += int(blockDim.x)) {
+    float a0 = 0.0f, a1 = 0.0f, a2 = 0.0f;
+    for(int gz = min((K - 2) / g.br[2], Nb[2] - 1); K >= 2 && gz >= 0; --gz) {
+      const int oz = K - (2 + gz * g.br[2]);
+      if(oz >= D[2]) break;
+      const int tz = gz / g.nb[2], lz = gz - tz * g.nb[2];
+      for(int a = 0; a < nx; ++a) {
+        const int tx = bx[a] / g.nb[0], lx = bx[a] - tx * g.nb[0];
+        for(int c = 0; c < ny; ++c) {
+          const int ty = by[c] / g.nb[1], ly = by[c] - ty * g.nb[1];
+          const int bin = g.tile_lin(tx, ty, tz) * g.bpt + (lx * g.nb[1] + ly) * g.nb[2] + lz;
+          const float* w = stage + int64_t(bin) * stride + (ox[a] * D[1] + oy[c]) * D[2] + oz;
+          a0 = a0 + w[0];
+          a1 = a1 + w[V];
+          a2 = a2 + w[2 * V];
+        }
+      }
+    }
+    // J was cleared before the particle step; it holds only the rare
+    // contributions that missed their brick's window (global atomics)
+    const uint32_t q = g.at(In, Jn, K);
+    J0[q] = J0[q] + a0;
+    J1[q] = J1[q] + a1;
+    J2[q] = J2[q] + a2;
+  }
 }
 
 // in-bin counting sort by brick-local cell (layout_right), out-of-brick last
@@ -1092,9 +1290,48 @@ void bin_range(const rk_world* w, int tile, int& b0, int& nb)
 // grid for xcd_brick(): 8 * ceil(n/8) blocks so the permutation covers [0, n)
 unsigned xcd_grid(int n) { return unsigned((n + 7) / 8 * 8); }
 
+// Compare div_cfl's two forms over every float bit pattern (NaNs of either
+// form count as equal); mismatching lanes are counted once per wave.
+__global__ void __launch_bounds__(256) k_verify_div(float cfl, float rcp, uint32_t* mismatches)
+{
+  const uint64_t stride = uint64_t(gridDim.x) * blockDim.x;
+  uint32_t bad = 0;
+  for(uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < (uint64_t(1) << 32); i += stride) {
+    const float x = __uint_as_float(uint32_t(i));
+    const float a = x / cfl;
+    const float q = x * rcp;
+    const float r = fmaf(-q, cfl, x);
+    const float b = fmaf(r, rcp, q);
+    const bool same = __float_as_uint(a) == __float_as_uint(b) || (a != a && b != b);
+    bad += same ? 0u : 1u;
+  }
+  const uint64_t bal = __ballot(bad != 0);
+  if(bal == 0) return;
+  // wave sum of bad through LDS-free shuffles
+  for(int o = 32; o > 0; o >>= 1) bad += __shfl_down(bad, o);
+  if(lane_id() == 0) atomicAdd(mismatches, bad);
+}
+
 size_t means_lds(const Geo& g) { return sizeof(float) * 6 * size_t(g.br[0] + 2) * (g.br[1] + 2) * (g.br[2] + 2); }
-size_t cur_lds(const Geo& g) { return sizeof(float) * 3 * size_t(g.br[0] + 3) * (g.br[1] + 3) * (g.br[2] + 3); }
+size_t cur_lds(const Geo& g) { return sizeof(double) * 3 * size_t(g.br[0] + 3) * (g.br[1] + 3) * (g.br[2] + 3); }
 }  // namespace
+
+int verify_fast_division(float cfl, float rcp, uint64_t* mismatches)
+{
+  uint32_t* d = nullptr;
+  RK_HIP(hipMalloc(reinterpret_cast<void**>(&d), sizeof(uint32_t)));
+  hipError_t e = hipMemset(d, 0, sizeof(uint32_t));
+  if(e == hipSuccess) {
+    hipLaunchKernelGGL(k_verify_div, dim3(8192), dim3(256), 0, 0, cfl, rcp, d);
+    e = hipGetLastError();
+  }
+  uint32_t h = 0;
+  if(e == hipSuccess) e = hipMemcpy(&h, d, sizeof(uint32_t), hipMemcpyDeviceToHost);
+  (void)hipFree(d);
+  RK_HIP(e);
+  *mismatches = h;
+  return RK_OK;
+}
 
 int launch_push(rk_world* w, int tile, hipStream_t s)
 {
@@ -1134,6 +1371,15 @@ int launch_append(rk_world* w, int set, hipStream_t s)
   return RK_OK;
 }
 
+int launch_append_mailbox(rk_world* w, hipStream_t s)
+{
+  const int grid = std::min(w->g.nbricks, 8192);
+  hipLaunchKernelGGL(k_append_mailbox, dim3(unsigned(grid)), dim3(WG), 0, s, w->g, make_sp(w), make_mig(w),
+                     w->g.nbricks, w->counters, w->ovf_list, w->ovf_cap);
+  RK_HIP(hipGetLastError());
+  return RK_OK;
+}
+
 int launch_reset_counter(rk_world* w, int which, hipStream_t s)
 {
   hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, s, w->counters, which);
@@ -1144,8 +1390,16 @@ int launch_reset_counter(rk_world* w, int which, hipStream_t s)
 int launch_fused(rk_world* w, hipStream_t s)
 {
   const size_t dyn = means_lds(w->g) + cur_lds(w->g);
-  hipLaunchKernelGGL(k_fused, dim3(xcd_grid(w->g.nbricks)), dim3(WG), dyn, s, w->g, make_sp(w), make_fields(w),
-                     make_mig(w), w->g.nbricks, w->counters);
+  hipLaunchKernelGGL(k_fused, dim3(xcd_grid(w->g.nbricks)), dim3(FWG), dyn, s, w->g, make_sp(w), make_fields(w),
+                     make_mig(w), w->g.nbricks, w->counters, w->jstage, w->jstage_stride);
+  RK_HIP(hipGetLastError());
+  return RK_OK;
+}
+
+int launch_current_gather(rk_world* w, hipStream_t s)
+{
+  hipLaunchKernelGGL(k_current_gather, dim3(unsigned(w->g.L[1]), unsigned(w->g.L[0])), dim3(128), 0, s, w->g,
+                     w->jstage, w->jstage_stride, w->J[0], w->J[1], w->J[2]);
   RK_HIP(hipGetLastError());
   return RK_OK;
 }

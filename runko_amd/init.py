@@ -73,6 +73,9 @@ def fill_uniform_plasma(world, ppc, theta, seed=1, chunk_bins=4096):
     gen.manual_seed(int(seed))
     nbd = [world.n[a] // br[a] for a in range(3)]
     cl = torch.stack(torch.meshgrid(*(torch.arange(v, device=dev) for v in br), indexing="ij"), -1).reshape(-1, 3)
+    # a fixed random slot order inside every bin: consecutive particles (and
+    # so the lanes of a wave) sit in different cells
+    slot_perm = torch.randperm(per_bin, generator=gen, device=dev)
     for b0 in range(0, world.nbricks, chunk_bins):
         b1 = min(world.nbricks, b0 + chunk_bins)
         b = torch.arange(b0, b1, device=dev)
@@ -86,7 +89,7 @@ def fill_uniform_plasma(world, ppc, theta, seed=1, chunk_bins=4096):
         nb = b1 - b0
         base = (origin[:, None, None, :] + cl[None, :, None, :]).expand(nb, cells, ppc, 3).to(torch.float64)
         pos = (base + torch.rand((nb, cells, ppc, 3), generator=gen, device=dev, dtype=torch.float64))
-        pos = pos.to(torch.float32).reshape(nb, per_bin, 3)
+        pos = pos.to(torch.float32).reshape(nb, per_bin, 3)[:, slot_perm, :]
         ordinal = (lb[:, None] * per_bin + torch.arange(per_bin, device=dev)[None, :])
         ids = (tile[:, None] << 40) | ordinal
         for s in range(world.nsp):
@@ -140,7 +143,7 @@ def set_turbulent_field(world, B0, n_modes=8, dB_over_B0=1.0, seed=2):
 
 
 def turbulence_world(N=512, tile=32, ppc=16, theta=0.3, sigma=10.0, cfl=0.45, c_omp=10.0, n_modes=8,
-                     dB_over_B0=1.0, seed=1, brick=None, mig_fraction=0.08):
+                     dB_over_B0=1.0, seed=1, brick=None, mig_fraction=0.01, mailbox_fraction=0.12):
     """Decaying-turbulence box: N^3 cells in tile^3 tiles, periodic."""
     nt = N // tile
     q0, q1, B0 = plasma_parameters(cfl, c_omp, ppc, theta, sigma)
@@ -149,6 +152,7 @@ def turbulence_world(N=512, tile=32, ppc=16, theta=0.3, sigma=10.0, cfl=0.45, c_
     set_turbulent_field(w, B0, n_modes=n_modes, dB_over_B0=dB_over_B0, seed=seed + 1)
     fill_uniform_plasma(w, ppc, theta, seed=seed)
     w.ensure_migration(max(1 << 16, int(mig_fraction * w.total_particles())))
+    w.ensure_mailbox(mailbox_fraction)
     w.comm_local(1)
     w.comm_local(2)
     torch.cuda.synchronize()

@@ -87,7 +87,11 @@ class TileGrid:
             world.presize(s, demand)
             world.ensure_migration(max(int(max((p[2].numel() for p in parts), default=0)), 1))
             for t, (pos, vel, ids) in enumerate(parts):
-                world.inject(t, s, pos, vel, ids)
+                # storage order inside a bin is free (the reference's is unpinned
+                # too); a random order keeps the lanes of a wave on different
+                # cells, so their LDS current atomics rarely collide
+                perm = torch.randperm(ids.numel(), device=ids.device)
+                world.inject(t, s, pos[:, perm], vel[:, perm], ids[perm])
         torch.cuda.synchronize()
         for t, tile in enumerate(tiles):
             tile._rebind(world, t)

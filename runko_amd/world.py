@@ -113,6 +113,9 @@ class World:
         self.mig = [None, None]
         self._mig_cap = [0, 0]
         self.ensure_migration(1 << 12)
+        self.mbox = None
+        self._mbox_per_brick = 0
+        self.jstage = None
 
     # ------------------------------------------------------------------ plumbing
     def __del__(self):
@@ -144,6 +147,31 @@ class World:
                                                    rec["id"].data_ptr(), rec["meta"].data_ptr()))
             self.mig[st] = rec
             self._mig_cap[st] = c
+
+    def ensure_mailbox(self, fraction=0.12):
+        """Per-brick mailboxes for the fused particle step, sized to `fraction`
+        of the largest brick's total capacity (all species), and the per-brick
+        current staging slots (3 (br+3)^3 floats each) of the same step."""
+        if self.jstage is None:
+            stride = 3 * (self.brick[0] + 3) * (self.brick[1] + 3) * (self.brick[2] + 3)
+            self.jstage = torch.empty((self.nbricks, stride), dtype=torch.float32, device=self.dev)
+            torch.cuda.synchronize()
+            L.check(L.lib().rk_world_set_current_staging(self.h, self.jstage.data_ptr(), stride))
+        caps = sum(p.cap.to(torch.int64) for p in self.pools)
+        per = int(math.ceil(fraction * int(caps.max().item()))) if self.nsp else 0
+        per = max(64, (per + 31) // 32 * 32)
+        if per <= self._mbox_per_brick:
+            return
+        c = per * self.nbricks
+        rec = dict(pos=torch.empty((3, c), dtype=torch.float32, device=self.dev),
+                   vel=torch.empty((3, c), dtype=torch.float32, device=self.dev),
+                   id=torch.empty((c,), dtype=torch.int64, device=self.dev),
+                   meta=torch.empty((c,), dtype=torch.int32, device=self.dev))
+        torch.cuda.synchronize()
+        L.check(L.lib().rk_world_set_mailbox(self.h, per, L.p3(rec["pos"]), L.p3(rec["vel"]), rec["id"].data_ptr(),
+                                             rec["meta"].data_ptr()))
+        self.mbox = rec
+        self._mbox_per_brick = per
 
     @property
     def ntiles(self):
@@ -191,6 +219,8 @@ class World:
         L.check(rc)
 
     def particle_step_fused(self):
+        if self._mbox_per_brick == 0:
+            self.ensure_mailbox()
         L.check(L.lib().rk_particle_step_fused(self.h, _stream()))
 
     def pending_overflow(self):
@@ -222,6 +252,8 @@ class World:
                                   newp.start.data_ptr(), newp.cap.data_ptr(), _stream()))
         torch.cuda.synchronize()
         self.pools[s] = newp
+        if self._mbox_per_brick:
+            self.ensure_mailbox()
 
     def presize(self, s, demand):
         """Lay out species s with bins sized for `demand` particles per bin

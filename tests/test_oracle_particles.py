@@ -71,10 +71,12 @@ def test_boris_gyration_phase_and_norm(orc):
 
 def test_boris_e_cross_b_drift(orc):
     # uniform E x B: the time-averaged velocity approaches the drift E x B / B^2
-    w = tile_world(orc, n=(24, 24, 24))
+    # the orbit drifts 0.018 cells/step along x for 1200 steps: start near the
+    # low edge of a 32-cell tile so it stays inside (x from 4 to about 26)
+    w = tile_world(orc, n=(32, 24, 24))
     E, B = (0.0, 0.002, 0.0), (0.0, 0.0, 0.05)
     uniform_fields(w, E=E, B=B)
-    w.add_particles(0, 0, np.array([[12.0, 12.0, 12.0, 0.0, 0.0, 0.0]]))
+    w.add_particles(0, 0, np.array([[4.0, 12.0, 12.0, 0.0, 0.0, 0.0]]))
     xs = []
     for _ in range(1200):
         w.push_particles(0)
@@ -82,6 +84,7 @@ def test_boris_e_cross_b_drift(orc):
     vdrift = CFL * E[1] / B[2]  # (E x B)_x / B^2 in code velocity units (x per step)
     # average over whole gyro periods: fit a line to x(t)
     t = np.arange(len(xs))
+    assert 3.0 < min(xs) and max(xs) < 29.0
     slope = np.polyfit(t, np.array(xs), 1)[0]
     assert slope == pytest.approx(vdrift, rel=2e-2)
 
