@@ -161,6 +161,20 @@ def run_cpu_baseline(inputs, seconds):
                        f"{laps} full laps ({dt:.1f} s), same step order as the GPU run")
 
 
+def run_copy_probe(mib):
+    """Plain device copies of a known size: the scale for rocprof's FETCH_SIZE
+    and WRITE_SIZE counters (KiB per dispatch) when they are read for k_fused."""
+    import torch
+
+    n = int(mib) * (1 << 20) // 4
+    src = torch.ones(n, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    log(f"[bench] copy probe: 3 copies of {4 * n} bytes (read {4 * n / 1024:.0f} KiB, write {4 * n / 1024:.0f} KiB each)")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -173,11 +187,15 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="also run plain copies of MIB MiB (scale for rocprof byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         sys.exit("bench.py measures one GPU: every box here has a single MI355X, and this build has no "
                  "multi-GPU path (--gpus must be 1)")
 
+    if args.copy_probe:
+        run_copy_probe(args.copy_probe)
     res = run_turbulence(args)
     log(f"[bench] batched turbulence: {res['ms_per_step']:.2f} ms/step, fused kernel {res['fused_ms']:.2f} ms, "
         f"{res['achieved_gbs']:.0f} GB/s algorithmic")

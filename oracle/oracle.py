@@ -50,6 +50,7 @@ def lib():
         L.orc_num_alive.argtypes = [vp, ip, ip]
         L.orc_num_alive.restype = ctypes.c_long
         L.orc_get_particles.argtypes = [vp, ip, ip, vp, vp]
+        L.orc_interpolate.argtypes = [vp, ip, ip, vp]
         L.orc_num_outgoing.argtypes = [vp, ip]
         L.orc_num_outgoing.restype = ctypes.c_long
         L.orc_get_outgoing.argtypes = [vp, ip, vp, vp, vp, vp]
@@ -145,6 +146,12 @@ class OracleWorld:
         ids = np.empty(n, dtype=np.uint64)
         _check(lib().orc_get_particles(self._h, tile, s, out.ctypes.data, ids.ctypes.data))
         return out, ids
+
+    def interpolate(self, tile, s):
+        """E and B ([alive, 6] float64) interpolated at the alive particles."""
+        out = np.zeros((self.num_alive(tile, s), 6))
+        _check(lib().orc_interpolate(self._h, tile, s, out.ctypes.data))
+        return out
 
     def get_outgoing(self, tile):
         n = lib().orc_num_outgoing(self._h, tile)

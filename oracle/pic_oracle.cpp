@@ -1014,6 +1014,31 @@ int orc_get_particles(void* hp, int tile, int s, double* out, std::uint64_t* ids
   }); });
 }
 
+// interpolated fields at the alive particles of (tile, s): out [alive][6]
+// (Ex Ey Ez Bx By Bz) float64, the values push_particles feeds to Boris
+int orc_interpolate(void* hp, int tile, int s, double* out)
+{
+  auto* h = static_cast<Handle*>(hp);
+  return guard([&] { dispatch(h, [&](auto& w) { WT;
+    const auto& t = w.tiles.at(std::size_t(tile));
+    const auto& c = t.sp.at(std::size_t(s));
+    T origo[3];
+    tile_origo(t, origo);
+    std::size_t q = 0;
+    for(std::size_t p = 0; p < c.size(); ++p) {
+      if(c.id[p] == DEAD) continue;
+      const T pos[3] = {c.x[0][p], c.x[1][p], c.x[2][p]};
+      T E[3], B[3];
+      interpolate(t.lat, origo, pos, E, B);
+      for(int d = 0; d < 3; ++d) {
+        out[6 * q + std::size_t(d)] = static_cast<double>(E[d]);
+        out[6 * q + 3 + std::size_t(d)] = static_cast<double>(B[d]);
+      }
+      ++q;
+    }
+  }); });
+}
+
 // outgoing buffer after pack: count, then records [cnt][6], ids, dir index
 long orc_num_outgoing(void* hp, int tile)
 {

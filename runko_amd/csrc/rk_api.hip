@@ -228,6 +228,7 @@ int rk_world_destroy(rk_world* w)
   (void)hipFree(w->red);
   (void)hipFree(w->ovf_list);
   (void)hipFree(w->mbox_count);
+  (void)hipFree(w->mig_table);
   delete w;
   return RK_OK;
 }
@@ -286,7 +287,7 @@ int rk_world_set_migration(rk_world* w, int32_t set, int64_t capacity, float* co
     RK_HIP(hipMalloc(reinterpret_cast<void**>(&w->ovf_list), sizeof(uint32_t) * size_t(std::max<int64_t>(need, 1))));
     w->ovf_cap = need;
   }
-  return RK_OK;
+  return upload_mig_table(w);
 }
 
 int rk_world_set_current_staging(rk_world* w, float* buf, int64_t stride)
@@ -325,7 +326,7 @@ int rk_world_set_mailbox(rk_world* w, int64_t per_brick, float* const pos[3], fl
     RK_HIP(hipMalloc(reinterpret_cast<void**>(&w->ovf_list), sizeof(uint32_t) * size_t(need)));
     w->ovf_cap = need;
   }
-  return RK_OK;
+  return upload_mig_table(w);
 }
 
 int rk_push_half_b(rk_world* w, int32_t tile, void* stream)

@@ -148,6 +148,7 @@ struct rk_world {
   rk::MigSet mbox;
   int64_t mbox_per_brick = 0;
   uint32_t* mbox_count = nullptr;
+  void* mig_table = nullptr;  // device copy of the record-set pointers (upload_mig_table)
   // library scratch
   uint32_t* counters = nullptr;   // rk::C_NUM
   uint32_t* ovf_list = nullptr;   // overflowed record refs: (set << 30) | index, set 2 = mailbox
@@ -223,6 +224,7 @@ int launch_append(rk_world* w, int set, hipStream_t s);
 int launch_append_mailbox(rk_world* w, hipStream_t s);
 int launch_current_gather(rk_world* w, hipStream_t s);
 int verify_fast_division(float cfl, float rcp, uint64_t* mismatches);
+int upload_mig_table(rk_world* w);
 int launch_reset_counter(rk_world* w, int which, hipStream_t s);
 int launch_sort(rk_world* w, int tile, hipStream_t s);
 int launch_fused(rk_world* w, hipStream_t s);

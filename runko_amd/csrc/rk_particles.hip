@@ -399,6 +399,11 @@ __device__ inline void lds_add(double* p, float v)
   __hip_atomic_fetch_add(p, double(v), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
+__device__ inline void lds_add_d(double* p, double v)
+{
+  __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
 // Deposit one particle.  toffh: world haloed index of the owning tile's
 // haloed 0; shift: periodic image shift (world cells) applied to fit the window.
 __device__ inline void deposit_particle(const Geo& g, const FieldPtrs& f, const CurWin& C, double* ldsJ, const Zig& z,
@@ -478,27 +483,39 @@ __device__ inline void stage_current(const CurWin& C, const double* ldsJ, float*
 // block-wide helpers
 // ---------------------------------------------------------------------------
 __device__ inline int lane_id() { return int(threadIdx.x) & 63; }
+
+__device__ inline int64_t readfirstlane64(int64_t v)
+{
+  const uint32_t lo = __builtin_amdgcn_readfirstlane(uint32_t(uint64_t(v)));
+  const uint32_t hi = __builtin_amdgcn_readfirstlane(uint32_t(uint64_t(v) >> 32));
+  return int64_t((uint64_t(hi) << 32) | lo);
+}
 __device__ inline int wave_id() { return int(threadIdx.x) >> 6; }
 
 __device__ inline uint64_t lanemask_lt() { return (uint64_t(1) << lane_id()) - 1; }
 
-// exclusive block scan of a 0/1 flag over NT lanes; returns rank, writes total
+// exclusive block scan of a 0/1 flag over NT lanes; returns rank, writes total.
+// The per-wave counts are scanned across lanes 0..NW-1 with DPP row shifts and
+// picked with readlane at the (scalar) wave index: no per-wave select masks.
 template<int NT = WG>
 __device__ inline int block_rank(bool flag, int* wcnt, int& total)
 {
   constexpr int NW = NT / 64;
+  static_assert(NW <= 16, "one DPP row");
   const uint64_t bal = __ballot(flag);
-  if(lane_id() == 0) wcnt[wave_id()] = __popcll(bal);
+  const int wid = __builtin_amdgcn_readfirstlane(int(threadIdx.x) >> 6);
+  if(lane_id() == 0) wcnt[wid] = __popcll(bal);
   __syncthreads();
-  int off = 0, tot = 0;
-#pragma unroll
-  for(int wv = 0; wv < NW; ++wv) {
-    const int c = wcnt[wv];
-    off += (wv < wave_id()) ? c : 0;
-    tot += c;
-  }
+  const int l = lane_id();
+  const int c = l < NW ? wcnt[l] : 0;
+  int x = c;
+  x += __builtin_amdgcn_update_dpp(0, x, 0x111, 0xf, 0xf, true);  // row_shr:1
+  x += __builtin_amdgcn_update_dpp(0, x, 0x112, 0xf, 0xf, true);  // row_shr:2
+  x += __builtin_amdgcn_update_dpp(0, x, 0x114, 0xf, 0xf, true);  // row_shr:4
+  if(NW > 8) x += __builtin_amdgcn_update_dpp(0, x, 0x118, 0xf, 0xf, true);  // row_shr:8
+  const int off = __builtin_amdgcn_readlane(x - c, wid);
+  total = __builtin_amdgcn_readlane(x, NW - 1);
   __syncthreads();
-  total = tot;
   return off + __popcll(bal & lanemask_lt());
 }
 
@@ -936,57 +953,256 @@ __global__ void k_reset(uint32_t* counters, int which)
   if(threadIdx.x == 0) counters[which] = 0;
 }
 
+// ---------------------------------------------------------------------------
+// fused batched step
+// ---------------------------------------------------------------------------
+// Brick shape known at compile time (BR > 0: BR^3 bricks) or read from Geo
+// (BR == 0).  With a compile-time shape every LDS address of the gather and
+// of the deposit is one per-particle base plus immediate offsets.
+template<int BR>
+struct Shape {
+  int b[3];
+  __device__ explicit Shape(const Geo& g)
+  {
+#pragma unroll
+    for(int d = 0; d < 3; ++d) b[d] = BR > 0 ? BR : g.br[d];
+  }
+  __device__ int M(int d) const { return b[d] + 2; }  // node means staged per axis
+  __device__ int D(int d) const { return b[d] + 3; }  // current nodes per axis
+  __device__ int MV() const { return M(0) * M(1) * M(2); }
+  __device__ int DV() const { return D(0) * D(1) * D(2); }
+  __device__ int div(int c, const Geo& g, int d) const { return BR > 0 ? c / BR : g.div_br(c, d); }
+};
+
+// node means of the brick window, one array per component (ex ey ez bx by bz)
+template<int BR, int NT>
+__device__ inline void stage_means6(const Geo& g, const FieldPtrs& f, const Shape<BR>& S, const int m0[3], float* lds)
+{
+  const int V = S.MV();
+  for(int q = int(threadIdx.x); q < V; q += NT) {
+    const int lk = q % S.M(2), lj = (q / S.M(2)) % S.M(1), li = q / (S.M(2) * S.M(1));
+    const Means m = node_means(g, f, m0[0] + li, m0[1] + lj, m0[2] + lk);
+    lds[q] = m.ex;
+    lds[V + q] = m.ey;
+    lds[2 * V + q] = m.ez;
+    lds[3 * V + q] = m.bx;
+    lds[4 * V + q] = m.by;
+    lds[5 * V + q] = m.bz;
+  }
+}
+
+// gather from the staged means; false when a corner lies outside the window
+template<int BR>
+__device__ inline bool gather6(const Shape<BR>& S, const float* lds, int li, int lj, int lk, const float d[3],
+                               float E[3], float B[3])
+{
+  if(!(li >= 0 && lj >= 0 && lk >= 0 && li + 1 < S.M(0) && lj + 1 < S.M(1) && lk + 1 < S.M(2))) return false;
+  const int V = S.MV();
+  const float* p = lds + (li * S.M(1) + lj) * S.M(2) + lk;
+  float v[6][8];
+#pragma unroll
+  for(int c = 0; c < 8; ++c) {
+    const int o = ((c >> 2) * S.M(1) + ((c >> 1) & 1)) * S.M(2) + (c & 1);
+#pragma unroll
+    for(int comp = 0; comp < 6; ++comp) v[comp][c] = p[comp * V + o];
+  }
+  E[0] = lerp3(d, v[0]);
+  E[1] = lerp3(d, v[1]);
+  E[2] = lerp3(d, v[2]);
+  B[0] = lerp3(d, v[3]);
+  B[1] = lerp3(d, v[4]);
+  B[2] = lerp3(d, v[5]);
+  return true;
+}
+
+// the 12 non-zero contributions of a zigzag segment, in the order of
+// particle_current_zigzag_1st.c++:300-336: node offsets (x, y, z bits) and
+// component of each
+struct SegVals {
+  float v[12];
+};
+
+__device__ inline SegVals segment_values(float Fx, float Fy, float Fz, float Wx, float Wy, float Wz)
+{
+  const float one = 1.0f;
+  SegVals r;
+  r.v[0] = Fx * (one - Wy) * (one - Wz);   // x at 000
+  r.v[1] = Fy * (one - Wx) * (one - Wz);   // y at 000
+  r.v[2] = Fz * (one - Wx) * (one - Wy);   // z at 000
+  r.v[3] = Fy * Wx * (one - Wz);           // y at 100
+  r.v[4] = Fz * Wx * (one - Wy);           // z at 100
+  r.v[5] = Fx * Wy * (one - Wz);           // x at 010
+  r.v[6] = Fz * (one - Wx) * Wy;           // z at 010
+  r.v[7] = Fx * (one - Wy) * Wz;           // x at 001
+  r.v[8] = Fy * (one - Wx) * Wz;           // y at 001
+  r.v[9] = Fx * Wy * Wz;                   // x at 011
+  r.v[10] = Fy * Wx * Wz;                  // y at 101
+  r.v[11] = Fz * Wx * Wy;                  // z at 110
+  return r;
+}
+
+// add segment a -- plus segment b when both share their cell -- at window
+// node base p
+template<int BR>
+__device__ inline void emit_window(const Shape<BR>& S, double* p, const SegVals& a, const SegVals& b, bool with_b)
+{
+  const int sx = S.D(1) * S.D(2), sy = S.D(2), V = S.DV();
+  double* px = p;
+  double* py = p + V;
+  double* pz = p + 2 * V;
+  // the two segments are summed in FP32 (one rounding, as one of the
+  // reference's FP32 atomic adds would do), then accumulated in FP64
+  auto val = [&](int k) { return double(with_b ? a.v[k] + b.v[k] : a.v[k]); };
+  lds_add_d(px, val(0));
+  lds_add_d(py, val(1));
+  lds_add_d(pz, val(2));
+  lds_add_d(py + sx, val(3));
+  lds_add_d(pz + sx, val(4));
+  lds_add_d(px + sy, val(5));
+  lds_add_d(pz + sy, val(6));
+  lds_add_d(px + 1, val(7));
+  lds_add_d(py + 1, val(8));
+  lds_add_d(px + sy + 1, val(9));
+  lds_add_d(py + sx + 1, val(10));
+  lds_add_d(pz + sx + sy, val(11));
+}
+
+// Deposit into the brick's current window.  off[d] maps a tile-haloed node
+// index of the owning tile to a window index (periodic image included).
+// False when a node falls outside the window (the caller then uses HBM).
+// A particle that stays in its cell (i1 == i2) adds both segments in one
+// pass; only cell-crossing lanes issue the second pass.
+template<int BR>
+__device__ inline bool deposit_win(const Shape<BR>& S, double* ldsJ, const Zig& z, const int off[3])
+{
+  int a[3], b[3];
+  bool in = true, same = true;
+#pragma unroll
+  for(int d = 0; d < 3; ++d) {
+    a[d] = z.i1[d] + off[d];
+    b[d] = z.i2[d] + off[d];
+    in = in && a[d] >= 0 && b[d] >= 0 && a[d] + 1 < S.D(d) && b[d] + 1 < S.D(d);
+    same = same && z.i1[d] == z.i2[d];
+  }
+  if(!in) return false;
+  const SegVals s1 = segment_values(z.Fx1, z.Fy1, z.Fz1, z.Wx1, z.Wy1, z.Wz1);
+  const SegVals s2 = segment_values(z.Fx2, z.Fy2, z.Fz2, z.Wx2, z.Wy2, z.Wz2);
+  emit_window(S, ldsJ + (a[0] * S.D(1) + a[1]) * S.D(2) + a[2], s1, s2, same);
+  if(!same) emit_window(S, ldsJ + (b[0] * S.D(1) + b[1]) * S.D(2) + b[2], s2, s2, false);
+  return true;
+}
+
+// same contributions straight to HBM (global atomics), for nodes outside the window
+__device__ inline void deposit_global(const Geo& g, const FieldPtrs& f, const Zig& z, const int toffh[3], uint32_t* err)
+{
+  zigzag_emit(z, [&](int i, int j, int k, int mask, float jx, float jy, float jz) {
+    const int I = toffh[0] + i, J = toffh[1] + j, K = toffh[2] + k;
+    if(I < 0 || J < 0 || K < 0 || I >= g.L[0] || J >= g.L[1] || K >= g.L[2]) {
+      atomicAdd(err, 1u);
+      return;
+    }
+    const uint32_t q = g.at(I, J, K);
+    if(mask & 1) unsafeAtomicAdd(&f.J[0][q], jx);
+    if(mask & 2) unsafeAtomicAdd(&f.J[1][q], jy);
+    if(mask & 4) unsafeAtomicAdd(&f.J[2][q], jz);
+  });
+}
+
+// fmodf(x, L) for L > 0; exact shortcut for |x| < 2L (Sterbenz: x -+ L is exact)
+__device__ inline float fmod_near(float x, float L)
+{
+  const float ax = fabsf(x);
+  if(ax < L) return x;
+  if(ax < 2.0f * L) return x - copysignf(L, x);
+  return fmodf(x, L);
+}
+
 // Fused batched step of one brick: gather + Boris + classification + zigzag
 // deposit (in the post-migration frame) + mailbox record + compaction, then
 // the brick's current window goes to its staging slot (no global atomics).
 constexpr int FWG = 512;  // 8 waves per brick
 
+// Per-brick constants of the tile-leaver path, indexed by axis and by the
+// step dv + 1 in {0, 1, 2} towards the destination tile (kept in LDS so they
+// occupy no scalar registers in the particle loop).
+struct NbrTab {
+  float dorig[3][3];  // destination tile lattice origo: tile mins - 3
+  int dtoffh[3][3];   // destination tile's world haloed offset
+  int off[3][3];      // destination tile-haloed node -> window index (unwrapped image)
+  float wlo[3], whi[3], wlen[3];  // periodic box
+};
+
+template<int BR>
 __global__ void __launch_bounds__(FWG) __attribute__((amdgpu_waves_per_eu(4)))
-  k_fused(Geo g, SpParams sp, FieldPtrs f, MigPtrs m, int nbins, uint32_t* counters, float* jstage,
-          int64_t jstage_stride)
+  k_fused(Geo g, SpParams sp, FieldPtrs f, MigPtrs m, const MigPtrs* mtab, int nbins, uint32_t* counters,
+          float* jstage, int64_t jstage_stride)
 {
   extern __shared__ double lds_dyn[];
   __shared__ CompactLdsT<FWG> L;
+  __shared__ NbrTab NT;
   const int b = xcd_brick(int(blockIdx.x), nbins);
   if(b >= nbins) return;
+  const Shape<BR> S(g);
   const Brick Bk = brick_of(g, b);
-  MeanWin W;
-  CurWin C;
+  double* ldsJ = lds_dyn;
+  float* ldsM = reinterpret_cast<float*>(lds_dyn + 3 * S.DV());
+  for(int q = int(threadIdx.x); q < 3 * S.DV(); q += FWG) ldsJ[q] = 0.0;
+  const bool any = bin_total(sp, b) != 0;
+  if(any) stage_means6<BR, FWG>(g, f, S, Bk.c0, ldsM);
+  __syncthreads();
+  // window index of a particle's cell: tile-haloed cell - 3 - lb*br (means),
+  // tile-haloed node - 2 - lb*br (current)
+  int moff[3], coff[3];
 #pragma unroll
   for(int d = 0; d < 3; ++d) {
-    W.m0[d] = Bk.c0[d];
-    W.M[d] = g.br[d] + 2;
-    C.d0[d] = Bk.c0[d] - 1;
-    C.D[d] = g.br[d] + 3;
+    moff[d] = -HALO - Bk.lb[d] * S.b[d];
+    coff[d] = moff[d] + 1;
   }
-  double* ldsJ = lds_dyn;
-  float* ldsM = reinterpret_cast<float*>(lds_dyn + 3 * C.vol());
-  const int V = C.vol();
-  for(int q = int(threadIdx.x); q < 3 * V; q += FWG) ldsJ[q] = 0.0;
-  const bool any = bin_total(sp, b) != 0;
-  if(any) stage_means(g, f, W, ldsM);
+  if(threadIdx.x < 9) {
+    const int d = int(threadIdx.x) / 3, k = int(threadIdx.x) % 3;
+    int tc = Bk.tc[d] + k - 1;
+    tc = tc < 0 ? tc + g.nt[d] : (tc >= g.nt[d] ? tc - g.nt[d] : tc);
+    NT.dorig[d][k] = g.tile_min(tc, d) - 3.0f;
+    NT.dtoffh[d][k] = tc * g.n[d];
+    // the destination's nodes sit next to this tile in the unwrapped frame:
+    // the image nearest to this brick
+    NT.off[d][k] = (k - 1) * g.n[d] - HALO + 1 - Bk.lb[d] * S.b[d];
+    if(k == 0) {
+      NT.wlo[d] = g.wmin[d];
+      NT.whi[d] = g.wmax[d];
+      NT.wlen[d] = g.wmax[d] - g.wmin[d];
+    }
+  }
   __syncthreads();
   int nrec = 0;  // records in this brick's mailbox (uniform over the block)
   for(int s = 0; any && s < sp.nsp; ++s) {
-    const int64_t start = sp.start[s][b];
-    const int count = int(sp.count[s][b]);
+    // bin start and count are uniform: keep them (and the bin's array bases)
+    // scalar so particle addresses are a 32-bit lane offset from a scalar base
+    const int64_t start = readfirstlane64(sp.start[s][b]);
+    const int count = __builtin_amdgcn_readfirstlane(int(sp.count[s][b]));
     if(count == 0) continue;
     const float qm = sp.qm[s];
     const float charge = sp.charge[s];
+    float* const P0 = sp.pos[s][0] + start;
+    float* const P1 = sp.pos[s][1] + start;
+    float* const P2 = sp.pos[s][2] + start;
+    float* const U0 = sp.vel[s][0] + start;
+    float* const U1 = sp.vel[s][1] + start;
+    float* const U2 = sp.vel[s][2] + start;
     int nholes = 0;
     for(int base = 0; base < count; base += FWG) {
       const int t = base + int(threadIdx.x);
       bool lv0 = false, lv1 = false;
       float pos[3], vel[3];
-      uint64_t id = 0;
       int dir = 13;
-      const int64_t q = start + t;
+      const uint32_t q = uint32_t(t);
       if(t < count) {
-#pragma unroll
-        for(int d = 0; d < 3; ++d) {
-          pos[d] = sp.pos[s][d][q];
-          vel[d] = sp.vel[s][d][q];
-        }
+        pos[0] = P0[q];
+        pos[1] = P1[q];
+        pos[2] = P2[q];
+        vel[0] = U0[q];
+        vel[1] = U1[q];
+        vel[2] = U2[q];
         float pr[3], dd[3];
         int ci[3];
 #pragma unroll
@@ -996,34 +1212,41 @@ __global__ void __launch_bounds__(FWG) __attribute__((amdgpu_waves_per_eu(4)))
           dd[d] = pr[d] - float(uint32_t(ci[d]));
         }
         float E[3], Bf[3];
-        gather(g, f, W, ldsM, Bk.toffh[0] + ci[0], Bk.toffh[1] + ci[1], Bk.toffh[2] + ci[2], dd, E, Bf,
-               &counters[C_STENCIL]);
+        if(!gather6(S, ldsM, ci[0] + moff[0], ci[1] + moff[1], ci[2] + moff[2], dd, E, Bf))
+          gather(g, f, MeanWin {{0, 0, 0}, {0, 0, 0}}, nullptr, Bk.toffh[0] + ci[0], Bk.toffh[1] + ci[1],
+                 Bk.toffh[2] + ci[2], dd, E, Bf, &counters[C_STENCIL]);
         boris(g, qm, E, Bf, pos, vel);
         // classification against the tile (pack_outgoing) and the brick
         dir = tile_dir(pos, Bk.tmin, Bk.tmax);
-        int dtoffh[3] = {Bk.toffh[0], Bk.toffh[1], Bk.toffh[2]};
-        float dorigo[3] = {Bk.origo[0], Bk.origo[1], Bk.origo[2]};
-        float pw[3] = {pos[0], pos[1], pos[2]};
+        float x2[3];
+        int off[3], dtoffh[3];
         if(dir != 13) {
           lv0 = true;
-          const int dv[3] = {dir / 9 - 1, (dir / 3) % 3 - 1, dir % 3 - 1};
+          const int k3[3] = {dir / 9, (dir / 3) % 3, dir % 3};
 #pragma unroll
           for(int d = 0; d < 3; ++d) {
-            int tc = Bk.tc[d] + dv[d];
-            tc = tc < 0 ? tc + g.nt[d] : (tc >= g.nt[d] ? tc - g.nt[d] : tc);
-            pw[d] = wrap_coord(pos[d], g.wmin[d], g.wmax[d]);
-            dtoffh[d] = tc * g.n[d];
-            dorigo[d] = g.tile_min(tc, d) - 3.0f;
+            // every coordinate wrapped by the global box, particle.h:523-549
+            const float pw = (pos[d] < 0 ? NT.whi[d] : NT.wlo[d]) + fmod_near(pos[d], NT.wlen[d]);
+            x2[d] = pw - NT.dorig[d][k3[d]];
+            dtoffh[d] = NT.dtoffh[d][k3[d]];
+            off[d] = NT.off[d][k3[d]];
           }
         } else {
-          lv1 = brick_in_tile(g, Bk.tile, Bk.origo, pos) != b;
+          int lb[3];
+#pragma unroll
+          for(int d = 0; d < 3; ++d) {
+            int c = int(floorf(pos[d] - Bk.origo[d])) - HALO;
+            c = c < 0 ? 0 : (c >= g.n[d] ? g.n[d] - 1 : c);
+            lb[d] = S.div(c, g, d);
+            x2[d] = pos[d] - Bk.origo[d];
+            off[d] = coff[d];
+            dtoffh[d] = Bk.toffh[d];
+          }
+          lv1 = lb[0] != Bk.lb[0] || lb[1] != Bk.lb[1] || lb[2] != Bk.lb[2];
         }
         // zigzag deposit in the frame of the tile that owns the particle after migration
-        const float x2[3] = {pw[0] - dorigo[0], pw[1] - dorigo[1], pw[2] - dorigo[2]};
         const Zig z = zigzag_setup(x2, vel, charge, g.cfl);
-        deposit_particle(g, f, C, ldsJ, z, dtoffh, &counters[C_STENCIL]);
-        // records keep the unwrapped position; the append applies the wrap
-        if(lv0 || lv1) id = sp.id[s][q];
+        if(!deposit_win(S, ldsJ, z, off)) deposit_global(g, f, z, dtoffh, &counters[C_STENCIL]);
       }
       // leavers take consecutive slots of this brick's mailbox (rank from a
       // block scan: no global atomics); past its capacity they spill into
@@ -1036,24 +1259,27 @@ __global__ void __launch_bounds__(FWG) __attribute__((amdgpu_waves_per_eu(4)))
       const uint32_t slot0 = wave_reserve(spill && lv0, &counters[C_MIG0]);
       const uint32_t slot1 = wave_reserve(spill && lv1, &counters[C_MIG1]);
       bool kept = false;
-      if(spill && ((lv0 && int64_t(slot0) >= m.cap[0]) || (lv1 && int64_t(slot1) >= m.cap[1]))) {
+      if(spill && ((lv0 && int64_t(slot0) >= mtab->cap[0]) || (lv1 && int64_t(slot1) >= mtab->cap[1]))) {
         // no room anywhere: the particle stays in this bin (pushed and
         // deposited); it is classified again next step
         kept = true;
         atomicAdd(&counters[C_MIGLOST], 1u);
       }
       if(t < count) {
-        const uint32_t meta = lv0 ? meta_pack(s, 0, dir, uint32_t(b)) : meta_pack(s, 1, 13, uint32_t(b));
-        if(hole && !spill) write_record(m, 2, uint32_t(int64_t(b) * m.per_brick + local), pos, vel, id, meta);
-        else if(spill && !kept) write_record(m, lv0 ? 0 : 1, lv0 ? slot0 : slot1, pos, vel, id, meta);
         if(hole && !kept) {
-          sp.id[s][q] = DEAD_ID;
+          // records keep the unwrapped position; the append applies the wrap
+          const uint64_t id = sp.id[s][start + q];
+          const uint32_t meta = lv0 ? meta_pack(s, 0, dir, uint32_t(b)) : meta_pack(s, 1, 13, uint32_t(b));
+          if(!spill) write_record(m, 2, uint32_t(int64_t(b) * m.per_brick + local), pos, vel, id, meta);
+          else write_record(*mtab, lv0 ? 0 : 1, lv0 ? slot0 : slot1, pos, vel, id, meta);
+          sp.id[s][start + q] = DEAD_ID;
         } else {
-#pragma unroll
-          for(int d = 0; d < 3; ++d) {
-            sp.pos[s][d][q] = pos[d];
-            sp.vel[s][d][q] = vel[d];
-          }
+          P0[q] = pos[0];
+          P1[q] = pos[1];
+          P2[q] = pos[2];
+          U0[q] = vel[0];
+          U1[q] = vel[1];
+          U2[q] = vel[2];
         }
       }
       // holes: every leaver that left the bin; kept particles are not holes
@@ -1072,7 +1298,9 @@ __global__ void __launch_bounds__(FWG) __attribute__((amdgpu_waves_per_eu(4)))
   }
   if(threadIdx.x == 0) m.mcount[b] = uint32_t(nrec);
   __syncthreads();
-  stage_current(C, ldsJ, jstage + int64_t(b) * jstage_stride);
+  const int V3 = 3 * S.DV();
+  float* stage = jstage + int64_t(b) * jstage_stride;
+  for(int q = int(threadIdx.x); q < V3; q += FWG) stage[q] = float(ldsJ[q]);
 }
 
 // J on every lattice node += sum of the staged windows of the bricks that
@@ -1389,9 +1617,17 @@ int launch_reset_counter(rk_world* w, int which, hipStream_t s)
 
 int launch_fused(rk_world* w, hipStream_t s)
 {
-  const size_t dyn = means_lds(w->g) + cur_lds(w->g);
-  hipLaunchKernelGGL(k_fused, dim3(xcd_grid(w->g.nbricks)), dim3(FWG), dyn, s, w->g, make_sp(w), make_fields(w),
-                     make_mig(w), w->g.nbricks, w->counters, w->jstage, w->jstage_stride);
+  const Geo& g = w->g;
+  const size_t dyn = means_lds(g) + cur_lds(g);
+  const dim3 grid(xcd_grid(g.nbricks));
+  auto go = [&](auto kern) {
+    hipLaunchKernelGGL(kern, grid, dim3(FWG), dyn, s, g, make_sp(w), make_fields(w), make_mig(w),
+                       static_cast<const MigPtrs*>(w->mig_table), g.nbricks, w->counters, w->jstage,
+                       w->jstage_stride);
+  };
+  if(g.br[0] == 8 && g.br[1] == 8 && g.br[2] == 8) go(k_fused<8>);
+  else if(g.br[0] == 4 && g.br[1] == 4 && g.br[2] == 4) go(k_fused<4>);
+  else go(k_fused<0>);
   RK_HIP(hipGetLastError());
   return RK_OK;
 }
@@ -1513,4 +1749,16 @@ int kinetic_energies(rk_world* w, int tile, double* out, hipStream_t st)
   return RK_OK;
 }
 
+}  // namespace rk
+
+namespace rk {
+// device copy of the record-set pointers (read by rarely taken kernel paths)
+int upload_mig_table(rk_world* w)
+{
+  if(!w->mig_table) RK_HIP(hipMalloc(&w->mig_table, sizeof(MigPtrs)));
+  const MigPtrs m = make_mig(w);
+  RK_HIP(hipDeviceSynchronize());
+  RK_HIP(hipMemcpy(w->mig_table, &m, sizeof(MigPtrs), hipMemcpyHostToDevice));
+  return RK_OK;
+}
 }  // namespace rk

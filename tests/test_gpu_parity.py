@@ -247,3 +247,68 @@ def test_comm_type_error():
         sim.for_one_lap(lambda x: x.comm_local(3))
     with pytest.raises(RuntimeError):
         sim.for_one_lap(lambda x: x.not_a_method())
+
+
+def _gauss_numpy(grid, passes, charges):
+    """div E - F^passes(rho_CIC) in FP64 from the readback (same stencil and
+    summation order as rk_gauss_residual); returns (max, rms rho, rms div E)."""
+    world = grid.world
+    N, n = world.N, world.n
+    rho = np.zeros(N)
+    for s, q in enumerate(charges):
+        for t, tile in enumerate(grid.tiles()):
+            pos = np.stack(tile.get_positions(s), 1).astype(np.float32)
+            if len(pos) == 0:
+                continue
+            origo = np.array([(c * m) for c, m in zip(tile.index, n)], dtype=np.float32) - np.float32(3.0)
+            p = pos - origo  # float32, as the kernel computes it
+            f = np.floor(p)
+            wgt = p.astype(np.float64) - f.astype(np.float64)
+            base = np.array(tile.index) * np.array(n) + f.astype(np.int64) - 3
+            for a in (0, 1):
+                for b in (0, 1):
+                    for c in (0, 1):
+                        wa = (wgt[:, 0] if a else 1.0 - wgt[:, 0]) * (wgt[:, 1] if b else 1.0 - wgt[:, 1]) * \
+                             (wgt[:, 2] if c else 1.0 - wgt[:, 2])
+                        idx = tuple((base[:, d] + (a, b, c)[d]) % N[d] for d in range(3))
+                        np.add.at(rho, idx, q * wa)
+    w1 = (1.0, 2.0, 1.0)
+    for _ in range(passes):
+        acc = np.zeros_like(rho)
+        for a in range(3):
+            for b in range(3):
+                for c in range(3):
+                    acc += w1[a] * w1[b] * w1[c] / 64.0 * np.roll(rho, (1 - a, 1 - b, 1 - c), (0, 1, 2))
+        rho = acc
+    E = world.F[0].double().cpu().numpy()
+    inner = tuple(slice(3, 3 + v) for v in N)
+    div = ((E[0][inner] - E[0][tuple(slice(2, 2 + v) if d == 0 else slice(3, 3 + v) for d, v in enumerate(N))]) +
+           (E[1][inner] - E[1][tuple(slice(2, 2 + v) if d == 1 else slice(3, 3 + v) for d, v in enumerate(N))]) +
+           (E[2][inner] - E[2][tuple(slice(2, 2 + v) if d == 2 else slice(3, 3 + v) for d, v in enumerate(N))]))
+    return np.max(np.abs(div - rho)), np.sqrt(np.mean(rho**2)), np.sqrt(np.mean(div**2))
+
+
+@pytest.mark.parametrize("execution", ["plain", "batched"])
+def test_gauss_residual(execution):
+    # neutral start (species 1 on top of species 0, E = 0): div E - F^3 rho
+    # starts at 0; after laps it stays at the size of the 3D zigzag defect.
+    # The device diagnostic must equal the FP64 numpy restatement.
+    nt, n = (2, 2, 2), (8, 8, 8)
+    inputs = random_inputs(nt, n, 2, seed=12, ppc=4, umax=1.0)
+    for F, parts in inputs.values():
+        F[0:3] = 0.0
+        parts[1][:, :3] = parts[0][:, :3]
+    conf, grid, sim = build_gpu(nt, n, Q, M, inputs, execution=execution)
+    g0 = grid.world.gauss_residual(3)
+    assert g0[0] < 1e-12 and g0[1] < 1e-12
+    step = canonical_lap(sim)
+    for _ in range(6):
+        sim.for_one_lap(step)
+    g = grid.world.gauss_residual(3)
+    h = _gauss_numpy(grid, 3, Q)
+    print(f"gauss residual {execution}: device {g}, numpy {h}")
+    assert g[1] > 0 and g[2] > 0
+    assert abs(g[0] - h[0]) <= 1e-9 * h[1]
+    assert g[1] == pytest.approx(h[1], rel=1e-9)
+    assert g[2] == pytest.approx(h[2], rel=1e-9)
+    assert g[0] < 0.05 * g[1]

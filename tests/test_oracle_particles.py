@@ -24,6 +24,36 @@ def uniform_fields(w, E=(0, 0, 0), B=(0, 0, 0)):
         w.set_fields(t, F)
 
 
+# staggered sample points (tile mins 0): E_x at (i+1/2, j, k), B_x at (i, j+1/2, k+1/2), ...
+_STAGGER = [(0.5, 0, 0), (0, 0.5, 0), (0, 0, 0.5), (0, 0.5, 0.5), (0.5, 0, 0.5), (0.5, 0.5, 0)]
+
+
+@pytest.mark.parametrize("prec,tol", [(32, 2e-5), (64, 1e-12)])
+def test_interpolation_of_affine_fields_is_exact(orc, prec, tol):
+    # node means of a linear field sampled on the Yee stagger equal the field at
+    # the node, and trilinear weights reproduce a linear function: the gathered
+    # value is f(x_p) for every component.  A wrong neighbour in a mean or a
+    # swapped weight shifts it by O(slope / 2).
+    n = (8, 8, 8)
+    w = tile_world(orc, n=n, prec=prec)
+    rng = np.random.default_rng(5)
+    coef = rng.uniform(-0.2, 0.2, (6, 4))
+    coef[:, 0] += 1.0
+    ii, jj, kk = np.meshgrid(*(np.arange(v, dtype=np.float64) for v in n), indexing="ij")
+    F = np.zeros((9,) + n)
+    for c in range(6):
+        sx, sy, sz = _STAGGER[c]
+        F[c] = coef[c, 0] + coef[c, 1] * (ii + sx) + coef[c, 2] * (jj + sy) + coef[c, 3] * (kk + sz)
+    w.set_fields(0, F)
+    # every corner and mean stays inside the interior: cells 1 .. n-2
+    pos = 1.0 + rng.random((500, 3)) * (np.array(n) - 3.0)
+    pos = pos.astype(np.float32).astype(np.float64)
+    w.add_particles(0, 0, np.concatenate([pos, np.zeros_like(pos)], 1))
+    got = w.interpolate(0, 0)
+    want = coef[:, 0][None, :] + pos @ coef[:, 1:].T
+    assert np.max(np.abs(got - want)) < tol
+
+
 @pytest.mark.parametrize("axis", [0, 1, 2])
 def test_boris_uniform_e_response(orc, axis):
     # tests/py/test_pic_particle_pusher.py:42-140: q < 0 at rest in E along one
