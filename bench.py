@@ -45,7 +45,8 @@ def run_turbulence(args):
     N, tile, ppc = args.cells, args.tile, args.ppc
     t0 = time.time()
     w, info = turbulence_world(N=N, tile=tile, ppc=ppc, theta=0.3, sigma=10.0, cfl=0.45, c_omp=10.0, n_modes=8,
-                               dB_over_B0=1.0, seed=1, mig_fraction=args.mig_fraction)
+                               dB_over_B0=1.0, seed=1, mig_fraction=args.mig_fraction,
+                               slot_order=args.slot_order)
     npart = w.total_particles()
     ncell = N ** 3
     log(f"[bench] turbulence {N}^3, tiles {tile}^3, {npart} particles, brick {w.brick}, setup {time.time() - t0:.1f}s")
@@ -74,6 +75,26 @@ def run_turbulence(args):
     prof = {name: {"ms_total": float(ms[i]), "launches": int(cnt[i])} for i, name in enumerate(_lib.PROF_CATEGORIES)}
     assert w.total_particles() == npart, "particle count changed"
     err = w.error_counters()
+    plain = None
+    if args.plain_steps > 0:
+        # the same world advanced in the plain per-tile form (one call per tile
+        # per method, each method synchronous), as the reference driver issues it
+        from runko_amd.stepper import PlainLap
+        w.ensure_migration(int(0.03 * npart))
+        pl = PlainLap(w)
+        pl()
+        pl.settle()
+        log("[bench] plain warm-up step done")
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        for i in range(args.plain_steps):
+            pl()
+            pl.settle()
+        torch.cuda.synchronize()
+        pwall = time.perf_counter() - t1
+        assert w.total_particles() == npart, "particle count changed (plain)"
+        plain = dict(steps=args.plain_steps, ms_per_step=1e3 * pwall / args.plain_steps,
+                     particle_steps_per_s=npart * args.plain_steps / pwall, launches_per_step=11 * w.ntiles)
     fused_ms = ms[0] / max(cnt[0], 1)
     alg_bytes = 48.0 * npart + 36.0 * ncell  # per fused launch: pos/u r+w, E/B staging read, J write
     achieved = alg_bytes / (fused_ms * 1e-3) / 1e9
@@ -83,7 +104,7 @@ def run_turbulence(args):
         ms_per_step=1e3 * wall / args.steps, prof=prof, fused_ms=float(fused_ms), alg_bytes=alg_bytes,
         achieved_gbs=achieved, field_ms_per_step=field_ms / args.steps,
         cell_updates_per_s=ncell * args.steps / (field_ms * 1e-3) if field_ms > 0 else None,
-        error_counters=[int(v) for v in err], brick=list(w.brick), B0=info["B0"])
+        error_counters=[int(v) for v in err], brick=list(w.brick), B0=info["B0"], plain=plain)
 
 
 def config0_inputs(seed=42):
@@ -187,6 +208,10 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--plain-steps", type=int, default=1,
+                    help="steps of the plain per-tile form on the same world (0: skip)")
+    ap.add_argument("--slot-order", default="interleaved", choices=["interleaved", "random"],
+                    help="initial particle order inside a bin")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="also run plain copies of MIB MiB (scale for rocprof byte counters)")
     args = ap.parse_args()
@@ -247,11 +272,22 @@ def main():
             "algorithmic_bytes_per_launch": res["alg_bytes"],
             "avg_launch_ms": res["fused_ms"],
         },
+        "step_roofline": {
+            "algorithmic_bytes_per_step": 48.0 * res["npart"] + 228.0 * res["ncell"],
+            "achieved": (48.0 * res["npart"] + 228.0 * res["ncell"]) / (res["ms_per_step"] * 1e-3) / 1e9,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": (48.0 * res["npart"] + 228.0 * res["ncell"]) / (res["ms_per_step"] * 1e-3) / 1e9 / HBM_PEAK_GBS,
+        },
         "cpu_baseline": cpu,
         "fdtd_cell_updates_per_s": res["cell_updates_per_s"],
         "kernel_ms": res["prof"],
         "error_counters": res["error_counters"],
     }
+    if res["plain"] is not None:
+        line["batched_vs_plain"] = dict(workload=line["config"]["workload"].replace(", batched form", ""),
+                                        batched_ms_per_step=res["ms_per_step"], plain=res["plain"],
+                                        ratio=res["plain"]["ms_per_step"] / res["ms_per_step"])
     line.update(extra)
     print(json.dumps(line), flush=True)
 

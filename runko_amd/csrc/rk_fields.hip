@@ -105,34 +105,60 @@ __global__ void __launch_bounds__(BX* BY)
 }
 
 // 27-tap (1,2,1)^3/64, taps in layout_right order, accumulator starting at 0.
-__global__ void __launch_bounds__(BX* BY)
-  k_filter27(Geo g, Region r, const float* __restrict__ S0, const float* __restrict__ S1,
-             const float* __restrict__ S2, float* __restrict__ D0, float* __restrict__ D1,
-             float* __restrict__ D2)
+// Same 27-tap filter, each lane sweeping a column of FCH outputs along x: the
+// 3x3 (y, z) neighbourhood of three consecutive x planes stays in registers,
+// so every input is loaded once per column instead of three times.  Each
+// output still accumulates its 27 taps in layout_right order from 0.
+constexpr int FCH = 8;
+
+__device__ inline void filter_column(const Geo& g, const float* __restrict__ S, float* __restrict__ D, int i0, int i1,
+                                     int j, int k)
 {
-  const int k = r.r0[2] + int(blockIdx.x) * BX + int(threadIdx.x);
-  const int j = r.r0[1] + int(blockIdx.y) * BY + int(threadIdx.y);
-  const int i = r.r0[0] + int(blockIdx.z);
-  if(k >= r.r1[2] || j >= r.r1[1]) return;
-  const float w[3] = {1.0f, 2.0f, 1.0f};
-  float a0 = 0.0f, a1 = 0.0f, a2 = 0.0f;
+  const uint32_t si = uint32_t(g.L[1]) * uint32_t(g.L[2]), sj = uint32_t(g.L[2]);
+  float P[3][3][3];  // [plane i-1, i, i+1][b][c]
+  auto load = [&](float(&Q)[3][3], int i) {
+    const uint32_t c0 = g.at(i, j, k);
 #pragma unroll
-  for(int a = 0; a < 3; ++a)
+    for(int b = 0; b < 3; ++b)
+#pragma unroll
+      for(int cc = 0; cc < 3; ++cc) Q[b][cc] = S[c0 + uint32_t(b - 1) * sj + uint32_t(cc - 1)];
+  };
+  load(P[0], i0 - 1);
+  load(P[1], i0);
+  (void)si;
+  for(int i = i0; i < i1; ++i) {
+    load(P[2], i + 1);
+    const float w[3] = {1.0f, 2.0f, 1.0f};
+    float acc = 0.0f;
+#pragma unroll
+    for(int a = 0; a < 3; ++a)
+#pragma unroll
+      for(int b = 0; b < 3; ++b)
+#pragma unroll
+        for(int cc = 0; cc < 3; ++cc) acc = acc + ((w[a] * w[b] * w[cc]) * (1.0f / 64.0f)) * P[a][b][cc];
+    D[g.at(i, j, k)] = acc;
 #pragma unroll
     for(int b = 0; b < 3; ++b)
 #pragma unroll
       for(int cc = 0; cc < 3; ++cc) {
-        // C3[a][b][c] = w[a]*w[b]*w[c]/64, exact in float
-        const float C = (w[a] * w[b] * w[cc]) * (1.0f / 64.0f);
-        const uint32_t q = g.at(i + a - 1, j + b - 1, k + cc - 1);
-        a0 = a0 + C * S0[q];
-        a1 = a1 + C * S1[q];
-        a2 = a2 + C * S2[q];
+        P[0][b][cc] = P[1][b][cc];
+        P[1][b][cc] = P[2][b][cc];
       }
-  const uint32_t c = g.at(i, j, k);
-  D0[c] = a0;
-  D1[c] = a1;
-  D2[c] = a2;
+  }
+}
+
+__global__ void __launch_bounds__(BX* BY)
+  k_filter27c(Geo g, Region r, const float* __restrict__ S0, const float* __restrict__ S1,
+              const float* __restrict__ S2, float* __restrict__ D0, float* __restrict__ D1, float* __restrict__ D2)
+{
+  const int k = r.r0[2] + int(blockIdx.x) * BX + int(threadIdx.x);
+  const int j = r.r0[1] + int(blockIdx.y) * BY + int(threadIdx.y);
+  const int i0 = r.r0[0] + int(blockIdx.z) * FCH;
+  const int i1 = min(i0 + FCH, r.r1[0]);
+  if(k >= r.r1[2] || j >= r.r1[1]) return;
+  filter_column(g, S0, D0, i0, i1, j, k);
+  filter_column(g, S1, D1, i0, i1, j, k);
+  filter_column(g, S2, D2, i0, i1, j, k);
 }
 
 // separable z, y, x passes (0.25, 0.5, 0.25), each output evaluated with the
@@ -338,9 +364,12 @@ int launch_filter(rk_world* w, const int r0[3], const int r1[3], hipStream_t s)
 {
   const Region r = make_region(r0, r1);
   if(empty(r)) return RK_OK;
-  if(w->d.current_filter == 0)
-    hipLaunchKernelGGL(k_filter27, region_grid(r), dim3(BX, BY), 0, s, w->g, r, w->J[0], w->J[1], w->J[2],
-                       w->Jf[0], w->Jf[1], w->Jf[2]);
+  if(w->d.current_filter == 0) {
+    dim3 grid = region_grid(r);
+    grid.z = unsigned((r.r1[0] - r.r0[0] + FCH - 1) / FCH);
+    hipLaunchKernelGGL(k_filter27c, grid, dim3(BX, BY), 0, s, w->g, r, w->J[0], w->J[1], w->J[2], w->Jf[0],
+                       w->Jf[1], w->Jf[2]);
+  }
   else
     hipLaunchKernelGGL(k_filter_sep, region_grid(r), dim3(BX, BY), 0, s, w->g, r, w->J[0], w->J[1], w->J[2],
                        w->Jf[0], w->Jf[1], w->Jf[2]);

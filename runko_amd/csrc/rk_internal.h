@@ -57,8 +57,8 @@ struct Geo {
   float wmax[3];
   float cfl;
   float rcp_cfl;      // RN(1/cfl)
-  int fast_div;       // 1: x/cfl evaluated as q = x*r; q += fma(-q, cfl, x)*r (verified
-                      //    bit-identical to IEEE division for all 2^32 float inputs at world creation)
+  int fast_div;       // 1: x/cfl evaluated as q = x*r; q += fma(-q, cfl, x)*r for 2^-100 <= |x| <= 2^100
+                      //    (verified bit-identical to IEEE division over that range at world creation)
   uint32_t br_magic[3];  // ceil(2^32 / br) for br > 1: c / br == umulhi(c, magic) for 0 <= c < 2^16
 
   __host__ __device__ int div_br(int c, int d) const

@@ -273,12 +273,23 @@ __device__ inline float dot3(const float a[3], const float b[3])
   return r;
 }
 
-// x / cfl, correctly rounded: either the division itself or, when the world
-// verified it bit-identical for all 2^32 float inputs x, one multiply by
-// RN(1/cfl) followed by one FMA remainder correction (Markstein's scheme)
+// x / cfl, correctly rounded.  For 2^-100 <= |x| <= 2^100 the world may use
+// one multiply by RN(1/cfl) and one FMA remainder correction (Markstein's
+// scheme) -- only when rk_world_create verified it bit-identical to the IEEE
+// quotient for every float in that range; other x (zero, subnormal-range,
+// huge, NaN) take the division itself.
+__device__ inline bool fast_div_range(float x)
+{
+  const float ax = fabsf(x);
+  return ax >= 0x1p-100f && ax <= 0x1p100f;
+}
+
+// FAST: 0 = read the world's flag, 1 = reciprocal form (flag known set),
+// -1 = IEEE division (flag known clear)
+template<int FAST = 0>
 __device__ inline float div_cfl(const Geo& g, float x)
 {
-  if(g.fast_div) {
+  if((FAST > 0 || (FAST == 0 && g.fast_div)) && fast_div_range(x)) {
     const float q = x * g.rcp_cfl;
     const float r = fmaf(-q, g.cfl, x);
     return fmaf(r, g.rcp_cfl, q);
@@ -287,6 +298,7 @@ __device__ inline float div_cfl(const Geo& g, float x)
 }
 
 // particle_boris.h:39-59, expression order as written there
+template<int FAST = 0>
 __device__ inline void boris(const Geo& g, float qm, const float E[3], const float B[3], float pos[3], float vel[3])
 {
   const float cfl = g.cfl;
@@ -301,7 +313,7 @@ __device__ inline void boris(const Geo& g, float qm, const float E[3], const flo
   const float bfac = hq * ginv;
   float B0[3];
 #pragma unroll
-  for(int c = 0; c < 3; ++c) B0[c] = div_cfl(g, B[c] * bfac);
+  for(int c = 0; c < 3; ++c) B0[c] = div_cfl<FAST>(g, B[c] * bfac);
   const float f = 2.0f / (1.0f + dot3(B0, B0));
   float u1[3];
   u1[0] = (u0[0] + (u0[1] * B0[2] - u0[2] * B0[1])) * f;
@@ -312,7 +324,7 @@ __device__ inline void boris(const Geo& g, float qm, const float E[3], const flo
   u2[1] = (u0[1] + (-u1[0] * B0[2] + u1[2] * B0[0])) + E0[1];
   u2[2] = (u0[2] + (u1[0] * B0[1] - u1[1] * B0[0])) + E0[2];
 #pragma unroll
-  for(int c = 0; c < 3; ++c) vel[c] = div_cfl(g, u2[c]);
+  for(int c = 0; c < 3; ++c) vel[c] = div_cfl<FAST>(g, u2[c]);
   const float ginv2 = cfl / sqrtf(cfl * cfl + dot3(u2, u2));
 #pragma unroll
   for(int c = 0; c < 3; ++c) pos[c] = pos[c] + (vel[c] * ginv2) * cfl;
@@ -519,6 +531,32 @@ __device__ inline int block_rank(bool flag, int* wcnt, int& total)
   return off + __popcll(bal & lanemask_lt());
 }
 
+// Same scan with one barrier: the per-wave counts go to one of two buffers
+// that consecutive calls alternate (wcnt holds 2*NT/64 ints, par is flipped
+// here).  Call k+2 overwrites call k's buffer only after every wave passed
+// call k+1's barrier, i.e. after every read of call k.
+template<int NT>
+__device__ inline int block_rank1(bool flag, int* wcnt, int& par, int& total)
+{
+  constexpr int NW = NT / 64;
+  int* buf = wcnt + par * NW;
+  par ^= 1;
+  const uint64_t bal = __ballot(flag);
+  const int wid = __builtin_amdgcn_readfirstlane(int(threadIdx.x) >> 6);
+  if(lane_id() == 0) buf[wid] = __popcll(bal);
+  __syncthreads();
+  const int l = lane_id();
+  const int c = l < NW ? buf[l] : 0;
+  int x = c;
+  x += __builtin_amdgcn_update_dpp(0, x, 0x111, 0xf, 0xf, true);  // row_shr:1
+  x += __builtin_amdgcn_update_dpp(0, x, 0x112, 0xf, 0xf, true);  // row_shr:2
+  x += __builtin_amdgcn_update_dpp(0, x, 0x114, 0xf, 0xf, true);  // row_shr:4
+  if(NW > 8) x += __builtin_amdgcn_update_dpp(0, x, 0x118, 0xf, 0xf, true);  // row_shr:8
+  const int off = __builtin_amdgcn_readlane(x - c, wid);
+  total = __builtin_amdgcn_readlane(x, NW - 1);
+  return off + __popcll(bal & lanemask_lt());
+}
+
 // reserve one slot per flagged lane in a counter with a single atomic per wave
 __device__ inline uint32_t wave_reserve(bool flag, uint32_t* counter)
 {
@@ -548,7 +586,7 @@ template<int NT = WG>
 struct CompactLdsT {
   uint32_t holes[MAX_HOLES];
   uint32_t tailbits[MAX_HOLES / 32];
-  int wcnt[NT / 64];
+  int wcnt[2 * (NT / 64)];  // block_rank uses the first half, block_rank1 both
   int nholes;
 };
 using CompactLds = CompactLdsT<WG>;
@@ -1132,7 +1170,7 @@ struct NbrTab {
   float wlo[3], whi[3], wlen[3];  // periodic box
 };
 
-template<int BR>
+template<int BR, int FAST>
 __global__ void __launch_bounds__(FWG) __attribute__((amdgpu_waves_per_eu(4)))
   k_fused(Geo g, SpParams sp, FieldPtrs f, MigPtrs m, const MigPtrs* mtab, int nbins, uint32_t* counters,
           float* jstage, int64_t jstage_stride)
@@ -1190,19 +1228,35 @@ __global__ void __launch_bounds__(FWG) __attribute__((amdgpu_waves_per_eu(4)))
     float* const U1 = sp.vel[s][1] + start;
     float* const U2 = sp.vel[s][2] + start;
     int nholes = 0;
+    int par = 0;
+    // the next chunk's particles are loaded one iteration ahead, so their
+    // latency overlaps this chunk's arithmetic
+    float npos[3] = {0.0f, 0.0f, 0.0f}, nvel[3] = {0.0f, 0.0f, 0.0f};
+    if(int(threadIdx.x) < count) {
+      const uint32_t q0 = threadIdx.x;
+      npos[0] = P0[q0];
+      npos[1] = P1[q0];
+      npos[2] = P2[q0];
+      nvel[0] = U0[q0];
+      nvel[1] = U1[q0];
+      nvel[2] = U2[q0];
+    }
     for(int base = 0; base < count; base += FWG) {
       const int t = base + int(threadIdx.x);
       bool lv0 = false, lv1 = false;
-      float pos[3], vel[3];
+      float pos[3] = {npos[0], npos[1], npos[2]}, vel[3] = {nvel[0], nvel[1], nvel[2]};
       int dir = 13;
       const uint32_t q = uint32_t(t);
+      if(t + FWG < count) {
+        const uint32_t qn = uint32_t(t + FWG);
+        npos[0] = P0[qn];
+        npos[1] = P1[qn];
+        npos[2] = P2[qn];
+        nvel[0] = U0[qn];
+        nvel[1] = U1[qn];
+        nvel[2] = U2[qn];
+      }
       if(t < count) {
-        pos[0] = P0[q];
-        pos[1] = P1[q];
-        pos[2] = P2[q];
-        vel[0] = U0[q];
-        vel[1] = U1[q];
-        vel[2] = U2[q];
         float pr[3], dd[3];
         int ci[3];
 #pragma unroll
@@ -1215,7 +1269,7 @@ __global__ void __launch_bounds__(FWG) __attribute__((amdgpu_waves_per_eu(4)))
         if(!gather6(S, ldsM, ci[0] + moff[0], ci[1] + moff[1], ci[2] + moff[2], dd, E, Bf))
           gather(g, f, MeanWin {{0, 0, 0}, {0, 0, 0}}, nullptr, Bk.toffh[0] + ci[0], Bk.toffh[1] + ci[1],
                  Bk.toffh[2] + ci[2], dd, E, Bf, &counters[C_STENCIL]);
-        boris(g, qm, E, Bf, pos, vel);
+        boris<FAST>(g, qm, E, Bf, pos, vel);
         // classification against the tile (pack_outgoing) and the brick
         dir = tile_dir(pos, Bk.tmin, Bk.tmax);
         float x2[3];
@@ -1253,17 +1307,24 @@ __global__ void __launch_bounds__(FWG) __attribute__((amdgpu_waves_per_eu(4)))
       // the shared record sets
       const bool hole = lv0 || lv1;
       int tot;
-      const int r = block_rank<FWG>(hole, L.wcnt, tot);
+      const int r = block_rank1<FWG>(hole, L.wcnt, par, tot);
       const int64_t local = int64_t(nrec) + r;
-      const bool spill = hole && local >= m.per_brick;
-      const uint32_t slot0 = wave_reserve(spill && lv0, &counters[C_MIG0]);
-      const uint32_t slot1 = wave_reserve(spill && lv1, &counters[C_MIG1]);
-      bool kept = false;
-      if(spill && ((lv0 && int64_t(slot0) >= mtab->cap[0]) || (lv1 && int64_t(slot1) >= mtab->cap[1]))) {
-        // no room anywhere: the particle stays in this bin (pushed and
-        // deposited); it is classified again next step
-        kept = true;
-        atomicAdd(&counters[C_MIGLOST], 1u);
+      bool spill = false, kept = false;
+      uint32_t slot0 = 0, slot1 = 0;
+      int r2 = r, tot2 = tot;
+      if(int64_t(nrec) + tot > m.per_brick) {
+        // uniform: this chunk fills the mailbox
+        spill = hole && local >= m.per_brick;
+        slot0 = wave_reserve(spill && lv0, &counters[C_MIG0]);
+        slot1 = wave_reserve(spill && lv1, &counters[C_MIG1]);
+        if(spill && ((lv0 && int64_t(slot0) >= mtab->cap[0]) || (lv1 && int64_t(slot1) >= mtab->cap[1]))) {
+          // no room anywhere: the particle stays in this bin (pushed and
+          // deposited); it is classified again next step
+          kept = true;
+          atomicAdd(&counters[C_MIGLOST], 1u);
+        }
+        // holes: every leaver that left the bin; kept particles are not holes
+        r2 = block_rank1<FWG>(hole && !kept, L.wcnt, par, tot2);
       }
       if(t < count) {
         if(hole && !kept) {
@@ -1282,11 +1343,7 @@ __global__ void __launch_bounds__(FWG) __attribute__((amdgpu_waves_per_eu(4)))
           U2[q] = vel[2];
         }
       }
-      // holes: every leaver that left the bin; kept particles are not holes
-      const bool hole2 = hole && !kept;
-      int tot2;
-      const int r2 = block_rank<FWG>(hole2, L.wcnt, tot2);
-      if(hole2 && nholes + r2 < MAX_HOLES) L.holes[nholes + r2] = uint32_t(t);
+      if(hole && !kept && nholes + r2 < MAX_HOLES) L.holes[nholes + r2] = uint32_t(t);
       nholes += tot2;
       nrec += (int64_t(nrec) + tot <= m.per_brick) ? tot : int(m.per_brick - nrec);
     }
@@ -1518,14 +1575,15 @@ void bin_range(const rk_world* w, int tile, int& b0, int& nb)
 // grid for xcd_brick(): 8 * ceil(n/8) blocks so the permutation covers [0, n)
 unsigned xcd_grid(int n) { return unsigned((n + 7) / 8 * 8); }
 
-// Compare div_cfl's two forms over every float bit pattern (NaNs of either
-// form count as equal); mismatching lanes are counted once per wave.
+// Compare div_cfl's two forms over every float in its fast range (every bit
+// pattern with 2^-100 <= |x| <= 2^100); mismatches are summed per wave.
 __global__ void __launch_bounds__(256) k_verify_div(float cfl, float rcp, uint32_t* mismatches)
 {
   const uint64_t stride = uint64_t(gridDim.x) * blockDim.x;
   uint32_t bad = 0;
   for(uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < (uint64_t(1) << 32); i += stride) {
     const float x = __uint_as_float(uint32_t(i));
+    if(!fast_div_range(x)) continue;
     const float a = x / cfl;
     const float q = x * rcp;
     const float r = fmaf(-q, cfl, x);
@@ -1625,9 +1683,11 @@ int launch_fused(rk_world* w, hipStream_t s)
                        static_cast<const MigPtrs*>(w->mig_table), g.nbricks, w->counters, w->jstage,
                        w->jstage_stride);
   };
-  if(g.br[0] == 8 && g.br[1] == 8 && g.br[2] == 8) go(k_fused<8>);
-  else if(g.br[0] == 4 && g.br[1] == 4 && g.br[2] == 4) go(k_fused<4>);
-  else go(k_fused<0>);
+  const bool cube8 = g.br[0] == 8 && g.br[1] == 8 && g.br[2] == 8;
+  const bool cube4 = g.br[0] == 4 && g.br[1] == 4 && g.br[2] == 4;
+  if(cube8) g.fast_div ? go(k_fused<8, 1>) : go(k_fused<8, -1>);
+  else if(cube4) g.fast_div ? go(k_fused<4, 1>) : go(k_fused<4, -1>);
+  else go(k_fused<0, 0>);
   RK_HIP(hipGetLastError());
   return RK_OK;
 }

@@ -60,9 +60,16 @@ def juttner_momenta(n, theta, gen, device):
     return torch.stack([u * (2 * X1 - 1), s * torch.cos(2 * math.pi * X2), s * torch.sin(2 * math.pi * X2)])
 
 
-def fill_uniform_plasma(world, ppc, theta, seed=1, chunk_bins=4096):
+def fill_uniform_plasma(world, ppc, theta, seed=1, chunk_bins=4096, slot_order="interleaved"):
     """Every cell gets ppc particles of every species (all species at the same
-    positions), bins laid out for exactly that occupancy plus slack."""
+    positions), bins laid out for exactly that occupancy plus slack.
+
+    slot_order: "interleaved" stores slot p of a bin in cell p mod (cells per
+    brick), so the 64 lanes of a wave sit in 64 different, neighbouring cells
+    (LDS gathers and current atomics spread over the banks); "random" uses a
+    seeded random slot permutation."""
+    if slot_order not in ("interleaved", "random"):
+        raise ValueError(f"unknown slot_order {slot_order!r}")
     dev = world.dev
     br = world.brick
     cells = br[0] * br[1] * br[2]
@@ -89,7 +96,10 @@ def fill_uniform_plasma(world, ppc, theta, seed=1, chunk_bins=4096):
         nb = b1 - b0
         base = (origin[:, None, None, :] + cl[None, :, None, :]).expand(nb, cells, ppc, 3).to(torch.float64)
         pos = (base + torch.rand((nb, cells, ppc, 3), generator=gen, device=dev, dtype=torch.float64))
-        pos = pos.to(torch.float32).reshape(nb, per_bin, 3)[:, slot_perm, :]
+        if slot_order == "random":
+            pos = pos.to(torch.float32).reshape(nb, per_bin, 3)[:, slot_perm, :]
+        else:
+            pos = pos.to(torch.float32).transpose(1, 2).reshape(nb, per_bin, 3)
         ordinal = (lb[:, None] * per_bin + torch.arange(per_bin, device=dev)[None, :])
         ids = (tile[:, None] << 40) | ordinal
         for s in range(world.nsp):
@@ -143,14 +153,15 @@ def set_turbulent_field(world, B0, n_modes=8, dB_over_B0=1.0, seed=2):
 
 
 def turbulence_world(N=512, tile=32, ppc=16, theta=0.3, sigma=10.0, cfl=0.45, c_omp=10.0, n_modes=8,
-                     dB_over_B0=1.0, seed=1, brick=None, mig_fraction=0.01, mailbox_fraction=0.12):
+                     dB_over_B0=1.0, seed=1, brick=None, mig_fraction=0.01, mailbox_fraction=0.12,
+                     slot_order="interleaved"):
     """Decaying-turbulence box: N^3 cells in tile^3 tiles, periodic."""
     nt = N // tile
     q0, q1, B0 = plasma_parameters(cfl, c_omp, ppc, theta, sigma)
     w = World((tile,) * 3, (nt,) * 3, (0, 0, 0), (nt,) * 3, periodic=True, cfl=cfl, charges=[q0, q1],
               masses=[1.0, 1.0], current_filter="binomial2", brick=brick)
     set_turbulent_field(w, B0, n_modes=n_modes, dB_over_B0=dB_over_B0, seed=seed + 1)
-    fill_uniform_plasma(w, ppc, theta, seed=seed)
+    fill_uniform_plasma(w, ppc, theta, seed=seed, slot_order=slot_order)
     w.ensure_migration(max(1 << 16, int(mig_fraction * w.total_particles())))
     w.ensure_mailbox(mailbox_fraction)
     w.comm_local(1)

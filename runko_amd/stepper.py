@@ -1,9 +1,15 @@
-"""The canonical PIC lap of projects/pic-turbulence/pic.py:187-221 in batched
-form on one world: one launch per method over every tile, the particle phase
-(push, pack_outgoing, comm pic_particle, sort, deposit_current) as the fused
-particle step, E += cfl curl B and E -= J in one pass.  It issues the same
-kernels as Simulation in "batched" mode, without the lap-function dispatch.
+"""The canonical PIC lap of projects/pic-turbulence/pic.py:187-221 on one world,
+without the lap-function dispatch of Simulation:
+
+* BatchedLap: one launch per method over every tile, the particle phase (push,
+  pack_outgoing, comm pic_particle, sort, deposit_current) as the fused
+  particle step, E += cfl curl B and E -= J in one pass (Simulation "batched").
+* PlainLap: the same lap in the per-tile form of the reference driver, one call
+  per tile per method, each method synchronous over all tiles (Simulation
+  "plain").
 """
+
+import torch
 
 from .tools import comm_mode
 
@@ -32,5 +38,41 @@ class BatchedLap:
     def settle(self):
         """Synchronous check after a lap: bins that overflowed are repacked and
         their pending particles appended (none are lost)."""
+        if self.w.pending_overflow():
+            self.w.settle_overflow()
+
+
+class PlainLap:
+    def __init__(self, world, filter_passes=3):
+        self.w = world
+        self.filter_passes = filter_passes
+
+    def _each(self, name):
+        for t in range(self.w.ntiles):
+            self.w.call(name, t)
+        torch.cuda.synchronize()
+
+    def __call__(self):
+        w = self.w
+        self._each("rk_push_half_b")
+        w.comm_local(int(comm_mode.emf_B))
+        self._each("rk_push_particles")
+        self._each("rk_pack_outgoing_particles")
+        w.comm_local(int(comm_mode.pic_particle))
+        self._each("rk_deposit_current")
+        w.comm_local(int(comm_mode.emf_J_exchange))
+        w.comm_local(int(comm_mode.emf_J))
+        for p in range(self.filter_passes):
+            self._each("rk_filter_current")
+            if p == 0 and self.filter_passes > 1:
+                w.comm_local(int(comm_mode.emf_J))
+        self._each("rk_push_half_b")
+        w.comm_local(int(comm_mode.emf_B))
+        self._each("rk_push_e")
+        self._each("rk_add_current")
+        w.comm_local(int(comm_mode.emf_E))
+        torch.cuda.synchronize()
+
+    def settle(self):
         if self.w.pending_overflow():
             self.w.settle_overflow()
