@@ -142,14 +142,26 @@ def run_config0_gpu(inputs, laps, execution):
     from gpu_common import build_gpu, canonical_lap
 
     nt, n, Q, M, tiles = inputs
-    conf, grid, sim = build_gpu(nt, n, Q, M, tiles, execution=execution)
+    graph = execution == "graph"
+    conf, grid, sim = build_gpu(nt, n, Q, M, tiles, execution="batched" if graph else execution)
     npart = grid.world.total_particles()
     step = canonical_lap(sim)
     sim.for_one_lap(step)  # warm-up (includes the lap-0 sort)
+    if graph:
+        # the same lap replayed from captured HIP graphs (runko_amd.stepper.GraphLap)
+        from runko_amd.stepper import GraphLap
+        gl = GraphLap(grid.world)
+        for _ in range(2):
+            gl()
+            gl.settle()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(laps):
-        sim.for_one_lap(step)
+        if graph:
+            gl()
+            gl.settle()
+        else:
+            sim.for_one_lap(step)
     torch.cuda.synchronize()
     dt = time.perf_counter() - t0
     return dict(particle_steps_per_s=npart * laps / dt, ms_per_step=1e3 * dt / laps, laps=laps, npart=npart)
@@ -232,10 +244,18 @@ def main():
         inputs = config0_inputs()
     if not args.no_plain:
         b = run_config0_gpu(inputs, laps=20, execution="batched")
+        gph = run_config0_gpu(inputs, laps=100, execution="graph")
         p = run_config0_gpu(inputs, laps=5, execution="plain")
-        extra["batched_vs_plain_64cubed"] = dict(batched=b, plain_per_tile=p,
-                                                 ratio=b["particle_steps_per_s"] / p["particle_steps_per_s"])
-        log(f"[bench] 64^3: batched {b['particle_steps_per_s']:.3e}, plain {p['particle_steps_per_s']:.3e} p-s/s")
+        step_bytes = 48.0 * b["npart"] + 228.0 * 64 ** 3
+        extra["pair_plasma_64cubed"] = dict(
+            workload="3D periodic pair plasma, 64^3 cells in 4^3 tiles of 16^3, 8 ppc/species, theta 0.3, "
+                     "sigma 1 (Simulation API, canonical lap)",
+            batched=b, graph=gph, plain_per_tile=p,
+            graph_vs_plain=gph["particle_steps_per_s"] / p["particle_steps_per_s"],
+            batched_vs_plain=b["particle_steps_per_s"] / p["particle_steps_per_s"],
+            graph_step_roofline_frac=step_bytes / (gph["ms_per_step"] * 1e-3) / 1e9 / HBM_PEAK_GBS)
+        log(f"[bench] 64^3: graph {gph['particle_steps_per_s']:.3e}, batched {b['particle_steps_per_s']:.3e}, "
+            f"plain {p['particle_steps_per_s']:.3e} p-s/s")
     cpu = None
     if not args.no_cpu_baseline:
         cpu = run_cpu_baseline(inputs, args.cpu_seconds)

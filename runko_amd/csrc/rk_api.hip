@@ -125,20 +125,21 @@ int rk_version(void) { return 1; }
 
 // x/cfl through a reciprocal and an FMA correction is used only where it was
 // checked bit-identical to the division for every float input (cached per cfl)
-static int fast_division_ok(float cfl, float rcp, int& ok)
+static int fast_division_ok(float cfl, float rcp, int& ok_div, int& ok_sqrt)
 {
   static std::mutex mu;
-  static std::map<uint32_t, int> cache;
+  static std::map<uint32_t, std::pair<int, int>> cache;
   uint32_t key;
   std::memcpy(&key, &cfl, sizeof key);
   std::lock_guard<std::mutex> lk(mu);
   auto it = cache.find(key);
   if(it == cache.end()) {
-    uint64_t bad = 0;
-    if(int rc = verify_fast_division(cfl, rcp, &bad)) return rc;
-    it = cache.emplace(key, bad == 0 ? 1 : 0).first;
+    uint64_t bad[2] = {0, 0};
+    if(int rc = verify_fast_division(cfl, rcp, bad)) return rc;
+    it = cache.emplace(key, std::make_pair(bad[0] == 0 ? 1 : 0, bad[1] == 0 ? 1 : 0)).first;
   }
-  ok = it->second;
+  ok_div = it->second.first;
+  ok_sqrt = it->second.second;
   return RK_OK;
 }
 
@@ -179,6 +180,7 @@ int rk_world_create(rk_world** out, const rk_world_desc* d)
   g.cfl = float(d->cfl);
   g.rcp_cfl = 1.0f / g.cfl;
   g.fast_div = 0;
+  g.fast_sqrt = 0;
   for(int a = 0; a < 3; ++a) {
     if(g.n[a] >= (1 << 16)) {
       delete w;
@@ -209,7 +211,7 @@ int rk_world_create(rk_world** out, const rk_world_desc* d)
     return hip_fail(e, "rk_world_create");
   }
   w->filter_done.assign(size_t(g.ntiles()), 0);
-  if(int rc = fast_division_ok(g.cfl, g.rcp_cfl, g.fast_div)) {
+  if(int rc = fast_division_ok(g.cfl, g.rcp_cfl, g.fast_div, g.fast_sqrt)) {
     rk_world_destroy(w);
     return rc;
   }

@@ -57,8 +57,11 @@ struct Geo {
   float wmax[3];
   float cfl;
   float rcp_cfl;      // RN(1/cfl)
-  int fast_div;       // 1: x/cfl evaluated as q = x*r; q += fma(-q, cfl, x)*r for 2^-100 <= |x| <= 2^100
-                      //    (verified bit-identical to IEEE division over that range at world creation)
+  int fast_div;       // 1: x/cfl as q = x*r; q += fma(-q, cfl, x)*r for 2^-100 <= |x| <= 2^100
+  int fast_sqrt;      // 1: cfl/sqrt(X), 2/t, 1/sqrt(t) from hardware sqrt/rcp estimates with exact
+                      //    corrections on their ranges (both flags set only when the forms were
+                      //    verified bit-identical to the IEEE operations for every float of the
+                      //    range at world creation)
   uint32_t br_magic[3];  // ceil(2^32 / br) for br > 1: c / br == umulhi(c, magic) for 0 <= c < 2^16
 
   __host__ __device__ int div_br(int c, int d) const
@@ -223,7 +226,7 @@ int launch_pack(rk_world* w, int tile, hipStream_t s);
 int launch_append(rk_world* w, int set, hipStream_t s);
 int launch_append_mailbox(rk_world* w, hipStream_t s);
 int launch_current_gather(rk_world* w, hipStream_t s);
-int verify_fast_division(float cfl, float rcp, uint64_t* mismatches);
+int verify_fast_division(float cfl, float rcp, uint64_t mismatches[2]);
 int upload_mig_table(rk_world* w);
 int launch_reset_counter(rk_world* w, int which, hipStream_t s);
 int launch_sort(rk_world* w, int tile, hipStream_t s);

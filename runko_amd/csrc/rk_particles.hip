@@ -284,8 +284,8 @@ __device__ inline bool fast_div_range(float x)
   return ax >= 0x1p-100f && ax <= 0x1p100f;
 }
 
-// FAST: 0 = read the world's flag, 1 = reciprocal form (flag known set),
-// -1 = IEEE division (flag known clear)
+// FAST: 0 = read the world's flags (fast_div, fast_sqrt); -1 = IEEE forms;
+// 1 = reciprocal x/cfl only; 2 = reciprocal x/cfl and the sqrt/rcp forms
 template<int FAST = 0>
 __device__ inline float div_cfl(const Geo& g, float x)
 {
@@ -295,6 +295,56 @@ __device__ inline float div_cfl(const Geo& g, float x)
     return fmaf(r, g.rcp_cfl, q);
   }
   return x / g.cfl;
+}
+
+// Correctly rounded sqrt for 2^-60 <= x <= 2^60 from the hardware estimate
+// and a one-ulp residual test on each side, and c / s from the hardware
+// reciprocal (one Newton step) and one FMA remainder correction.  Used only
+// where the world verified the composite forms below bit-identical to the
+// IEEE operations for every float of their range; other inputs take sqrtf
+// and '/' themselves.
+__device__ inline float sqrt_near(float x)
+{
+  const float s = __builtin_amdgcn_sqrtf(x);
+  const float sm = __uint_as_float(__float_as_uint(s) - 1u);
+  const float sp = __uint_as_float(__float_as_uint(s) + 1u);
+  const float rm = fmaf(-sm, s, x);
+  const float rp = fmaf(-sp, s, x);
+  const float r = rm <= 0.0f ? sm : s;
+  return rp > 0.0f ? sp : r;
+}
+
+__device__ inline float div_near(float c, float s)
+{
+  float r = __builtin_amdgcn_rcpf(s);
+  r = fmaf(fmaf(-s, r, 1.0f), r, r);
+  const float q = c * r;
+  const float e = fmaf(-q, s, c);
+  return fmaf(e, r, q);
+}
+
+// cfl / sqrtf(X), X = cfl^2 + u.u (particle_boris.h:43,56)
+template<int FAST = 0>
+__device__ inline float cfl_over_sqrt(const Geo& g, float X)
+{
+  if((FAST > 1 || (FAST == 0 && g.fast_sqrt)) && X >= g.cfl * g.cfl && X <= 0x1p60f)
+    return div_near(g.cfl, sqrt_near(X));
+  return g.cfl / sqrtf(X);
+}
+
+// c / t for c in {1, 2} and t = 1 + v.v (particle_boris.h:47, zigzag 1/sqrt)
+template<int FAST = 0>
+__device__ inline float two_over(const Geo& g, float t)
+{
+  if((FAST > 1 || (FAST == 0 && g.fast_sqrt)) && t >= 1.0f && t <= 0x1p60f) return div_near(2.0f, t);
+  return 2.0f / t;
+}
+
+template<int FAST = 0>
+__device__ inline float inv_sqrt(const Geo& g, float t)
+{
+  if((FAST > 1 || (FAST == 0 && g.fast_sqrt)) && t >= 1.0f && t <= 0x1p60f) return div_near(1.0f, sqrt_near(t));
+  return 1.0f / sqrtf(t);
 }
 
 // particle_boris.h:39-59, expression order as written there
@@ -309,12 +359,12 @@ __device__ inline void boris(const Geo& g, float qm, const float E[3], const flo
     E0[c] = E[c] * hq;
     u0[c] = vel[c] * cfl + E0[c];
   }
-  const float ginv = cfl / sqrtf(cfl * cfl + dot3(u0, u0));
+  const float ginv = cfl_over_sqrt<FAST>(g, cfl * cfl + dot3(u0, u0));
   const float bfac = hq * ginv;
   float B0[3];
 #pragma unroll
   for(int c = 0; c < 3; ++c) B0[c] = div_cfl<FAST>(g, B[c] * bfac);
-  const float f = 2.0f / (1.0f + dot3(B0, B0));
+  const float f = two_over<FAST>(g, 1.0f + dot3(B0, B0));
   float u1[3];
   u1[0] = (u0[0] + (u0[1] * B0[2] - u0[2] * B0[1])) * f;
   u1[1] = (u0[1] + (-u0[0] * B0[2] + u0[2] * B0[0])) * f;
@@ -325,7 +375,7 @@ __device__ inline void boris(const Geo& g, float qm, const float E[3], const flo
   u2[2] = (u0[2] + (u1[0] * B0[1] - u1[1] * B0[0])) + E0[2];
 #pragma unroll
   for(int c = 0; c < 3; ++c) vel[c] = div_cfl<FAST>(g, u2[c]);
-  const float ginv2 = cfl / sqrtf(cfl * cfl + dot3(u2, u2));
+  const float ginv2 = cfl_over_sqrt<FAST>(g, cfl * cfl + dot3(u2, u2));
 #pragma unroll
   for(int c = 0; c < 3; ++c) pos[c] = pos[c] + (vel[c] * ginv2) * cfl;
 }
@@ -342,10 +392,11 @@ struct Zig {
   float Wx1, Wy1, Wz1, Wx2, Wy2, Wz2;
 };
 
-__device__ inline Zig zigzag_setup(const float x2[3], const float u[3], float charge, float cfl)
+template<int FAST = -1>
+__device__ inline Zig zigzag_setup(const Geo& g, const float x2[3], const float u[3], float charge, float cfl)
 {
   Zig z;
-  const float invgam = 1.0f / sqrtf(1.0f + dot3(u, u));
+  const float invgam = inv_sqrt<FAST>(g, 1.0f + dot3(u, u));
   const float s = cfl * invgam;
   float x1[3], fi1[3], fi2[3], xr[3], F1[3], F2[3], W1[3], W2[3];
 #pragma unroll
@@ -763,7 +814,7 @@ __global__ void __launch_bounds__(WG)
       const float u[3] = {sp.vel[s][0][q], sp.vel[s][1][q], sp.vel[s][2][q]};
       const float x2[3] = {sp.pos[s][0][q] - Bk.origo[0], sp.pos[s][1][q] - Bk.origo[1],
                            sp.pos[s][2][q] - Bk.origo[2]};
-      const Zig z = zigzag_setup(x2, u, sp.charge[s], g.cfl);
+      const Zig z = zigzag_setup<0>(g, x2, u, sp.charge[s], g.cfl);
       deposit_particle(g, f, C, lds, z, Bk.toffh, &counters[C_STENCIL]);
     }
   }
@@ -1176,7 +1227,7 @@ __global__ void __launch_bounds__(FWG) __attribute__((amdgpu_waves_per_eu(4)))
           float* jstage, int64_t jstage_stride)
 {
   extern __shared__ double lds_dyn[];
-  __shared__ CompactLdsT<FWG> L;
+  __shared__ int wcnt[2 * (FWG / 64)];
   __shared__ NbrTab NT;
   const int b = xcd_brick(int(blockIdx.x), nbins);
   if(b >= nbins) return;
@@ -1227,11 +1278,17 @@ __global__ void __launch_bounds__(FWG) __attribute__((amdgpu_waves_per_eu(4)))
     float* const U0 = sp.vel[s][0] + start;
     float* const U1 = sp.vel[s][1] + start;
     float* const U2 = sp.vel[s][2] + start;
-    int nholes = 0;
+    uint64_t* const ID = sp.id[s] + start;
+    // In-order compaction: the survivors of every chunk are written at the
+    // running offset wr (wr <= the chunk's first slot, and every lane holds
+    // its particle in registers before the scan's barrier), so the bin ends
+    // compact with whole-line writes and no separate compaction pass.
+    int wr = 0;
     int par = 0;
     // the next chunk's particles are loaded one iteration ahead, so their
     // latency overlaps this chunk's arithmetic
     float npos[3] = {0.0f, 0.0f, 0.0f}, nvel[3] = {0.0f, 0.0f, 0.0f};
+    uint64_t nid = 0;
     if(int(threadIdx.x) < count) {
       const uint32_t q0 = threadIdx.x;
       npos[0] = P0[q0];
@@ -1240,13 +1297,14 @@ __global__ void __launch_bounds__(FWG) __attribute__((amdgpu_waves_per_eu(4)))
       nvel[0] = U0[q0];
       nvel[1] = U1[q0];
       nvel[2] = U2[q0];
+      nid = ID[q0];
     }
     for(int base = 0; base < count; base += FWG) {
       const int t = base + int(threadIdx.x);
       bool lv0 = false, lv1 = false;
       float pos[3] = {npos[0], npos[1], npos[2]}, vel[3] = {nvel[0], nvel[1], nvel[2]};
+      const uint64_t id = nid;
       int dir = 13;
-      const uint32_t q = uint32_t(t);
       if(t + FWG < count) {
         const uint32_t qn = uint32_t(t + FWG);
         npos[0] = P0[qn];
@@ -1255,6 +1313,7 @@ __global__ void __launch_bounds__(FWG) __attribute__((amdgpu_waves_per_eu(4)))
         nvel[0] = U0[qn];
         nvel[1] = U1[qn];
         nvel[2] = U2[qn];
+        nid = ID[qn];
       }
       if(t < count) {
         float pr[3], dd[3];
@@ -1299,7 +1358,7 @@ __global__ void __launch_bounds__(FWG) __attribute__((amdgpu_waves_per_eu(4)))
           lv1 = lb[0] != Bk.lb[0] || lb[1] != Bk.lb[1] || lb[2] != Bk.lb[2];
         }
         // zigzag deposit in the frame of the tile that owns the particle after migration
-        const Zig z = zigzag_setup(x2, vel, charge, g.cfl);
+        const Zig z = zigzag_setup<FAST>(g, x2, vel, charge, g.cfl);
         if(!deposit_win(S, ldsJ, z, off)) deposit_global(g, f, z, dtoffh, &counters[C_STENCIL]);
       }
       // leavers take consecutive slots of this brick's mailbox (rank from a
@@ -1307,11 +1366,10 @@ __global__ void __launch_bounds__(FWG) __attribute__((amdgpu_waves_per_eu(4)))
       // the shared record sets
       const bool hole = lv0 || lv1;
       int tot;
-      const int r = block_rank1<FWG>(hole, L.wcnt, par, tot);
+      const int r = block_rank1<FWG>(hole, wcnt, par, tot);
       const int64_t local = int64_t(nrec) + r;
       bool spill = false, kept = false;
       uint32_t slot0 = 0, slot1 = 0;
-      int r2 = r, tot2 = tot;
       if(int64_t(nrec) + tot > m.per_brick) {
         // uniform: this chunk fills the mailbox
         spill = hole && local >= m.per_brick;
@@ -1323,34 +1381,35 @@ __global__ void __launch_bounds__(FWG) __attribute__((amdgpu_waves_per_eu(4)))
           kept = true;
           atomicAdd(&counters[C_MIGLOST], 1u);
         }
-        // holes: every leaver that left the bin; kept particles are not holes
-        r2 = block_rank1<FWG>(hole && !kept, L.wcnt, par, tot2);
       }
-      if(t < count) {
-        if(hole && !kept) {
-          // records keep the unwrapped position; the append applies the wrap
-          const uint64_t id = sp.id[s][start + q];
-          const uint32_t meta = lv0 ? meta_pack(s, 0, dir, uint32_t(b)) : meta_pack(s, 1, 13, uint32_t(b));
-          if(!spill) write_record(m, 2, uint32_t(int64_t(b) * m.per_brick + local), pos, vel, id, meta);
-          else write_record(*mtab, lv0 ? 0 : 1, lv0 ? slot0 : slot1, pos, vel, id, meta);
-          sp.id[s][start + q] = DEAD_ID;
-        } else {
-          P0[q] = pos[0];
-          P1[q] = pos[1];
-          P2[q] = pos[2];
-          U0[q] = vel[0];
-          U1[q] = vel[1];
-          U2[q] = vel[2];
-        }
+      const bool stays = t < count && (!hole || kept);
+      // survivor's slot: wr + lanes before it in the chunk - leavers before
+      // it that departed (r counts all leavers, rk the kept ones)
+      int kept_tot = 0;
+      int rk = 0;
+      if(int64_t(nrec) + tot > m.per_brick) rk = block_rank1<FWG>(hole && kept, wcnt, par, kept_tot);
+      const int dst = wr + int(threadIdx.x) - (r - rk);
+      if(t < count && hole && !kept) {
+        // records keep the unwrapped position; the append applies the wrap
+        const uint32_t meta = lv0 ? meta_pack(s, 0, dir, uint32_t(b)) : meta_pack(s, 1, 13, uint32_t(b));
+        if(!spill) write_record(m, 2, uint32_t(int64_t(b) * m.per_brick + local), pos, vel, id, meta);
+        else write_record(*mtab, lv0 ? 0 : 1, lv0 ? slot0 : slot1, pos, vel, id, meta);
       }
-      if(hole && !kept && nholes + r2 < MAX_HOLES) L.holes[nholes + r2] = uint32_t(t);
-      nholes += tot2;
+      if(stays) {
+        const uint32_t d = uint32_t(dst);
+        P0[d] = pos[0];
+        P1[d] = pos[1];
+        P2[d] = pos[2];
+        U0[d] = vel[0];
+        U1[d] = vel[1];
+        U2[d] = vel[2];
+        if(d != uint32_t(t)) ID[d] = id;
+      }
+      wr += min(FWG, count - base) - (tot - kept_tot);
       nrec += (int64_t(nrec) + tot <= m.per_brick) ? tot : int(m.per_brick - nrec);
     }
     __syncthreads();
-    compact_bin<FWG>(sp, s, start, count, nholes, L, counters);
-    __syncthreads();
-    if(threadIdx.x == 0) sp.count[s][b] = uint32_t(count - nholes);
+    if(threadIdx.x == 0) sp.count[s][b] = uint32_t(wr);
     __syncthreads();
   }
   if(threadIdx.x == 0) m.mcount[b] = uint32_t(nrec);
@@ -1575,47 +1634,56 @@ void bin_range(const rk_world* w, int tile, int& b0, int& nb)
 // grid for xcd_brick(): 8 * ceil(n/8) blocks so the permutation covers [0, n)
 unsigned xcd_grid(int n) { return unsigned((n + 7) / 8 * 8); }
 
-// Compare div_cfl's two forms over every float in its fast range (every bit
-// pattern with 2^-100 <= |x| <= 2^100); mismatches are summed per wave.
-__global__ void __launch_bounds__(256) k_verify_div(float cfl, float rcp, uint32_t* mismatches)
+// Compare the fast forms of x/cfl, cfl/sqrt(X), 2/t and 1/sqrt(t) with the
+// IEEE operations over every float of their fast ranges; mismatches summed.
+__global__ void __launch_bounds__(256) k_verify_fast(float cfl, float rcp, uint32_t* mismatches)
 {
   const uint64_t stride = uint64_t(gridDim.x) * blockDim.x;
-  uint32_t bad = 0;
+  uint32_t bad = 0, bad2 = 0;
   for(uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x; i < (uint64_t(1) << 32); i += stride) {
     const float x = __uint_as_float(uint32_t(i));
-    if(!fast_div_range(x)) continue;
-    const float a = x / cfl;
-    const float q = x * rcp;
-    const float r = fmaf(-q, cfl, x);
-    const float b = fmaf(r, rcp, q);
-    const bool same = __float_as_uint(a) == __float_as_uint(b) || (a != a && b != b);
-    bad += same ? 0u : 1u;
+    if(fast_div_range(x)) {
+      const float q = x * rcp;
+      const float r = fmaf(-q, cfl, x);
+      bad += __float_as_uint(fmaf(r, rcp, q)) != __float_as_uint(x / cfl) ? 1u : 0u;
+    }
+    if(x >= cfl * cfl && x <= 0x1p60f)
+      bad2 += __float_as_uint(div_near(cfl, sqrt_near(x))) != __float_as_uint(cfl / sqrtf(x)) ? 1u : 0u;
+    if(x >= 1.0f && x <= 0x1p60f) {
+      bad2 += __float_as_uint(div_near(2.0f, x)) != __float_as_uint(2.0f / x) ? 1u : 0u;
+      bad2 += __float_as_uint(div_near(1.0f, sqrt_near(x))) != __float_as_uint(1.0f / sqrtf(x)) ? 1u : 0u;
+    }
   }
-  const uint64_t bal = __ballot(bad != 0);
-  if(bal == 0) return;
-  // wave sum of bad through LDS-free shuffles
-  for(int o = 32; o > 0; o >>= 1) bad += __shfl_down(bad, o);
-  if(lane_id() == 0) atomicAdd(mismatches, bad);
+  if(__ballot(bad != 0 || bad2 != 0) == 0) return;
+  for(int o = 32; o > 0; o >>= 1) {
+    bad += __shfl_down(bad, o);
+    bad2 += __shfl_down(bad2, o);
+  }
+  if(lane_id() == 0) {
+    atomicAdd(&mismatches[0], bad);
+    atomicAdd(&mismatches[1], bad2);
+  }
 }
 
 size_t means_lds(const Geo& g) { return sizeof(float) * 6 * size_t(g.br[0] + 2) * (g.br[1] + 2) * (g.br[2] + 2); }
 size_t cur_lds(const Geo& g) { return sizeof(double) * 3 * size_t(g.br[0] + 3) * (g.br[1] + 3) * (g.br[2] + 3); }
 }  // namespace
 
-int verify_fast_division(float cfl, float rcp, uint64_t* mismatches)
+int verify_fast_division(float cfl, float rcp, uint64_t mismatches[2])
 {
   uint32_t* d = nullptr;
-  RK_HIP(hipMalloc(reinterpret_cast<void**>(&d), sizeof(uint32_t)));
-  hipError_t e = hipMemset(d, 0, sizeof(uint32_t));
+  RK_HIP(hipMalloc(reinterpret_cast<void**>(&d), 2 * sizeof(uint32_t)));
+  hipError_t e = hipMemset(d, 0, 2 * sizeof(uint32_t));
   if(e == hipSuccess) {
-    hipLaunchKernelGGL(k_verify_div, dim3(8192), dim3(256), 0, 0, cfl, rcp, d);
+    hipLaunchKernelGGL(k_verify_fast, dim3(8192), dim3(256), 0, 0, cfl, rcp, d);
     e = hipGetLastError();
   }
-  uint32_t h = 0;
-  if(e == hipSuccess) e = hipMemcpy(&h, d, sizeof(uint32_t), hipMemcpyDeviceToHost);
+  uint32_t h[2] = {0, 0};
+  if(e == hipSuccess) e = hipMemcpy(h, d, 2 * sizeof(uint32_t), hipMemcpyDeviceToHost);
   (void)hipFree(d);
   RK_HIP(e);
-  *mismatches = h;
+  mismatches[0] = h[0];
+  mismatches[1] = h[1];
   return RK_OK;
 }
 
@@ -1685,8 +1753,9 @@ int launch_fused(rk_world* w, hipStream_t s)
   };
   const bool cube8 = g.br[0] == 8 && g.br[1] == 8 && g.br[2] == 8;
   const bool cube4 = g.br[0] == 4 && g.br[1] == 4 && g.br[2] == 4;
-  if(cube8) g.fast_div ? go(k_fused<8, 1>) : go(k_fused<8, -1>);
-  else if(cube4) g.fast_div ? go(k_fused<4, 1>) : go(k_fused<4, -1>);
+  const int lvl = g.fast_div ? (g.fast_sqrt ? 2 : 1) : -1;
+  if(cube8) lvl == 2 ? go(k_fused<8, 2>) : (lvl == 1 ? go(k_fused<8, 1>) : go(k_fused<8, -1>));
+  else if(cube4) lvl == 2 ? go(k_fused<4, 2>) : (lvl == 1 ? go(k_fused<4, 1>) : go(k_fused<4, -1>));
   else go(k_fused<0, 0>);
   RK_HIP(hipGetLastError());
   return RK_OK;

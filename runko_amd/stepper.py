@@ -76,3 +76,43 @@ class PlainLap:
     def settle(self):
         if self.w.pending_overflow():
             self.w.settle_overflow()
+
+
+class GraphLap:
+    """BatchedLap replayed from captured HIP graphs.
+
+    Every filter pass swaps the roles of J and the filter target, so with an
+    odd number of passes consecutive laps run on swapped buffers: two graphs
+    are captured (even and odd laps) and replayed alternately, starting with
+    the even one.  Capture records kernels only (the library allocates nothing
+    and never synchronises inside these calls; kernel timing must be off).
+    After every replay settle() checks the overflow counter (synchronous), as
+    for BatchedLap.
+    """
+
+    def __init__(self, world, filter_passes=3):
+        self.w = world
+        self.lap = BatchedLap(world, filter_passes)
+        if world.mbox is None or world.jstage is None:
+            world.ensure_mailbox()
+        torch.cuda.synchronize()
+        self.graphs = []
+        side = torch.cuda.Stream()
+        side.wait_stream(torch.cuda.current_stream())
+        with torch.cuda.stream(side):
+            for _ in range(2 if filter_passes % 2 else 1):
+                g = torch.cuda.CUDAGraph()
+                with torch.cuda.graph(g, stream=side):
+                    self.lap()
+                self.graphs.append(g)
+        torch.cuda.current_stream().wait_stream(side)
+        torch.cuda.synchronize()
+        self.k = 0
+
+    def __call__(self):
+        self.graphs[self.k].replay()
+        self.k = (self.k + 1) % len(self.graphs)
+
+    def settle(self):
+        if self.w.pending_overflow():
+            self.w.settle_overflow()
