@@ -35,6 +35,21 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+def profiled_traffic(args):
+    """HBM bytes per k_fused launch from the committed rocprofv3 summary of
+    this workload (FETCH_SIZE + WRITE_SIZE, copy-calibrated), or None."""
+    if (args.cells, args.tile, args.ppc) != (512, 32, 16):
+        return None
+    path = os.path.join(ROOT, "profiles", "k_fused_turbulence512.md")
+    try:
+        for line in open(path):
+            if line.startswith("traffic_bytes_per_launch k_fused ="):
+                return float(line.split("=")[1])
+    except OSError:
+        return None
+    return None
+
+
 def run_turbulence(args):
     import torch
 
@@ -288,7 +303,8 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": res["achieved_gbs"] / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": profiled_traffic(args),
+            "traffic_source": "profiles/k_fused_turbulence512.md (rocprofv3 FETCH_SIZE + WRITE_SIZE, copy-calibrated)",
             "algorithmic_bytes_per_launch": res["alg_bytes"],
             "avg_launch_ms": res["fused_ms"],
         },
