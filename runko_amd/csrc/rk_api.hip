@@ -196,9 +196,19 @@ int rk_world_create(rk_world** out, const rk_world_desc* d)
     delete w;
     return fail(RK_EINVAL, "too many bricks (>= 2^24)");
   }
+  double maxq = 0.0;
   for(int s = 0; s < d->n_species; ++s) {
     w->sp[s].qm = static_cast<float>(double(sign_of(d->charge[s])) / d->mass[s]);
     w->sp[s].charge = static_cast<float>(d->charge[s]);
+    maxq = std::max(maxq, std::fabs(double(w->sp[s].charge)));
+  }
+  // every contribution is |q| * (a move < 1 cell) * weights <= 1: scale so that
+  // max|q| * jscale <= 2^40 (sums of 2^15 contributions stay far below 2^53)
+  {
+    int e = 0;
+    if(maxq > 0.0 && std::isfinite(maxq)) (void)std::frexp(maxq, &e);  // maxq < 2^e
+    g.jscale = std::ldexp(1.0, 40 - e);
+    g.jinv = std::ldexp(1.0, e - 40);
   }
   hipError_t e = hipMalloc(reinterpret_cast<void**>(&w->counters), sizeof(uint32_t) * C_NUM);
   if(e == hipSuccess) e = hipMemset(w->counters, 0, sizeof(uint32_t) * C_NUM);

@@ -62,6 +62,8 @@ struct Geo {
                       //    corrections on their ranges (both flags set only when the forms were
                       //    verified bit-identical to the IEEE operations for every float of the
                       //    range at world creation)
+  double jscale;      // fixed-point scale of the fused current windows: 2^40 / (max|q| rounded up to 2^k)
+  double jinv;        // 1 / jscale (exact)
   uint32_t br_magic[3];  // ceil(2^32 / br) for br > 1: c / br == umulhi(c, magic) for 0 <= c < 2^16
 
   __host__ __device__ int div_br(int c, int d) const
