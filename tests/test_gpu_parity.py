@@ -340,3 +340,24 @@ def test_graph_lap_matches_eager():
             oa, ob = torch.argsort(xa), torch.argsort(xb)
             assert torch.equal(xa[oa], xb[ob])
             assert torch.max(torch.abs(pa[:, oa] - pb[:, ob])).item() < 1e-4
+
+
+@pytest.mark.parametrize("n", [(10, 6, 9), (4, 4, 4), (6, 11, 5)])
+def test_one_lap_parity_uneven_bricks(orc, n):
+    # non-cubic and tiny bricks (generic-shape fused kernel, windows reached
+    # by bricks two away when br <= 2) against the oracle, batched form
+    nt = (2, 2, 2)
+    inputs = random_inputs(nt, n, 2, seed=13, ppc=3, umax=1.2)
+    conf, grid, sim = build_gpu(nt, n, Q, M, inputs, execution="batched")
+    w = build_oracle(orc, nt, n, Q, M, inputs)
+    sim.for_one_lap(canonical_lap(sim))
+    w.lap(0)
+    for s in range(2):
+        tg, sg, ig = gpu_particles(grid, s)
+        to, so, io = oracle_particles(w, s)
+        assert np.array_equal(ig, io)
+        assert np.array_equal(tg, to)
+        assert np.array_equal(sg, so)
+    g, o = gpu_fields(grid), w.global_fields()
+    assert rel_err(g[0:3], o[0:3]) < 1e-6
+    assert rel_err(g[6:9], o[6:9]) < 1e-6
