@@ -122,6 +122,40 @@ int rk_world_set_mailbox(rk_world* w, int64_t per_brick, float* const pos[3], fl
  * into J.  Required by rk_particle_step_fused. */
 int rk_world_set_current_staging(rk_world* w, float* buf, int64_t stride);
 
+/* Reflecting wall (conducting piston) normal to x, pic::reflector_wall
+ * (src/runko/pic/reflector_wall.h:13-25): position, velocity beta, Lorentz factor. */
+typedef struct rk_reflector_wall {
+  float walloc;
+  float betawall;
+  float gammawall;
+} rk_reflector_wall;
+
+/* pic::Tile::reflect_particles for the given walls (the tile's registered
+ * walls, in registration order; src/runko/pic/reflector_wall.c++:125-298;
+ * tile -1: every tile).  At most 8 walls.
+ * Particles behind a wall are reflected or, when they did not cross it this
+ * step, removed; the +q / -q correction currents are kept per world and added
+ * to J by the next deposit_current of each affected tile (tile.c++:411-414). */
+int rk_reflect_particles(rk_world* w, int32_t tile, const rk_reflector_wall* walls, int32_t n_walls,
+                         void* stream);
+
+/* emf::edge_bc (src/runko/emf/edge_bc.h:22-38): set field components in the
+ * region left (side 0) or right (side 1) of a global position along one axis. */
+typedef struct rk_edge_bc {
+  int32_t direction;  /* 0 x, 1 y, 2 z */
+  int32_t side;       /* 0 left of position, 1 right */
+  float position;
+  float E[3], B[3], J[3];
+  int32_t E_components, B_components, J_components;  /* bit c: set component c */
+} rk_edge_bc;
+
+/* emf::Tile::apply_edge_bc for comm_mode emf_E, emf_B or emf_J
+ * (src/runko/emf/tile.c++:808-847, yee_lattice.c++:263-306; tile -1: every
+ * tile).  Sets the region on the tile's interior cells and on the part of its
+ * haloed region that lies in the world's outer halo; halo cells inside the
+ * world are neighbour cells that the same width rule puts in the region. */
+int rk_apply_edge_bc(rk_world* w, int32_t tile, const rk_edge_bc* bc, int32_t mode, void* stream);
+
 /* ---- emf::Tile methods (src/runko/emf/tile.c++:358-426) ---- */
 int rk_push_half_b(rk_world* w, int32_t tile, void* stream);   /* B += cfl/2 curl E, fdtd2 */
 int rk_push_e(rk_world* w, int32_t tile, void* stream);        /* E += cfl curl B, fdtd2 */

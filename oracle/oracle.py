@@ -51,6 +51,8 @@ def lib():
         L.orc_num_alive.restype = ctypes.c_long
         L.orc_get_particles.argtypes = [vp, ip, ip, vp, vp]
         L.orc_interpolate.argtypes = [vp, ip, ip, vp]
+        L.orc_reflect_particles.argtypes = [vp, ip, vp, ip]
+        L.orc_apply_edge_bc.argtypes = [vp, ip, vp, ip]
         L.orc_num_outgoing.argtypes = [vp, ip]
         L.orc_num_outgoing.restype = ctypes.c_long
         L.orc_get_outgoing.argtypes = [vp, ip, vp, vp, vp, vp]
@@ -152,6 +154,20 @@ class OracleWorld:
         out = np.zeros((self.num_alive(tile, s), 6))
         _check(lib().orc_interpolate(self._h, tile, s, out.ctypes.data))
         return out
+
+    def reflect_particles(self, tile, walls):
+        """walls: iterable of (walloc, betawall, gammawall)."""
+        a = np.asarray(list(walls), dtype=np.float32).reshape(-1, 3)
+        a = np.ascontiguousarray(a)
+        _check(lib().orc_reflect_particles(self._h, tile, a.ctypes.data, len(a)))
+
+    def apply_edge_bc(self, tile, mode, direction=0, side=0, position=0.0, E=(0, 0, 0), B=(0, 0, 0),
+                      J=(0, 0, 0), E_components=7, B_components=7, J_components=7):
+        modes = {"emf_J": 0, "emf_E": 1, "emf_B": 2}
+        m = modes[mode] if isinstance(mode, str) else int(mode)
+        p = np.array([direction, side, position, *E, *B, *J, E_components, B_components, J_components],
+                     dtype=np.float32)
+        _check(lib().orc_apply_edge_bc(self._h, tile, p.ctypes.data, m))
 
     def get_outgoing(self, tile):
         n = lib().orc_num_outgoing(self._h, tile)

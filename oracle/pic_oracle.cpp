@@ -33,6 +33,10 @@
 //   sort                    permutation invariance, key order
 //   halo copy / J_exchange  J0/2J0/4J0/8J0 multiplicity counts
 //   energies                closed forms
+//   reflector wall          reference test behaviour (flip, park, skip, moving
+//                           wall boost), no current behind a stationary wall,
+//                           exact correction-current cancellation
+//   edge bc                 region widths against the reference width rule
 
 #include <algorithm>
 #include <array>
@@ -122,6 +126,9 @@ struct Tile {
   std::vector<std::size_t> ends;
   // incoming spans per species: (source tile, begin, end)
   std::vector<std::vector<std::array<std::size_t, 3>>> incoming;
+  // reflector-wall correction current, src/runko/pic/reflector_wall.c++:243-273
+  std::vector<T> corrJ[3];
+  bool corr_pending = false;
 };
 
 template<class T>
@@ -511,6 +518,181 @@ void deposit_current(Tile<T>& t, double cfl)
   for(auto& c : t.sp) zigzag(c, genJ, l, origo, cfl);
   for(int c = 0; c < 3; ++c)
     for(std::size_t q = 0; q < l.J[c].size(); ++q) l.J[c][q] = l.J[c][q] + genJ[c][q];
+  // pending reflector correction, src/runko/pic/tile.c++:411-414
+  if(t.corr_pending) {
+    for(int c = 0; c < 3; ++c)
+      for(std::size_t q = 0; q < l.J[c].size(); ++q) l.J[c][q] = l.J[c][q] + t.corrJ[c][q];
+    t.corr_pending = false;
+  }
+}
+
+// zigzag deposit of one sub-trajectory x1 -> x2 (lattice-local coordinates),
+// atomic-add semantics, src/runko/pic/reflector_wall.c++:36-118
+template<class T>
+void zigzag_segment(std::vector<T> (&Jg)[3], const Lattice<T>& l, const T x1[3], const T x2[3], T charge)
+{
+  const T one = T(1);
+  T fi1[3], fi2[3], xr[3], F1[3], F2[3], W1[3], W2[3];
+  for(int d = 0; d < 3; ++d) {
+    fi1[d] = std::floor(x1[d]);
+    fi2[d] = std::floor(x2[d]);
+  }
+  for(int d = 0; d < 3; ++d) {
+    const T a = std::min(fi1[d], fi2[d]) + one;
+    const T b = std::max(std::max(fi1[d], fi2[d]), T(0.5) * (x1[d] + x2[d]));
+    xr[d] = std::min(a, b);
+  }
+  for(int d = 0; d < 3; ++d) {
+    F1[d] = charge * (xr[d] - x1[d]);
+    F2[d] = charge * (x2[d] - xr[d]);
+    W1[d] = T(0.5) * (x1[d] + xr[d]) - fi1[d];
+    W2[d] = T(0.5) * (x2[d] + xr[d]) - fi2[d];
+  }
+  auto store = [&](std::uint32_t i, std::uint32_t j, std::uint32_t k, T jx, T jy, T jz) {
+    const std::size_t q = l.at(int(i), int(j), int(k));
+    Jg[0][q] += jx;
+    Jg[1][q] += jy;
+    Jg[2][q] += jz;
+  };
+  const std::uint32_t a1 = static_cast<std::uint32_t>(fi1[0]), b1 = static_cast<std::uint32_t>(fi1[1]),
+                      c1 = static_cast<std::uint32_t>(fi1[2]);
+  const std::uint32_t a2 = static_cast<std::uint32_t>(fi2[0]), b2 = static_cast<std::uint32_t>(fi2[1]),
+                      c2 = static_cast<std::uint32_t>(fi2[2]);
+  const T Fx1 = F1[0], Fy1 = F1[1], Fz1 = F1[2], Fx2 = F2[0], Fy2 = F2[1], Fz2 = F2[2];
+  const T Wx1 = W1[0], Wy1 = W1[1], Wz1 = W1[2], Wx2 = W2[0], Wy2 = W2[1], Wz2 = W2[2];
+  store(a1, b1, c1, Fx1 * (one - Wy1) * (one - Wz1), Fy1 * (one - Wx1) * (one - Wz1), Fz1 * (one - Wx1) * (one - Wy1));
+  store(a2, b2, c2, Fx2 * (one - Wy2) * (one - Wz2), Fy2 * (one - Wx2) * (one - Wz2), Fz2 * (one - Wx2) * (one - Wy2));
+  store(a1 + 1, b1, c1, T(0), Fy1 * Wx1 * (one - Wz1), Fz1 * Wx1 * (one - Wy1));
+  store(a2 + 1, b2, c2, T(0), Fy2 * Wx2 * (one - Wz2), Fz2 * Wx2 * (one - Wy2));
+  store(a1, b1 + 1, c1, Fx1 * Wy1 * (one - Wz1), T(0), Fz1 * (one - Wx1) * Wy1);
+  store(a2, b2 + 1, c2, Fx2 * Wy2 * (one - Wz2), T(0), Fz2 * (one - Wx2) * Wy2);
+  store(a1, b1, c1 + 1, Fx1 * (one - Wy1) * Wz1, Fy1 * (one - Wx1) * Wz1, T(0));
+  store(a2, b2, c2 + 1, Fx2 * (one - Wy2) * Wz2, Fy2 * (one - Wx2) * Wz2, T(0));
+  store(a1, b1 + 1, c1 + 1, Fx1 * Wy1 * Wz1, T(0), T(0));
+  store(a2, b2 + 1, c2 + 1, Fx2 * Wy2 * Wz2, T(0), T(0));
+  store(a1 + 1, b1, c1 + 1, T(0), Fy1 * Wx1 * Wz1, T(0));
+  store(a2 + 1, b2, c2 + 1, T(0), Fy2 * Wx2 * Wz2, T(0));
+  store(a1 + 1, b1 + 1, c1, T(0), T(0), Fz1 * Wx1 * Wy1);
+  store(a2 + 1, b2 + 1, c2, T(0), T(0), Fz2 * Wx2 * Wy2);
+}
+
+struct Wall {
+  float walloc, betawall, gammawall;
+};
+
+// pic::Tile::reflect_particles + ParticleContainer::reflect_at_wall,
+// src/runko/pic/reflector_wall.c++:125-273 (every expression in the order
+// written there; the masks are applied as written, so non-reflected particles
+// add zero-charge contributions exactly as the reference does)
+template<class T>
+void reflect_particles(Tile<T>& t, const std::vector<Wall>& walls, double cfl_d)
+{
+  if(walls.empty()) return;
+  const T cflT = static_cast<T>(cfl_d);
+  auto in_tile = [&](const Wall& w) {
+    return T(w.walloc) >= static_cast<T>(t.mins[0]) - cflT && T(w.walloc) <= static_cast<T>(t.maxs[0]);
+  };
+  bool any = false;
+  for(const auto& w : walls) any = any || in_tile(w);
+  if(!any) return;
+  for(int c = 0; c < 3; ++c) t.corrJ[c].assign(t.lat.J[c].size(), T(0));
+  t.corr_pending = true;
+  T origo[3];
+  tile_origo(t, origo);
+  const T EPS = T(1e-10f);
+  const T one = T(1);
+  for(const auto& w : walls) {
+    if(!in_tile(w)) continue;
+    const T walloc = T(w.walloc), betawall = T(w.betawall), gammawall = T(w.gammawall);
+    const T c = cflT;
+    const T walloc0 = walloc - betawall * c;
+    for(auto& ct : t.sp) {
+      const T charge = static_cast<T>(ct.charge);
+      for(std::size_t n = 0; n < ct.size(); ++n) {
+        if(ct.id[n] == DEAD) continue;
+        const T pos1[3] = {ct.x[0][n], ct.x[1][n], ct.x[2][n]};
+        const T u[3] = {ct.u[0][n], ct.u[1][n], ct.u[2][n]};
+        const T gam = std::sqrt(one + dot3(u, u));
+        const T invgam = one / gam;
+        T pos0[3];
+        for(int d = 0; d < 3; ++d) pos0[d] = pos1[d] - c * invgam * u[d];
+        const T mask_skip = (pos1[0] >= walloc) ? one : T(0);
+        const T mask_close = (walloc0 - pos0[0] <= c) ? one : T(0);
+        const T denom = betawall * c - c * u[0] * invgam;
+        const T dt = std::fabs((pos0[0] - walloc0) / (denom + EPS));
+        const T mask_crossed = (dt <= one) ? one : T(0);
+        const T mask_refl = (one - mask_skip) * mask_close * mask_crossed;
+        const T mask_park = (one - mask_skip) - mask_refl;
+        T pcol[3];
+        for(int d = 0; d < 3; ++d) pcol[d] = pos0[d] + c * dt * invgam * u[d];
+        const T ux_new = gammawall * gammawall * gam * (T(2) * betawall - u[0] * invgam * (one + betawall * betawall));
+        const T un[3] = {ux_new, u[1], u[2]};
+        const T gam_new = std::sqrt(one + dot3(un, un));
+        const T invgam_new = one / gam_new;
+        const T dt_refl = std::min(one, std::fabs((pos1[0] - pcol[0]) / (pos1[0] - pos0[0] + EPS)));
+        T prefl[3];
+        for(int d = 0; d < 3; ++d) prefl[d] = pcol[d] + c * dt_refl * invgam_new * un[d];
+        T p1l[3], f0[3], f1[3], r0[3], r1[3];
+        for(int d = 0; d < 3; ++d) p1l[d] = pos1[d] - origo[d];
+        for(int d = 0; d < 3; ++d) {
+          f0[d] = p1l[d] + mask_refl * (pos0[d] - pos1[d]);
+          f1[d] = p1l[d] + mask_refl * (pcol[d] - pos1[d]);
+        }
+        zigzag_segment(t.corrJ, t.lat, f0, f1, mask_refl * charge);
+        for(int d = 0; d < 3; ++d) {
+          const T x1d = prefl[d] - c * invgam_new * un[d];
+          r0[d] = p1l[d] + mask_refl * (x1d - pos1[d]);
+          r1[d] = p1l[d] + mask_refl * (pcol[d] - pos1[d]);
+        }
+        zigzag_segment(t.corrJ, t.lat, r0, r1, mask_refl * (-charge));
+        for(int d = 0; d < 3; ++d) ct.x[d][n] = (one - mask_refl) * pos1[d] + mask_refl * prefl[d];
+        ct.u[0][n] = (one - mask_refl) * u[0] + mask_refl * ux_new;
+        if(mask_park > T(0.5f)) ct.id[n] = DEAD;
+      }
+    }
+  }
+}
+
+// emf::edge_bc: YeeLattice::apply_edge_bc + Tile::edge_bc_width,
+// src/runko/emf/yee_lattice.c++:263-306, src/runko/emf/tile.c++:808-847
+struct EdgeBC {
+  int direction, side;
+  float position;
+  float E[3], B[3], J[3];
+  int E_components, B_components, J_components;
+};
+
+template<class T>
+void apply_edge_bc(Tile<T>& t, const EdgeBC& bc, int mode)
+{
+  const int d = bc.direction;
+  const float tmin = static_cast<float>(t.mins[d]), tmax = static_cast<float>(t.maxs[d]);
+  const std::size_t Nd = std::size_t(t.lat.n[d]);
+  std::size_t width = 0;
+  if(bc.side == 0) {
+    if(bc.position <= tmin) return;
+    width = bc.position >= tmax ? Nd : static_cast<std::size_t>(bc.position - tmin) + 1;
+  } else {
+    if(bc.position >= tmax) return;
+    width = bc.position <= tmin ? Nd : Nd - static_cast<std::size_t>(bc.position - tmin);
+  }
+  if(width == 0) return;
+  const std::size_t w = std::min(width, Nd);
+  const std::size_t dims[3] = {std::size_t(t.lat.L[0]), std::size_t(t.lat.L[1]), std::size_t(t.lat.L[2])};
+  std::size_t lo[3] = {0, 0, 0}, hi[3] = {dims[0], dims[1], dims[2]};
+  if(bc.side == 0) hi[d] = std::size_t(H) + w;
+  else lo[d] = std::size_t(H) + Nd - w;
+  std::vector<T>* F = mode == 1 ? t.lat.E : (mode == 2 ? t.lat.B : t.lat.J);
+  const float* val = mode == 1 ? bc.E : (mode == 2 ? bc.B : bc.J);
+  const int mask = mode == 1 ? bc.E_components : (mode == 2 ? bc.B_components : bc.J_components);
+  if(mode != 0 && mode != 1 && mode != 2) throw std::runtime_error("YeeLattice::apply_edge_bc does not support given communication mode");
+  for(std::size_t i = lo[0]; i < hi[0]; ++i)
+    for(std::size_t j = lo[1]; j < hi[1]; ++j)
+      for(std::size_t k = lo[2]; k < hi[2]; ++k) {
+        const std::size_t q = t.lat.at(int(i), int(j), int(k));
+        for(int c = 0; c < 3; ++c)
+          if(mask & (1 << c)) F[c][q] = static_cast<T>(val[c]);
+      }
 }
 
 // direction of a position w.r.t. [lo, hi), src/runko/pic/particle.c++:227-236
@@ -1100,6 +1282,48 @@ TILE_METHOD(push_particles, push_particles(t, w.cfl))
 TILE_METHOD(deposit_current, deposit_current(t, w.cfl))
 TILE_METHOD(pack_outgoing, pack_outgoing(t))
 TILE_METHOD(sort_particles, sort_particles(t))
+
+// walls: n x (walloc, betawall, gammawall)
+int orc_reflect_particles(void* hp, int tile, const float* walls, int n)
+{
+  std::vector<Wall> ws;
+  for(int i = 0; i < n; ++i) ws.push_back(Wall {walls[3 * i], walls[3 * i + 1], walls[3 * i + 2]});
+  auto* h = static_cast<Handle*>(hp);
+  return guard([&] {
+    dispatch(h, [&](auto& w) {
+      WT;
+      if(tile >= 0) reflect_particles<T>(w.tiles.at(std::size_t(tile)), ws, w.cfl);
+      else
+        for(auto& t : w.tiles) reflect_particles<T>(t, ws, w.cfl);
+    });
+  });
+}
+
+// bc: direction, side, position, Ex Ey Ez, Bx By Bz, Jx Jy Jz, E/B/J component masks
+int orc_apply_edge_bc(void* hp, int tile, const float* p, int mode)
+{
+  EdgeBC bc {};
+  bc.direction = int(p[0]);
+  bc.side = int(p[1]);
+  bc.position = p[2];
+  for(int c = 0; c < 3; ++c) {
+    bc.E[c] = p[3 + c];
+    bc.B[c] = p[6 + c];
+    bc.J[c] = p[9 + c];
+  }
+  bc.E_components = int(p[12]);
+  bc.B_components = int(p[13]);
+  bc.J_components = int(p[14]);
+  auto* h = static_cast<Handle*>(hp);
+  return guard([&] {
+    dispatch(h, [&](auto& w) {
+      WT;
+      if(tile >= 0) apply_edge_bc<T>(w.tiles.at(std::size_t(tile)), bc, mode);
+      else
+        for(auto& t : w.tiles) apply_edge_bc<T>(t, bc, mode);
+    });
+  });
+}
 
 // comm_local(mode): emf_J=0, emf_E=1, emf_B=2, pic_particle=3, emf_J_exchange=6
 int orc_comm_local(void* hp, int mode)

@@ -4,8 +4,10 @@ Drop-in use for runko drivers:  ``import runko_amd as runko``.
 
 Exports the names a runko driver uses on the PIC time-step path:
 ``pic.threeD.Tile``, ``pic.threeD.ParticleState``, ``pic.threeD.ParticleStateBatch``,
-``emf.threeD.Tile``, ``tools.comm_mode``, ``Configuration``, ``TileGrid``,
-``Simulation``, ``sample_boosted_juttner_synge``, ``runko_logger``, ``on_main_rank``.
+``pic.threeD.reflector_wall``, ``emf.threeD.Tile``, ``emf.threeD.edge_bc``,
+``tools.comm_mode``, ``Configuration``, ``TileGrid``, ``Simulation``,
+``MovingInjector``, ``sample_boosted_juttner_synge``, ``runko_logger``,
+``on_main_rank``.
 The compute path is the HIP library librunko_amd.so (gfx950); it is required.
 """
 
@@ -14,6 +16,7 @@ import logging
 from . import tools
 from .configuration import Configuration
 from .method_wrapper import MethodWrapper
+from .moving_injector import MovingInjector
 from .sampling import sample_boosted_juttner_synge
 from .simulation import Simulation
 from .tile_grid import TileGrid
@@ -40,4 +43,4 @@ def runko_logger(name="runko"):
 
 __all__ = ["pic", "emf", "tools", "comm_mode", "Configuration", "TileGrid", "Simulation", "MethodWrapper",
            "sample_boosted_juttner_synge", "on_main_rank", "runko_logger", "EmfTile", "PicTile",
-           "ParticleState", "ParticleStateBatch"]
+           "ParticleState", "ParticleStateBatch", "MovingInjector"]

@@ -37,6 +37,19 @@ class WorldDesc(ctypes.Structure):
     ]
 
 
+class ReflectorWall(ctypes.Structure):
+    """rk_reflector_wall"""
+    _fields_ = [("walloc", ctypes.c_float), ("betawall", ctypes.c_float), ("gammawall", ctypes.c_float)]
+
+
+class EdgeBC(ctypes.Structure):
+    """rk_edge_bc"""
+    _fields_ = [("direction", ctypes.c_int32), ("side", ctypes.c_int32), ("position", ctypes.c_float),
+                ("E", ctypes.c_float * 3), ("B", ctypes.c_float * 3), ("J", ctypes.c_float * 3),
+                ("E_components", ctypes.c_int32), ("B_components", ctypes.c_int32),
+                ("J_components", ctypes.c_int32)]
+
+
 _lib = None
 
 P3 = ctypes.c_void_p * 3
@@ -68,6 +81,8 @@ def lib():
     L.rk_world_set_migration.argtypes = [vp, i32, i64, P3, P3, vp, vp]
     L.rk_world_set_mailbox.argtypes = [vp, i64, P3, P3, vp, vp]
     L.rk_world_set_current_staging.argtypes = [vp, vp, i64]
+    L.rk_reflect_particles.argtypes = [vp, i32, vp, i32, vp]
+    L.rk_apply_edge_bc.argtypes = [vp, i32, ctypes.POINTER(EdgeBC), i32, vp]
     for name in ["rk_push_half_b", "rk_push_e", "rk_add_current", "rk_filter_current", "rk_push_particles",
                  "rk_deposit_current", "rk_pack_outgoing_particles", "rk_sort_particles"]:
         f = getattr(L, name)
@@ -110,7 +125,8 @@ EXPORTED = [
     "rk_particle_step_fused", "rk_comm_local", "rk_inject", "rk_pending_overflow", "rk_bin_demand",
     "rk_repack", "rk_retry_append", "rk_tile_particle_count", "rk_tile_particles", "rk_outgoing",
     "rk_energies", "rk_gauss_residual", "rk_error_counters", "rk_push_e_add_current", "rk_profile_enable",
-    "rk_profile_read", "rk_world_set_mailbox", "rk_world_set_current_staging",
+    "rk_profile_read", "rk_world_set_mailbox", "rk_world_set_current_staging", "rk_reflect_particles",
+    "rk_apply_edge_bc",
 ]
 
 PROF_CATEGORIES = ["fused_particle_step", "push", "deposit", "pack", "append", "fdtd", "filter", "ghost", "other"]

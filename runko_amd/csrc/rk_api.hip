@@ -241,6 +241,7 @@ int rk_world_destroy(rk_world* w)
   (void)hipFree(w->ovf_list);
   (void)hipFree(w->mbox_count);
   (void)hipFree(w->mig_table);
+  for(int c = 0; c < 3; ++c) (void)hipFree(w->corrJ[c]);
   delete w;
   return RK_OK;
 }
@@ -439,7 +440,9 @@ int rk_deposit_current(rk_world* w, int32_t tile, void* stream)
     w->j_accumulating = true;
   }
   ProfScope ps(w, 2, S(stream));
-  return launch_deposit(w, tile, S(stream));
+  if(int rc = launch_deposit(w, tile, S(stream))) return rc;
+  // pending reflector-wall correction currents, pic/tile.c++:411-414
+  return apply_corrections(w, tile, S(stream));
 }
 
 int rk_pack_outgoing_particles(rk_world* w, int32_t tile, void* stream)
@@ -494,6 +497,7 @@ int rk_particle_step_fused(rk_world* w, void* stream)
   {
     ProfScope ps(w, 2, s);
     if(int rc = launch_current_gather(w, s)) return rc;
+    if(int rc = apply_corrections(w, -1, s)) return rc;
   }
   ProfScope ps(w, 4, s);
   if(int rc = launch_append_mailbox(w, s)) return rc;

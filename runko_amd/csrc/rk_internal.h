@@ -154,6 +154,11 @@ struct rk_world {
   int64_t mbox_per_brick = 0;
   uint32_t* mbox_count = nullptr;
   void* mig_table = nullptr;  // device copy of the record-set pointers (upload_mig_table)
+  // reflector-wall correction current (world lattice, 3 components), added to
+  // J and cleared by the next deposit of each pending tile
+  float* corrJ[3] = {nullptr, nullptr, nullptr};
+  std::vector<char> corr_pending;
+  int corr_any = 0;
   // library scratch
   uint32_t* counters = nullptr;   // rk::C_NUM
   uint32_t* ovf_list = nullptr;   // overflowed record refs: (set << 30) | index, set 2 = mailbox
@@ -231,6 +236,7 @@ int launch_current_gather(rk_world* w, hipStream_t s);
 int verify_fast_division(float cfl, float rcp, uint64_t mismatches[2]);
 int upload_mig_table(rk_world* w);
 int launch_reset_counter(rk_world* w, int which, hipStream_t s);
+int apply_corrections(rk_world* w, int tile, hipStream_t s);  // rk_wall.hip
 int launch_sort(rk_world* w, int tile, hipStream_t s);
 int launch_fused(rk_world* w, hipStream_t s);
 int kinetic_energies(rk_world* w, int tile, double* out, hipStream_t s);

@@ -1,0 +1,439 @@
+// Reflecting wall (pic::reflector_wall) and edge boundary conditions
+// (emf::edge_bc), the boundary tools of the reference's shock setup
+// (projects/pic-shock/pic.py).
+//
+// Reference semantics followed (hel-astro-lab/runko v5.0.0):
+//   reflect_at_wall     src/runko/pic/reflector_wall.c++:125-237
+//   reflect_particles   src/runko/pic/reflector_wall.c++:257-298
+//   correction current  src/runko/pic/reflector_wall.c++:36-118, pic/tile.c++:411-414
+//   edge bc             src/runko/emf/yee_lattice.c++:263-306, emf/tile.c++:808-847
+//
+// Every expression of the reflection is evaluated in the order the reference
+// writes it (-ffp-contract=off), so reflected positions and velocities are
+// bit-identical to the CPU restatement.  The correction currents go to a
+// world-lattice buffer with float atomics (the reference also adds them with
+// atomics); deposit_current of each affected tile adds its window of that
+// buffer to J and clears it.
+#include "rk_internal.h"
+
+#include <algorithm>
+#include <vector>
+
+namespace rk {
+
+namespace {
+
+constexpr int WWG = 256;
+constexpr int MAX_WALLS = 8;
+
+struct WallSet {
+  rk_reflector_wall w[MAX_WALLS];
+  int n;
+};
+
+struct SpeciesPtrs {
+  float* pos[RK_MAX_SPECIES][3];
+  float* vel[RK_MAX_SPECIES][3];
+  uint64_t* id[RK_MAX_SPECIES];
+  const int64_t* start[RK_MAX_SPECIES];
+  uint32_t* count[RK_MAX_SPECIES];
+  float charge[RK_MAX_SPECIES];
+  int nsp;
+};
+
+SpeciesPtrs species_ptrs(const rk_world* w)
+{
+  SpeciesPtrs p {};
+  p.nsp = w->d.n_species;
+  for(int s = 0; s < p.nsp; ++s) {
+    for(int d = 0; d < 3; ++d) {
+      p.pos[s][d] = w->sp[s].pos[d];
+      p.vel[s][d] = w->sp[s].vel[d];
+    }
+    p.id[s] = w->sp[s].id;
+    p.start[s] = w->sp[s].bin_start;
+    p.count[s] = w->sp[s].bin_count;
+    p.charge[s] = w->sp[s].charge;
+  }
+  return p;
+}
+
+__device__ inline float dot3w(const float a[3], const float b[3])
+{
+  float r = a[0] * b[0];
+  r = r + a[1] * b[1];
+  r = r + a[2] * b[2];
+  return r;
+}
+
+// zigzag deposit of one sub-trajectory x1 -> x2 (tile-lattice coordinates)
+// into the correction buffer, reflector_wall.c++:36-118
+__device__ void zigzag_single(const Geo& g, float* const Jc[3], const int toffh[3], const float x1[3],
+                              const float x2[3], float charge, uint32_t* err)
+{
+  const float one = 1.0f;
+  float fi1[3], fi2[3], xr[3], F1[3], F2[3], W1[3], W2[3];
+#pragma unroll
+  for(int d = 0; d < 3; ++d) {
+    fi1[d] = floorf(x1[d]);
+    fi2[d] = floorf(x2[d]);
+  }
+#pragma unroll
+  for(int d = 0; d < 3; ++d) {
+    const float a = fminf(fi1[d], fi2[d]) + one;
+    const float b = fmaxf(fmaxf(fi1[d], fi2[d]), 0.5f * (x1[d] + x2[d]));
+    xr[d] = fminf(a, b);
+  }
+#pragma unroll
+  for(int d = 0; d < 3; ++d) {
+    F1[d] = charge * (xr[d] - x1[d]);
+    F2[d] = charge * (x2[d] - xr[d]);
+    W1[d] = 0.5f * (x1[d] + xr[d]) - fi1[d];
+    W2[d] = 0.5f * (x2[d] + xr[d]) - fi2[d];
+  }
+  const int a1 = int(uint32_t(fi1[0])), b1 = int(uint32_t(fi1[1])), c1 = int(uint32_t(fi1[2]));
+  const int a2 = int(uint32_t(fi2[0])), b2 = int(uint32_t(fi2[1])), c2 = int(uint32_t(fi2[2]));
+  auto st = [&](int i, int j, int k, float jx, float jy, float jz) {
+    const int I = toffh[0] + i, J = toffh[1] + j, K = toffh[2] + k;
+    if(I < 0 || J < 0 || K < 0 || I >= g.L[0] || J >= g.L[1] || K >= g.L[2]) {
+      atomicAdd(err, 1u);
+      return;
+    }
+    const uint32_t q = g.at(I, J, K);
+    unsafeAtomicAdd(&Jc[0][q], jx);
+    unsafeAtomicAdd(&Jc[1][q], jy);
+    unsafeAtomicAdd(&Jc[2][q], jz);
+  };
+  const float Fx1 = F1[0], Fy1 = F1[1], Fz1 = F1[2], Fx2 = F2[0], Fy2 = F2[1], Fz2 = F2[2];
+  const float Wx1 = W1[0], Wy1 = W1[1], Wz1 = W1[2], Wx2 = W2[0], Wy2 = W2[1], Wz2 = W2[2];
+  st(a1, b1, c1, Fx1 * (one - Wy1) * (one - Wz1), Fy1 * (one - Wx1) * (one - Wz1), Fz1 * (one - Wx1) * (one - Wy1));
+  st(a2, b2, c2, Fx2 * (one - Wy2) * (one - Wz2), Fy2 * (one - Wx2) * (one - Wz2), Fz2 * (one - Wx2) * (one - Wy2));
+  st(a1 + 1, b1, c1, 0.0f, Fy1 * Wx1 * (one - Wz1), Fz1 * Wx1 * (one - Wy1));
+  st(a2 + 1, b2, c2, 0.0f, Fy2 * Wx2 * (one - Wz2), Fz2 * Wx2 * (one - Wy2));
+  st(a1, b1 + 1, c1, Fx1 * Wy1 * (one - Wz1), 0.0f, Fz1 * (one - Wx1) * Wy1);
+  st(a2, b2 + 1, c2, Fx2 * Wy2 * (one - Wz2), 0.0f, Fz2 * (one - Wx2) * Wy2);
+  st(a1, b1, c1 + 1, Fx1 * (one - Wy1) * Wz1, Fy1 * (one - Wx1) * Wz1, 0.0f);
+  st(a2, b2, c2 + 1, Fx2 * (one - Wy2) * Wz2, Fy2 * (one - Wx2) * Wz2, 0.0f);
+  st(a1, b1 + 1, c1 + 1, Fx1 * Wy1 * Wz1, 0.0f, 0.0f);
+  st(a2, b2 + 1, c2 + 1, Fx2 * Wy2 * Wz2, 0.0f, 0.0f);
+  st(a1 + 1, b1, c1 + 1, 0.0f, Fy1 * Wx1 * Wz1, 0.0f);
+  st(a2 + 1, b2, c2 + 1, 0.0f, Fy2 * Wx2 * Wz2, 0.0f);
+  st(a1 + 1, b1 + 1, c1, 0.0f, 0.0f, Fz1 * Wx1 * Wy1);
+  st(a2 + 1, b2 + 1, c2, 0.0f, 0.0f, Fz2 * Wx2 * Wy2);
+}
+
+// One block per bin of the tile; walls in registration order.
+__global__ void __launch_bounds__(WWG)
+  k_reflect(Geo g, SpeciesPtrs sp, WallSet walls, int b0, float cfl, float origo0, float origo1, float origo2,
+            int toffh0, int toffh1, int toffh2, float* Jc0, float* Jc1, float* Jc2, uint32_t* counters)
+{
+  const int b = b0 + int(blockIdx.x);
+  const float origo[3] = {origo0, origo1, origo2};
+  const int toffh[3] = {toffh0, toffh1, toffh2};
+  float* const Jc[3] = {Jc0, Jc1, Jc2};
+  const float EPS = 1e-10f;
+  const float one = 1.0f;
+  const float c = cfl;
+  for(int wi = 0; wi < walls.n; ++wi) {
+    const float walloc = walls.w[wi].walloc, betawall = walls.w[wi].betawall, gammawall = walls.w[wi].gammawall;
+    const float walloc0 = walloc - betawall * c;
+    for(int s = 0; s < sp.nsp; ++s) {
+      const int64_t start = sp.start[s][b];
+      const int count = int(sp.count[s][b]);
+      const float charge = sp.charge[s];
+      for(int t = int(threadIdx.x); t < count; t += WWG) {
+        const int64_t q = start + t;
+        if(sp.id[s][q] == DEAD_ID) continue;
+        const float pos1[3] = {sp.pos[s][0][q], sp.pos[s][1][q], sp.pos[s][2][q]};
+        const float u[3] = {sp.vel[s][0][q], sp.vel[s][1][q], sp.vel[s][2][q]};
+        const float gam = sqrtf(one + dot3w(u, u));
+        const float invgam = one / gam;
+        float pos0[3];
+#pragma unroll
+        for(int d = 0; d < 3; ++d) pos0[d] = pos1[d] - c * invgam * u[d];
+        const float mask_skip = (pos1[0] >= walloc) ? one : 0.0f;
+        if(mask_skip == one) continue;  // in front of the wall: untouched (all masks zero)
+        const float mask_close = (walloc0 - pos0[0] <= c) ? one : 0.0f;
+        const float denom = betawall * c - c * u[0] * invgam;
+        const float dt = fabsf((pos0[0] - walloc0) / (denom + EPS));
+        const float mask_crossed = (dt <= one) ? one : 0.0f;
+        const float mask_refl = (one - mask_skip) * mask_close * mask_crossed;
+        const float mask_park = (one - mask_skip) - mask_refl;
+        if(mask_refl > 0.0f) {
+          float pcol[3];
+#pragma unroll
+          for(int d = 0; d < 3; ++d) pcol[d] = pos0[d] + c * dt * invgam * u[d];
+          const float ux_new =
+            gammawall * gammawall * gam * (2.0f * betawall - u[0] * invgam * (one + betawall * betawall));
+          const float un[3] = {ux_new, u[1], u[2]};
+          const float gam_new = sqrtf(one + dot3w(un, un));
+          const float invgam_new = one / gam_new;
+          const float dt_refl = fminf(one, fabsf((pos1[0] - pcol[0]) / (pos1[0] - pos0[0] + EPS)));
+          float prefl[3], p1l[3], f0[3], f1[3], r0[3], r1[3];
+#pragma unroll
+          for(int d = 0; d < 3; ++d) {
+            prefl[d] = pcol[d] + c * dt_refl * invgam_new * un[d];
+            p1l[d] = pos1[d] - origo[d];
+          }
+#pragma unroll
+          for(int d = 0; d < 3; ++d) {
+            f0[d] = p1l[d] + mask_refl * (pos0[d] - pos1[d]);
+            f1[d] = p1l[d] + mask_refl * (pcol[d] - pos1[d]);
+          }
+          zigzag_single(g, Jc, toffh, f0, f1, mask_refl * charge, &counters[C_STENCIL]);
+#pragma unroll
+          for(int d = 0; d < 3; ++d) {
+            const float x1d = prefl[d] - c * invgam_new * un[d];
+            r0[d] = p1l[d] + mask_refl * (x1d - pos1[d]);
+            r1[d] = p1l[d] + mask_refl * (pcol[d] - pos1[d]);
+          }
+          zigzag_single(g, Jc, toffh, r0, r1, mask_refl * (-charge), &counters[C_STENCIL]);
+#pragma unroll
+          for(int d = 0; d < 3; ++d) sp.pos[s][d][q] = (one - mask_refl) * pos1[d] + mask_refl * prefl[d];
+          sp.vel[s][0][q] = (one - mask_refl) * u[0] + mask_refl * ux_new;
+        }
+        if(mask_park > 0.5f) sp.id[s][q] = DEAD_ID;
+      }
+      __syncthreads();
+    }
+  }
+}
+
+// In-order removal of dead slots from every bin of a tile (one block per bin).
+__global__ void __launch_bounds__(WWG) k_compact_dead(SpeciesPtrs sp, int b0)
+{
+  __shared__ int wc[WWG / 64];
+  const int b = b0 + int(blockIdx.x);
+  const int lane = int(threadIdx.x) & 63, wv = int(threadIdx.x) >> 6;
+  for(int s = 0; s < sp.nsp; ++s) {
+    const int64_t start = sp.start[s][b];
+    const int count = int(sp.count[s][b]);
+    int wr = 0;
+    for(int base = 0; base < count; base += WWG) {
+      const int t = base + int(threadIdx.x);
+      bool alive = false;
+      float p[3] = {0, 0, 0}, v[3] = {0, 0, 0};
+      uint64_t id = 0;
+      if(t < count) {
+        id = sp.id[s][start + t];
+        alive = id != DEAD_ID;
+#pragma unroll
+        for(int d = 0; d < 3; ++d) {
+          p[d] = sp.pos[s][d][start + t];
+          v[d] = sp.vel[s][d][start + t];
+        }
+      }
+      const uint64_t bal = __ballot(alive);
+      if(lane == 0) wc[wv] = __popcll(bal);
+      __syncthreads();
+      int off = 0, tot = 0;
+#pragma unroll
+      for(int k = 0; k < WWG / 64; ++k) {
+        off += k < wv ? wc[k] : 0;
+        tot += wc[k];
+      }
+      __syncthreads();
+      if(alive) {
+        const int64_t dst = start + wr + off + __popcll(bal & ((uint64_t(1) << lane) - 1));
+#pragma unroll
+        for(int d = 0; d < 3; ++d) {
+          sp.pos[s][d][dst] = p[d];
+          sp.vel[s][d][dst] = v[d];
+        }
+        sp.id[s][dst] = id;
+      }
+      wr += tot;
+    }
+    __syncthreads();
+    if(threadIdx.x == 0) sp.count[s][b] = uint32_t(wr);
+    __syncthreads();
+  }
+}
+
+// J += Jc, Jc = 0 over one haloed tile window
+__global__ void k_add_corr(Geo g, int i0, int j0, int k0, int ni, int nj, int nk, float* J0, float* J1, float* J2,
+                           float* C0, float* C1, float* C2)
+{
+  const int j = j0 + int(blockIdx.x), i = i0 + int(blockIdx.y);
+  if(j >= j0 + nj || i >= i0 + ni) return;
+  for(int k = k0 + int(threadIdx.x); k < k0 + nk; k += int(blockDim.x)) {
+    const uint32_t q = g.at(i, j, k);
+    J0[q] = J0[q] + C0[q];
+    J1[q] = J1[q] + C1[q];
+    J2[q] = J2[q] + C2[q];
+    C0[q] = 0.0f;
+    C1[q] = 0.0f;
+    C2[q] = 0.0f;
+  }
+}
+
+struct Box {
+  int b0[3], bn[3];  // launch box: the tile's region with its halo
+  int r0[3], r1[3];  // the tile's own cells of the region
+};
+
+// Sets the masked components on the launch box where a cell lies in the
+// tile's own region or in the world's outer halo (the cells that stand in
+// for a boundary tile's private halo copies).
+__global__ void k_set_region(Geo g, Box b, float* F0, float* F1, float* F2, float v0, float v1, float v2, int mask)
+{
+  const int j = b.b0[1] + int(blockIdx.x), i = b.b0[0] + int(blockIdx.y);
+  if(j >= b.b0[1] + b.bn[1] || i >= b.b0[0] + b.bn[0]) return;
+  const bool ij_in = i >= b.r0[0] && i < b.r1[0] && j >= b.r0[1] && j < b.r1[1];
+  const bool ij_out = i < HALO || i >= HALO + g.N[0] || j < HALO || j >= HALO + g.N[1];
+  for(int k = b.b0[2] + int(threadIdx.x); k < b.b0[2] + b.bn[2]; k += int(blockDim.x)) {
+    const bool in = ij_in && k >= b.r0[2] && k < b.r1[2];
+    const bool out = ij_out || k < HALO || k >= HALO + g.N[2];
+    if(!in && !out) continue;
+    const uint32_t q = g.at(i, j, k);
+    if(mask & 1) F0[q] = v0;
+    if(mask & 2) F1[q] = v1;
+    if(mask & 4) F2[q] = v2;
+  }
+}
+
+int fail_w(int code, const char* msg)
+{
+  set_error(msg);
+  return code;
+}
+
+}  // namespace
+
+int apply_corrections(rk_world* w, int tile, hipStream_t s)
+{
+  if(!w->corr_any) return RK_OK;
+  const Geo& g = w->g;
+  const int t0 = tile < 0 ? 0 : tile, t1 = tile < 0 ? g.ntiles() : tile + 1;
+  for(int t = t0; t < t1; ++t) {
+    if(!w->corr_pending[size_t(t)]) continue;
+    int tc[3];
+    g.tile_coords(t, tc);
+    const int n0 = g.n[0] + 2 * HALO, n1 = g.n[1] + 2 * HALO, n2 = g.n[2] + 2 * HALO;
+    hipLaunchKernelGGL(k_add_corr, dim3(unsigned(n1), unsigned(n0)), dim3(64), 0, s, g, tc[0] * g.n[0],
+                       tc[1] * g.n[1], tc[2] * g.n[2], n0, n1, n2, w->J[0], w->J[1], w->J[2], w->corrJ[0],
+                       w->corrJ[1], w->corrJ[2]);
+    RK_HIP(hipGetLastError());
+    w->corr_pending[size_t(t)] = 0;
+  }
+  w->corr_any = std::any_of(w->corr_pending.begin(), w->corr_pending.end(), [](char c) { return c != 0; });
+  return RK_OK;
+}
+
+}  // namespace rk
+
+using namespace rk;
+
+extern "C" {
+
+int rk_reflect_particles(rk_world* w, int32_t tile, const rk_reflector_wall* walls, int32_t n_walls, void* stream)
+{
+  if(!w) return fail_w(RK_EINVAL, "null world");
+  const Geo& g = w->g;
+  if(tile < -1 || tile >= g.ntiles()) return fail_w(RK_EINVAL, "tile index out of range");
+  if(n_walls < 0 || n_walls > MAX_WALLS) return fail_w(RK_EINVAL, "at most 8 reflector walls per tile");
+  if(n_walls == 0) return RK_OK;
+  if(!walls) return fail_w(RK_EINVAL, "null wall array");
+  for(int s = 0; s < w->d.n_species; ++s)
+    if(!w->sp[s].id) return fail_w(RK_EINVAL, "particle pools not set");
+  hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+  if(!w->corrJ[0]) {
+    const size_t nb = sizeof(float) * size_t(g.nodes());
+    for(int c = 0; c < 3; ++c) {
+      RK_HIP(hipMalloc(reinterpret_cast<void**>(&w->corrJ[c]), nb));
+      RK_HIP(hipMemset(w->corrJ[c], 0, nb));
+    }
+    w->corr_pending.assign(size_t(g.ntiles()), 0);
+  }
+  const float cfl = g.cfl;
+  const SpeciesPtrs sp = species_ptrs(w);
+  const int t0 = tile < 0 ? 0 : tile, t1 = tile < 0 ? g.ntiles() : tile + 1;
+  for(int t = t0; t < t1; ++t) {
+    int tc[3];
+    g.tile_coords(t, tc);
+    const float tmin = g.tile_min(tc[0], 0);
+    const float tmax = float((g.toff[0] + tc[0] + 1) * g.n[0]);
+    // wall_is_in_tile, reflector_wall.c++:261-264
+    WallSet ws {};
+    for(int i = 0; i < n_walls; ++i)
+      if(walls[i].walloc >= tmin - cfl && walls[i].walloc <= tmax) ws.w[ws.n++] = walls[i];
+    if(ws.n == 0) continue;
+    w->corr_pending[size_t(t)] = 1;
+    w->corr_any = 1;
+    const int b0 = t * g.bpt;
+    ProfScope ps(w, 8, st);
+    hipLaunchKernelGGL(k_reflect, dim3(unsigned(g.bpt)), dim3(WWG), 0, st, g, sp, ws, b0, cfl,
+                       g.tile_min(tc[0], 0) - 3.0f, g.tile_min(tc[1], 1) - 3.0f, g.tile_min(tc[2], 2) - 3.0f,
+                       tc[0] * g.n[0], tc[1] * g.n[1], tc[2] * g.n[2], w->corrJ[0], w->corrJ[1], w->corrJ[2],
+                       w->counters);
+    RK_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_compact_dead, dim3(unsigned(g.bpt)), dim3(WWG), 0, st, sp, b0);
+    RK_HIP(hipGetLastError());
+  }
+  return RK_OK;
+}
+
+int rk_apply_edge_bc(rk_world* w, int32_t tile, const rk_edge_bc* bc, int32_t mode, void* stream)
+{
+  if(!w || !bc) return fail_w(RK_EINVAL, "null argument");
+  const Geo& g = w->g;
+  if(tile < -1 || tile >= g.ntiles()) return fail_w(RK_EINVAL, "tile index out of range");
+  if(bc->direction < 0 || bc->direction > 2 || bc->side < 0 || bc->side > 1)
+    return fail_w(RK_EINVAL, "edge_bc: direction must be 0..2 and side 0 or 1");
+  float* const* F = nullptr;
+  const float* v = nullptr;
+  int mask = 0;
+  switch(mode) {
+    case RK_COMM_EMF_E: F = w->E; v = bc->E; mask = bc->E_components; break;
+    case RK_COMM_EMF_B: F = w->B; v = bc->B; mask = bc->B_components; break;
+    case RK_COMM_EMF_J: F = w->J; v = bc->J; mask = bc->J_components; break;
+    default: return fail_w(RK_EINVAL, "YeeLattice::apply_edge_bc does not support given communication mode");
+  }
+  if(!F[0]) return fail_w(RK_EINVAL, "fields not set");
+  hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+  const int d = bc->direction;
+  const int t0 = tile < 0 ? 0 : tile, t1 = tile < 0 ? g.ntiles() : tile + 1;
+  for(int t = t0; t < t1; ++t) {
+    int tc[3];
+    g.tile_coords(t, tc);
+    // Tile::edge_bc_width, emf/tile.c++:810-828
+    const float tmin = g.tile_min(tc[d], d), tmax = float((g.toff[d] + tc[d] + 1) * g.n[d]);
+    const int Nd = g.n[d];
+    size_t width = 0;
+    if(bc->side == 0) {
+      if(bc->position <= tmin) continue;
+      width = bc->position >= tmax ? size_t(Nd) : static_cast<size_t>(bc->position - tmin) + 1;
+    } else {
+      if(bc->position >= tmax) continue;
+      width = bc->position <= tmin ? size_t(Nd) : size_t(Nd) - static_cast<size_t>(bc->position - tmin);
+    }
+    if(width == 0) continue;
+    const int wdt = int(std::min(width, size_t(Nd)));
+    // YeeLattice::apply_edge_bc sets the region on the tile's haloed
+    // lattice: the halo in the other two directions and the halo on the
+    // region's side of direction d.  Halo cells inside the world belong to
+    // neighbour tiles, which lie in the region by the same width rule and
+    // set those cells themselves; halo cells outside the world are set here.
+    Box b;
+    for(int a = 0; a < 3; ++a) {
+      b.r0[a] = HALO + tc[a] * g.n[a];
+      b.r1[a] = b.r0[a] + g.n[a];
+      b.b0[a] = b.r0[a] - HALO;
+      b.bn[a] = g.n[a] + 2 * HALO;
+    }
+    if(bc->side == 0) {
+      b.r1[d] = b.r0[d] + wdt;
+      b.bn[d] = HALO + wdt;
+    } else {
+      b.r0[d] = b.r1[d] - wdt;
+      b.b0[d] = b.r0[d];
+      b.bn[d] = wdt + HALO;
+    }
+    hipLaunchKernelGGL(k_set_region, dim3(unsigned(b.bn[1]), unsigned(b.bn[0])), dim3(64), 0, st, g, b, F[0], F[1],
+                       F[2], v[0], v[1], v[2], mask);
+    RK_HIP(hipGetLastError());
+  }
+  return RK_OK;
+}
+
+}  // extern "C"

@@ -17,6 +17,7 @@ Two execution modes share that interface:
               that phase needs no pass of its own.
 """
 
+import ctypes
 import logging
 import os
 import pathlib
@@ -25,7 +26,10 @@ import time
 import numpy as np
 import torch
 
+from . import _lib as L
 from .method_wrapper import MethodWrapper
+from .tiles import _walls_array
+from .world import _stream
 from .timer import Timer, timer_statistics
 from .tools import comm_mode
 
@@ -177,12 +181,26 @@ class Simulation:
                     op = {"push_particles": "push", "pack_outgoing_particles": "pack",
                           "sort_particles": "sort", "deposit_current": "deposit"}[m]
                     self._particle_op(op)
+                elif batched and m == "reflect_particles" and self._same_on_all_tiles("_walls"):
+                    # one launch per tile that a wall overlaps, issued by the library
+                    self._flush()
+                    tiles = self._tile_grid.tiles()
+                    walls = tiles[0]._walls
+                    if walls:
+                        arr = _walls_array(walls)
+                        L.check(L.lib().rk_reflect_particles(self._world.h, -1, arr, len(walls), _stream()))
                 else:
+                    self._flush()
                     for tile in self.local_tiles():
                         getattr(tile, m)()
             elif method.startswith("grid_"):
                 m = method[5:]
-                if batched and m in _RK_TILE and not vargs:
+                if batched and m == "apply_edge_bcs" and self._same_on_all_tiles("_edge_bcs"):
+                    self._flush()
+                    for bc in self._tile_grid.tiles()[0]._edge_bcs:
+                        L.check(L.lib().rk_apply_edge_bc(self._world.h, -1, ctypes.byref(bc._struct()),
+                                                         int(vargs[0]), _stream()))
+                elif batched and m in _RK_TILE and not vargs:
                     self._flush()
                     if m == "filter_current" and self._tile_grid.tiles()[0].current_filter is None:
                         raise RuntimeError("Trying to filter current without specifying `current_filter`!")
@@ -220,6 +238,13 @@ class Simulation:
         if not disable_timing:
             self._lap_timers.append(lap_timer)
             self._lap_wall_times.append(t_end - t_begin)
+
+    def _same_on_all_tiles(self, attr):
+        """True when every tile holds the same registered walls / edge bcs."""
+        key = lambda objs: [tuple(sorted(o.__dict__.items())) for o in objs]
+        tiles = self._tile_grid.tiles()
+        first = key(getattr(tiles[0], attr))
+        return all(key(getattr(t, attr)) == first for t in tiles[1:])
 
     def prelude(self, lap_function):
         self._execute_lap_function(lap_function, disable_timing=True)
