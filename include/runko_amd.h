@@ -238,6 +238,52 @@ int rk_gauss_residual(rk_world* w, int32_t filter_passes, double* out, void* str
  * (reference behaviour undefined; clamped here).  Synchronous. */
 int rk_error_counters(rk_world* w, int64_t* out2, void* stream);
 
+/* ---- Differentiable periodic box --------------------------------------
+ * No counterpart in the reference (runko has no differentiation).  One
+ * periodic box of n[0] x n[1] x n[2] cells with the same discrete step as one
+ * lap of projects/pic-turbulence/pic.py:187-221 (half B, push, deposit,
+ * filter passes, half B, E, E -= J), without tiles: fields are planar
+ * [3][n0][n1][n2] arrays (z fastest, Yee-staggered as the tile lattice),
+ * particles one flat SoA array per coordinate holding species 0, 1, ... in
+ * consecutive ranges, in a fixed order (no sort, no bins), so that a step's
+ * adjoint can address every particle by its index.  Values are float
+ * (precision 4) or double (precision 8). */
+typedef struct rk_diff_desc {
+  int32_t n[3];
+  int32_t n_species;
+  int64_t count[RK_MAX_SPECIES];  /* particles per species */
+  double charge[RK_MAX_SPECIES];
+  double mass[RK_MAX_SPECIES];
+  double cfl;
+  int32_t filter_passes;          /* binomial2 passes per step */
+  int32_t precision;              /* 4 or 8 bytes per value */
+} rk_diff_desc;
+
+/* A state (or the adjoint of a state): E, B, positions, momenta. */
+typedef struct rk_diff_state {
+  void* E[3];
+  void* B[3];
+  void* x[3];
+  void* u[3];
+} rk_diff_state;
+
+typedef struct rk_diff rk_diff;
+
+int rk_diff_create(rk_diff** out, const rk_diff_desc* desc);
+int rk_diff_destroy(rk_diff* d);
+
+/* One step: out = step(in).  out may alias in. */
+int rk_diff_step(rk_diff* d, const rk_diff_state* in, rk_diff_state* out, void* stream);
+
+/* Adjoint of one step at state `in`: given the adjoint of the step's output
+ * (adj_out), writes the adjoint of its input (adj_in), i.e. the
+ * vector-Jacobian product adj_in = J(in)^T adj_out.  adj_in may alias adj_out. */
+int rk_diff_step_adjoint(rk_diff* d, const rk_diff_state* in, const rk_diff_state* adj_out,
+                         rk_diff_state* adj_in, void* stream);
+
+/* Particles whose stencil found a position outside [-1, n+1] (skipped). */
+int rk_diff_error_count(rk_diff* d, int64_t* out, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -50,6 +50,26 @@ class EdgeBC(ctypes.Structure):
                 ("J_components", ctypes.c_int32)]
 
 
+class DiffDesc(ctypes.Structure):
+    """rk_diff_desc"""
+    _fields_ = [
+        ("n", ctypes.c_int32 * 3),
+        ("n_species", ctypes.c_int32),
+        ("count", ctypes.c_int64 * RK_MAX_SPECIES),
+        ("charge", ctypes.c_double * RK_MAX_SPECIES),
+        ("mass", ctypes.c_double * RK_MAX_SPECIES),
+        ("cfl", ctypes.c_double),
+        ("filter_passes", ctypes.c_int32),
+        ("precision", ctypes.c_int32),
+    ]
+
+
+class DiffState(ctypes.Structure):
+    """rk_diff_state"""
+    _fields_ = [("E", ctypes.c_void_p * 3), ("B", ctypes.c_void_p * 3), ("x", ctypes.c_void_p * 3),
+                ("u", ctypes.c_void_p * 3)]
+
+
 _lib = None
 
 P3 = ctypes.c_void_p * 3
@@ -102,6 +122,12 @@ def lib():
     L.rk_energies.argtypes = [vp, i32, vp, vp]
     L.rk_gauss_residual.argtypes = [vp, i32, vp, vp]
     L.rk_error_counters.argtypes = [vp, vp, vp]
+    L.rk_diff_create.argtypes = [ctypes.POINTER(vp), ctypes.POINTER(DiffDesc)]
+    L.rk_diff_destroy.argtypes = [vp]
+    L.rk_diff_step.argtypes = [vp, ctypes.POINTER(DiffState), ctypes.POINTER(DiffState), vp]
+    L.rk_diff_step_adjoint.argtypes = [vp, ctypes.POINTER(DiffState), ctypes.POINTER(DiffState),
+                                       ctypes.POINTER(DiffState), vp]
+    L.rk_diff_error_count.argtypes = [vp, ctypes.POINTER(i64), vp]
     _lib = L
     return _lib
 
@@ -126,7 +152,8 @@ EXPORTED = [
     "rk_repack", "rk_retry_append", "rk_tile_particle_count", "rk_tile_particles", "rk_outgoing",
     "rk_energies", "rk_gauss_residual", "rk_error_counters", "rk_push_e_add_current", "rk_profile_enable",
     "rk_profile_read", "rk_world_set_mailbox", "rk_world_set_current_staging", "rk_reflect_particles",
-    "rk_apply_edge_bc",
+    "rk_apply_edge_bc", "rk_diff_create", "rk_diff_destroy", "rk_diff_step", "rk_diff_step_adjoint",
+    "rk_diff_error_count",
 ]
 
 PROF_CATEGORIES = ["fused_particle_step", "push", "deposit", "pack", "append", "fdtd", "filter", "ghost", "other"]
