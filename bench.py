@@ -12,8 +12,10 @@ sync, 3 binomial passes, half-B push, E push with E -= J, E sync.
 
 Also reported: the plain per-tile form (one call per tile per method through
 the runko-compatible Simulation) vs the batched form on the 64^3 workload,
-the roofline of the dominant kernel from HIP events around its launches, and
-the oracle (FP32 C++/OpenMP restatement) timed on the host cores.
+the roofline of the dominant kernel from HIP events around its launches, the
+differentiation workload's adjoint/forward time (96^3 pair plasma, 16
+ppc/species, 200 steps, checkpoints every 20), and the oracle (FP32
+C++/OpenMP restatement) timed on the host cores.
 
 usage: python bench.py [--gpus 1] [--steps K] [--warmup W]
 """
@@ -209,6 +211,69 @@ def run_cpu_baseline(inputs, seconds):
                        f"{laps} full laps ({dt:.1f} s), same step order as the GPU run")
 
 
+def run_differentiation(args):
+    """Differentiation workload: periodic pair plasma on a cells^3 box, ppc
+    particles/cell/species, theta 0.3, initial dB from 12 seeded Fourier modes
+    with |k| <= 3 (learnable amplitudes), counter-drift along x (learnable);
+    loss = magnetic energy in the |k| band [4, 8] after `steps` steps; gradient
+    by the adjoint kernels with checkpoints every `every` steps."""
+    import gc
+
+    import torch
+
+    from runko_amd.diff import PairPlasmaProblem, band_magnetic_energy, rollout
+
+    gc.collect()
+    torch.cuda.empty_cache()
+    n, steps, every = args.adjoint_cells, args.adjoint_steps, args.adjoint_every
+    t0 = time.perf_counter()
+    prob = PairPlasmaProblem(n=(n, n, n), ppc=args.adjoint_ppc, theta=0.3, seed=11)
+    dev = prob.dev
+    amp = torch.linspace(-1.0, 1.0, 12, dtype=torch.float64, device=dev).requires_grad_(True)
+    drift = torch.tensor(0.05, dtype=torch.float64, device=dev).requires_grad_(True)
+    prob.loss(amp, drift, 2, 1).backward()  # warm-up (kernel loads, allocator)
+    torch.cuda.synchronize()
+    log(f"[bench] differentiation {n}^3, {prob.box.np} particles, setup {time.perf_counter() - t0:.1f}s")
+    amp.grad = None
+    drift.grad = None
+    t0 = time.perf_counter()
+    E, B, x, u = prob.initial_state(amp, drift)
+    out = rollout(prob.box, E, B, x, u, steps, every)
+    loss = band_magnetic_energy(out[1])
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    loss.backward()
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    fwd, bwd = t1 - t0, t2 - t1
+    log(f"[bench] differentiation: forward {fwd:.2f}s, backward {bwd:.2f}s for {steps} steps")
+    # the two step kernels alone, on the final state, timed with HIP events
+    state = tuple(t.detach().contiguous() for t in out)
+    spare = prob.box.empty_state()
+    lam = tuple(torch.ones_like(t) * 1e-3 for t in state)
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
+    k = 10
+    ev[0].record()
+    for _ in range(k):
+        prob.box.step(state, spare)
+    ev[1].record()
+    for _ in range(k):
+        prob.box.step_adjoint(state, lam, spare)
+    ev[2].record()
+    torch.cuda.synchronize()
+    f_ms, a_ms = ev[0].elapsed_time(ev[1]) / k, ev[1].elapsed_time(ev[2]) / k
+    grad = torch.cat([amp.grad, drift.grad.view(1)])
+    return dict(
+        workload=f"differentiation: periodic pair plasma {n}^3 cells, {args.adjoint_ppc} ppc/species, theta 0.3, "
+                 f"12 dB modes |k|<=3 + drift, loss = B energy in |k| in [4,8] after {steps} steps, "
+                 f"checkpoints every {every}, FP32",
+        particles=prob.box.np, steps=steps, checkpoint_every=every,
+        forward_s=fwd, backward_s=bwd, adjoint_over_forward=bwd / fwd,
+        forward_ms_per_step=f_ms, adjoint_ms_per_step=a_ms, adjoint_step_over_forward_step=a_ms / f_ms,
+        forward_particle_steps_per_s=prob.box.np * steps / fwd,
+        loss=float(loss.detach()), grad_norm=float(grad.norm()), error_count=prob.box.error_count())
+
+
 def run_copy_probe(mib):
     """Plain device copies of a known size: the scale for rocprof's FETCH_SIZE
     and WRITE_SIZE counters (KiB per dispatch) when they are read for k_fused."""
@@ -239,6 +304,13 @@ def main():
                     help="steps of the plain per-tile form on the same world (0: skip)")
     ap.add_argument("--slot-order", default="interleaved", choices=["interleaved", "random"],
                     help="initial particle order inside a bin")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the differentiation workload")
+    ap.add_argument("--only-adjoint", action="store_true",
+                    help="run only the differentiation workload and print its result object")
+    ap.add_argument("--adjoint-cells", type=int, default=96)
+    ap.add_argument("--adjoint-ppc", type=int, default=16)
+    ap.add_argument("--adjoint-steps", type=int, default=200)
+    ap.add_argument("--adjoint-every", type=int, default=20)
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="also run plain copies of MIB MiB (scale for rocprof byte counters)")
     args = ap.parse_args()
@@ -246,6 +318,12 @@ def main():
         sys.exit("bench.py measures one GPU: every box here has a single MI355X, and this build has no "
                  "multi-GPU path (--gpus must be 1)")
 
+    if args.only_adjoint:
+        d = run_differentiation(args)
+        log(f"[bench] adjoint/forward {d['adjoint_over_forward']:.2f} (rollout with recompute), "
+            f"step kernels {d['forward_ms_per_step']:.2f} / {d['adjoint_ms_per_step']:.2f} ms")
+        print(json.dumps(d), flush=True)
+        return
     if args.copy_probe:
         run_copy_probe(args.copy_probe)
     res = run_turbulence(args)
@@ -271,6 +349,11 @@ def main():
             graph_step_roofline_frac=step_bytes / (gph["ms_per_step"] * 1e-3) / 1e9 / HBM_PEAK_GBS)
         log(f"[bench] 64^3: graph {gph['particle_steps_per_s']:.3e}, batched {b['particle_steps_per_s']:.3e}, "
             f"plain {p['particle_steps_per_s']:.3e} p-s/s")
+    if not args.no_adjoint:
+        extra["differentiation"] = run_differentiation(args)
+        d = extra["differentiation"]
+        log(f"[bench] adjoint/forward {d['adjoint_over_forward']:.2f} (rollout with recompute), "
+            f"step kernels {d['forward_ms_per_step']:.2f} / {d['adjoint_ms_per_step']:.2f} ms")
     cpu = None
     if not args.no_cpu_baseline:
         cpu = run_cpu_baseline(inputs, args.cpu_seconds)

@@ -26,10 +26,14 @@
 //   the interpolation adjoint (scatter of lambda_E, lambda_B to node-mean
 //   adjoints, 48 atomics, then one grid pass maps node means back to the
 //   staggered components) and the gradient of the weights in x.
+// Atomic targets are interleaved per node ([cell][4] current, [cell][8]
+// node-mean adjoints), so one particle's atomics at a node share a cache line.
 // Floors, the zigzag relay point's branch and the wrap are piecewise
 // constant / identity, so their derivatives are taken on the branch the
 // forward step took.
 #include "rk_internal.h"
+
+#include <hipcub/hipcub.hpp>
 
 #include <cmath>
 
@@ -199,9 +203,9 @@ __global__ void __launch_bounds__(DWG) k_dfill(T* p, int64_t n, T v)
 // staggered components.  E_x(node N) = (Ex[N-x] + Ex[N]) / 2, so Ex[n]
 // receives (ME_x[n] + ME_x[n+x]) / 2; B_x(node N) = (Bx[N] + Bx[N-y] + Bx[N-z]
 // + Bx[N-y-z]) / 4, so Bx[n] receives the mean of MB_x over n, n+y, n+z, n+y+z.
+// M8 holds per node ME_x, ME_y, ME_z, MB_x, MB_y, MB_z (and two pad slots).
 template<class T>
-__global__ void __launch_bounds__(DWG)
-  k_nodemean_adj(DBox<T> b, G3<T> LE, G3<T> LB, G3<T> ME, G3<T> MB)
+__global__ void __launch_bounds__(DWG) k_nodemean_adj(DBox<T> b, G3<T> LE, G3<T> LB, const T* __restrict__ M8)
 {
   const int j = int(blockIdx.x), i = int(blockIdx.y);
   const int ip = wrap1(i + 1, b.n[0]), jp = wrap1(j + 1, b.n[1]);
@@ -210,12 +214,35 @@ __global__ void __launch_bounds__(DWG)
     const int64_t c = cidx(b.n, i, j, k);
     const int64_t cx = cidx(b.n, ip, j, k), cy = cidx(b.n, i, jp, k), cz = cidx(b.n, i, j, kp);
     const int64_t cxy = cidx(b.n, ip, jp, k), cxz = cidx(b.n, ip, j, kp), cyz = cidx(b.n, i, jp, kp);
-    LE.c[0][c] += (ME.c[0][c] + ME.c[0][cx]) * T(0.5);
-    LE.c[1][c] += (ME.c[1][c] + ME.c[1][cy]) * T(0.5);
-    LE.c[2][c] += (ME.c[2][c] + ME.c[2][cz]) * T(0.5);
-    LB.c[0][c] += ((MB.c[0][c] + MB.c[0][cy]) + (MB.c[0][cz] + MB.c[0][cyz])) * T(0.25);
-    LB.c[1][c] += ((MB.c[1][c] + MB.c[1][cx]) + (MB.c[1][cz] + MB.c[1][cxz])) * T(0.25);
-    LB.c[2][c] += ((MB.c[2][c] + MB.c[2][cx]) + (MB.c[2][cy] + MB.c[2][cxy])) * T(0.25);
+    auto M = [&](int64_t q, int comp) { return M8[q * 8 + comp]; };
+    LE.c[0][c] += (M(c, 0) + M(cx, 0)) * T(0.5);
+    LE.c[1][c] += (M(c, 1) + M(cy, 1)) * T(0.5);
+    LE.c[2][c] += (M(c, 2) + M(cz, 2)) * T(0.5);
+    LB.c[0][c] += ((M(c, 3) + M(cy, 3)) + (M(cz, 3) + M(cyz, 3))) * T(0.25);
+    LB.c[1][c] += ((M(c, 4) + M(cx, 4)) + (M(cz, 4) + M(cxz, 4))) * T(0.25);
+    LB.c[2][c] += ((M(c, 5) + M(cx, 5)) + (M(cy, 5) + M(cxy, 5))) * T(0.25);
+  }
+}
+
+// interleaved [cell][4] current <-> planar components
+template<class T>
+__global__ void __launch_bounds__(DWG) k_j4_split(const T* __restrict__ J4, G3<T> J, int64_t n)
+{
+  for(int64_t q = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; q < n; q += int64_t(gridDim.x) * blockDim.x) {
+    J.c[0][q] = J4[q * 4 + 0];
+    J.c[1][q] = J4[q * 4 + 1];
+    J.c[2][q] = J4[q * 4 + 2];
+  }
+}
+
+template<class T>
+__global__ void __launch_bounds__(DWG) k_j4_pack(G3<T> J, T* __restrict__ J4, int64_t n)
+{
+  for(int64_t q = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; q < n; q += int64_t(gridDim.x) * blockDim.x) {
+    J4[q * 4 + 0] = J.c[0][q];
+    J4[q * 4 + 1] = J.c[1][q];
+    J4[q * 4 + 2] = J.c[2][q];
+    J4[q * 4 + 3] = T(0);
   }
 }
 
@@ -245,8 +272,66 @@ __device__ inline bool in_box(const DBox<T>& b, const T x[3])
          x[2] <= T(b.n[2] + 1);
 }
 
+// Node means of E and B (interpolation_EB_linear_1st's corner values) at
+// every node, interleaved [node][8]: E_x, E_y, E_z, B_x, B_y, B_z, 0, 0.
+// E_x(N) = (Ex[N-x] + Ex[N]) / 2, B_x(N) = (Bx[N] + (Bx[N-y] + (Bx[N-z] +
+// Bx[N-y-z]))) / 4: the expressions of the per-corner computation, so the
+// particles read the same values they would compute.
 template<class T>
-__device__ inline void gather_fields(const DBox<T>& b, const G3<T>& E, const G3<T>& B, Fwd<T>& f, const T x[3])
+__global__ void __launch_bounds__(DWG) k_node_means(DBox<T> b, G3<T> E, G3<T> B, T* __restrict__ NM8)
+{
+  const int j = int(blockIdx.x), i = int(blockIdx.y);
+  const int im = wrap1(i - 1, b.n[0]), jm = wrap1(j - 1, b.n[1]);
+  for(int k = int(threadIdx.x); k < b.n[2]; k += int(blockDim.x)) {
+    const int km = wrap1(k - 1, b.n[2]);
+    const int64_t c = cidx(b.n, i, j, k);
+    T* o = NM8 + c * 8;
+    o[0] = (E.c[0][cidx(b.n, im, j, k)] + E.c[0][c]) / T(2);
+    o[1] = (E.c[1][cidx(b.n, i, jm, k)] + E.c[1][c]) / T(2);
+    o[2] = (E.c[2][cidx(b.n, i, j, km)] + E.c[2][c]) / T(2);
+    o[3] = (B.c[0][c] + (B.c[0][cidx(b.n, i, jm, k)] + (B.c[0][cidx(b.n, i, j, km)] + B.c[0][cidx(b.n, i, jm, km)]))) /
+           T(4);
+    o[4] = (B.c[1][c] + (B.c[1][cidx(b.n, im, j, k)] + (B.c[1][cidx(b.n, i, j, km)] + B.c[1][cidx(b.n, im, j, km)]))) /
+           T(4);
+    o[5] = (B.c[2][c] + (B.c[2][cidx(b.n, im, j, k)] + (B.c[2][cidx(b.n, i, jm, k)] + B.c[2][cidx(b.n, im, jm, k)]))) /
+           T(4);
+    o[6] = T(0);
+    o[7] = T(0);
+  }
+}
+
+template<class T>
+__device__ inline void load_node(const T* p, T v[6]);
+
+template<>
+__device__ inline void load_node<float>(const float* p, float v[6])
+{
+  const float4 a = *reinterpret_cast<const float4*>(p);
+  const float2 c = *reinterpret_cast<const float2*>(p + 4);
+  v[0] = a.x;
+  v[1] = a.y;
+  v[2] = a.z;
+  v[3] = a.w;
+  v[4] = c.x;
+  v[5] = c.y;
+}
+
+template<>
+__device__ inline void load_node<double>(const double* p, double v[6])
+{
+  const double2 a = *reinterpret_cast<const double2*>(p);
+  const double2 c = *reinterpret_cast<const double2*>(p + 2);
+  const double2 d = *reinterpret_cast<const double2*>(p + 4);
+  v[0] = a.x;
+  v[1] = a.y;
+  v[2] = c.x;
+  v[3] = c.y;
+  v[4] = d.x;
+  v[5] = d.y;
+}
+
+template<class T>
+__device__ inline void gather_fields(const DBox<T>& b, const T* NM8, Fwd<T>& f, const T x[3])
 {
   T p[3];
   uint32_t ii[3];
@@ -259,28 +344,19 @@ __device__ inline void gather_fields(const DBox<T>& b, const G3<T>& E, const G3<
     for(int o = 0; o < 3; ++o) f.I[a][o] = wrap1(wrap1(base + o - 1, b.n[a]), b.n[a]);
     f.d[a] = p[a] - static_cast<T>(ii[a]);
   }
-  auto at = [&](int a, int bb, int c) { return cidx(b.n, f.I[0][a], f.I[1][bb], f.I[2][c]); };
 #pragma unroll
   for(int ic = 0; ic < 2; ++ic)
 #pragma unroll
     for(int jc = 0; jc < 2; ++jc)
 #pragma unroll
       for(int kc = 0; kc < 2; ++kc) {
-        f.mE[0][ic][jc][kc] = (E.c[0][at(ic, jc + 1, kc + 1)] + E.c[0][at(ic + 1, jc + 1, kc + 1)]) / T(2);
-        f.mE[1][ic][jc][kc] = (E.c[1][at(ic + 1, jc, kc + 1)] + E.c[1][at(ic + 1, jc + 1, kc + 1)]) / T(2);
-        f.mE[2][ic][jc][kc] = (E.c[2][at(ic + 1, jc + 1, kc)] + E.c[2][at(ic + 1, jc + 1, kc + 1)]) / T(2);
-        f.mB[0][ic][jc][kc] =
-          (B.c[0][at(ic + 1, jc + 1, kc + 1)] +
-           (B.c[0][at(ic + 1, jc, kc + 1)] + (B.c[0][at(ic + 1, jc + 1, kc)] + B.c[0][at(ic + 1, jc, kc)]))) /
-          T(4);
-        f.mB[1][ic][jc][kc] =
-          (B.c[1][at(ic + 1, jc + 1, kc + 1)] +
-           (B.c[1][at(ic, jc + 1, kc + 1)] + (B.c[1][at(ic + 1, jc + 1, kc)] + B.c[1][at(ic, jc + 1, kc)]))) /
-          T(4);
-        f.mB[2][ic][jc][kc] =
-          (B.c[2][at(ic + 1, jc + 1, kc + 1)] +
-           (B.c[2][at(ic, jc + 1, kc + 1)] + (B.c[2][at(ic + 1, jc, kc + 1)] + B.c[2][at(ic, jc, kc + 1)]))) /
-          T(4);
+        T v[6];
+        load_node(NM8 + cidx(b.n, f.I[0][ic + 1], f.I[1][jc + 1], f.I[2][kc + 1]) * 8, v);
+#pragma unroll
+        for(int c = 0; c < 3; ++c) {
+          f.mE[c][ic][jc][kc] = v[c];
+          f.mB[c][ic][jc][kc] = v[3 + c];
+        }
       }
 #pragma unroll
   for(int c = 0; c < 3; ++c) {
@@ -377,24 +453,69 @@ __device__ inline void zig_setup(const DBox<T>& b, const T xw[3], const T u[3], 
   }
 }
 
+// the 7 (node, component) contributions of one segment, in store order:
+// v[m][c] goes to node (c[0][o0], c[1][o1], c[2][o2]) with offsets ZOFF[m]
+__constant__ const int ZOFF[7][3] = {{0, 0, 0}, {1, 0, 0}, {0, 1, 0}, {0, 0, 1}, {0, 1, 1}, {1, 0, 1}, {1, 1, 0}};
+
 template<class T>
-__device__ inline void zig_store(const DBox<T>& b, const G3<T>& J, const int c[3][2], const T F[3], const T W[3])
+__device__ inline void zig_values(const T F[3], const T W[3], T v[7][3])
 {
   const T one = T(1);
-  auto st = [&](int di, int dj, int dk, T jx, T jy, T jz) {
-    const int64_t q = cidx(b.n, c[0][di], c[1][dj], c[2][dk]);
-    if(jx != T(0)) atomic_add(&J.c[0][q], jx);
-    if(jy != T(0)) atomic_add(&J.c[1][q], jy);
-    if(jz != T(0)) atomic_add(&J.c[2][q], jz);
-  };
-  st(0, 0, 0, F[0] * (one - W[1]) * (one - W[2]), F[1] * (one - W[0]) * (one - W[2]),
-     F[2] * (one - W[0]) * (one - W[1]));
-  st(1, 0, 0, T(0), F[1] * W[0] * (one - W[2]), F[2] * W[0] * (one - W[1]));
-  st(0, 1, 0, F[0] * W[1] * (one - W[2]), T(0), F[2] * (one - W[0]) * W[1]);
-  st(0, 0, 1, F[0] * (one - W[1]) * W[2], F[1] * (one - W[0]) * W[2], T(0));
-  st(0, 1, 1, F[0] * W[1] * W[2], T(0), T(0));
-  st(1, 0, 1, T(0), F[1] * W[0] * W[2], T(0));
-  st(1, 1, 0, T(0), T(0), F[2] * W[0] * W[1]);
+  v[0][0] = F[0] * (one - W[1]) * (one - W[2]);
+  v[0][1] = F[1] * (one - W[0]) * (one - W[2]);
+  v[0][2] = F[2] * (one - W[0]) * (one - W[1]);
+  v[1][0] = T(0);
+  v[1][1] = F[1] * W[0] * (one - W[2]);
+  v[1][2] = F[2] * W[0] * (one - W[1]);
+  v[2][0] = F[0] * W[1] * (one - W[2]);
+  v[2][1] = T(0);
+  v[2][2] = F[2] * (one - W[0]) * W[1];
+  v[3][0] = F[0] * (one - W[1]) * W[2];
+  v[3][1] = F[1] * (one - W[0]) * W[2];
+  v[3][2] = T(0);
+  v[4][0] = F[0] * W[1] * W[2];
+  v[4][1] = T(0);
+  v[4][2] = T(0);
+  v[5][0] = T(0);
+  v[5][1] = F[1] * W[0] * W[2];
+  v[5][2] = T(0);
+  v[6][0] = T(0);
+  v[6][1] = T(0);
+  v[6][2] = F[2] * W[0] * W[1];
+}
+
+template<class T>
+__device__ inline void zig_store(const DBox<T>& b, T* J4, const int c[3][2], const T v[7][3])
+{
+#pragma unroll
+  for(int m = 0; m < 7; ++m) {
+    const int64_t q = cidx(b.n, c[0][ZOFF[m][0]], c[1][ZOFF[m][1]], c[2][ZOFF[m][2]]);
+#pragma unroll
+    for(int comp = 0; comp < 3; ++comp)
+      if(!(m == 1 && comp == 0) && !(m == 2 && comp == 1) && !(m == 3 && comp == 2) && !(m == 4 && comp != 0) &&
+         !(m == 5 && comp != 1) && !(m == 6 && comp != 2))
+        atomic_add(&J4[q * 4 + comp], v[m][comp]);
+  }
+}
+
+// both segments of a particle that stays in its cell land on the same 7
+// nodes: their sums take 12 atomics instead of 24
+template<class T>
+__device__ inline void zig_deposit(const DBox<T>& b, T* J4, const Zig<T>& z)
+{
+  T v1[7][3], v2[7][3];
+  zig_values(z.F1, z.W1, v1);
+  zig_values(z.F2, z.W2, v2);
+  if(z.fi1[0] == z.fi2[0] && z.fi1[1] == z.fi2[1] && z.fi1[2] == z.fi2[2]) {
+#pragma unroll
+    for(int m = 0; m < 7; ++m)
+#pragma unroll
+      for(int comp = 0; comp < 3; ++comp) v1[m][comp] = v1[m][comp] + v2[m][comp];
+    zig_store(b, J4, z.c1, v1);
+  } else {
+    zig_store(b, J4, z.c1, v1);
+    zig_store(b, J4, z.c2, v2);
+  }
 }
 
 template<class T>
@@ -407,7 +528,7 @@ __device__ inline int species_of(const DBox<T>& b, int64_t p)
 
 template<class T>
 __global__ void __launch_bounds__(DWG)
-  k_dpush(DBox<T> b, G3<T> E, G3<T> B1, G3<T> xin, G3<T> uin, G3<T> xout, G3<T> uout, G3<T> J,
+  k_dpush(DBox<T> b, const T* __restrict__ NM8, G3<T> xin, G3<T> uin, G3<T> xout, G3<T> uout, T* J4,
           unsigned long long* err)
 {
   const int64_t np = b.off[b.nsp];
@@ -429,14 +550,13 @@ __global__ void __launch_bounds__(DWG)
       continue;
     }
     Fwd<T> f;
-    gather_fields(b, E, B1, f, x);
+    gather_fields(b, NM8, f, x);
     BorisT<T> bs;
     boris_fwd(b.cfl, b.qm[s], f.EBE, f.EBB, x, u, bs, f.xn, f.un);
     wrap_pos(b, f.xn, f.xw);
     Zig<T> z;
     zig_setup(b, f.xw, f.un, b.q[s], z);
-    zig_store(b, J, z.c1, z.F1, z.W1);
-    zig_store(b, J, z.c2, z.F2, z.W2);
+    zig_deposit(b, J4, z);
 #pragma unroll
     for(int a = 0; a < 3; ++a) {
       xout.c[a][p] = f.xw[a];
@@ -447,7 +567,7 @@ __global__ void __launch_bounds__(DWG)
 
 // gathered adjoint of one zigzag segment: dL/dF and dL/dW (per axis)
 template<class T>
-__device__ inline void zig_seg_adj(const DBox<T>& b, const G3<T>& LJ, T jsign, const int c[3][2], const T F[3],
+__device__ inline void zig_seg_adj(const DBox<T>& b, const T* L4, T jsign, const int c[3][2], const T F[3],
                                    const T W[3], T gF[3], T gW[3])
 {
   const T one = T(1);
@@ -463,7 +583,7 @@ __device__ inline void zig_seg_adj(const DBox<T>& b, const G3<T>& LJ, T jsign, c
         int o[3] = {0, 0, 0};
         o[e] = de;
         o[g] = dg;
-        v[de][dg] = jsign * LJ.c[d][cidx(b.n, c[0][o[0]], c[1][o[1]], c[2][o[2]])];
+        v[de][dg] = jsign * L4[cidx(b.n, c[0][o[0]], c[1][o[1]], c[2][o[2]]) * 4 + d];
       }
     const T we = W[e], wg = W[g];
     const T G = (v[0][0] * (one - we) + v[1][0] * we) * (one - wg) + (v[0][1] * (one - we) + v[1][1] * we) * wg;
@@ -490,32 +610,48 @@ __device__ inline void cross_adj(const T a[3], const T b[3], const T gc[3], T ga
 
 template<class T>
 __global__ void __launch_bounds__(DWG)
-  k_dpush_adj(DBox<T> b, G3<T> E, G3<T> B1, G3<T> xin, G3<T> uin, G3<T> LJ, T jsign, G3<T> lx_out,
-              G3<T> lu_out, G3<T> lx_in, G3<T> lu_in, G3<T> ME, G3<T> MB)
+  k_dpush_adj(DBox<T> b, const T* __restrict__ NM8, G3<T> xin, G3<T> uin, const T* L4, T jsign, G3<T> lx_out,
+              G3<T> lu_out, G3<T> lx_in, G3<T> lu_in, const uint32_t* __restrict__ cstart,
+              const uint32_t* __restrict__ pslot, T* __restrict__ rec)
 {
   const int64_t np = b.off[b.nsp];
   const T cfl = b.cfl;
-  for(int64_t p = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; p < np; p += int64_t(gridDim.x) * blockDim.x) {
-    const int s = species_of(b, p);
+  // every lane of a wave runs every iteration (the scatter below reduces
+  // across lanes); lanes past the end or outside the box compute on a dummy
+  // particle and neither scatter nor store
+  for(int64_t base = int64_t(blockIdx.x) * blockDim.x; base < np; base += int64_t(gridDim.x) * blockDim.x) {
+    const int64_t p = base + int64_t(threadIdx.x);
+    const bool live = p < np;
+    const int64_t pl = live ? p : np - 1;
+    const int s = species_of(b, pl);
     T x[3], u[3], gxo[3], guo[3];
 #pragma unroll
     for(int a = 0; a < 3; ++a) {
-      x[a] = xin.c[a][p];
-      u[a] = uin.c[a][p];
-      gxo[a] = lx_out.c[a][p];
-      guo[a] = lu_out.c[a][p];
+      x[a] = xin.c[a][pl];
+      u[a] = uin.c[a][pl];
+      gxo[a] = lx_out.c[a][pl];
+      guo[a] = lu_out.c[a][pl];
     }
-    if(!in_box(b, x)) {  // counted by the forward step; passes its adjoint through
+    const bool ok = live && in_box(b, x);
+    if(live && !ok) {  // counted by the forward step; passes its adjoint through
 #pragma unroll
       for(int a = 0; a < 3; ++a) {
         lx_in.c[a][p] = gxo[a];
         lu_in.c[a][p] = guo[a];
       }
-      continue;
+    }
+    if(!ok) {
+#pragma unroll
+      for(int a = 0; a < 3; ++a) {
+        x[a] = T(0.5);
+        u[a] = T(0);
+        gxo[a] = T(0);
+        guo[a] = T(0);
+      }
     }
     // ---- recompute the forward step
     Fwd<T> f;
-    gather_fields(b, E, B1, f, x);
+    gather_fields(b, NM8, f, x);
     BorisT<T> bs;
     boris_fwd(cfl, b.qm[s], f.EBE, f.EBB, x, u, bs, f.xn, f.un);
     wrap_pos(b, f.xn, f.xw);
@@ -525,8 +661,8 @@ __global__ void __launch_bounds__(DWG)
 
     // ---- deposit adjoint: J(F, W) -> x1, x2, xr -> x'' and u'
     T gF1[3], gW1[3], gF2[3], gW2[3];
-    zig_seg_adj(b, LJ, jsign, z.c1, z.F1, z.W1, gF1, gW1);
-    zig_seg_adj(b, LJ, jsign, z.c2, z.F2, z.W2, gF2, gW2);
+    zig_seg_adj(b, L4, jsign, z.c1, z.F1, z.W1, gF1, gW1);
+    zig_seg_adj(b, L4, jsign, z.c2, z.F2, z.W2, gF2, gW2);
     T gx1[3], gx2[3];
 #pragma unroll
     for(int d = 0; d < 3; ++d) {
@@ -589,7 +725,11 @@ __global__ void __launch_bounds__(DWG)
       gvel[d] = gu0[d] * cfl;    // v0 = vel cfl
       gEE[d] = gE0[d] * hq;      // E0 = EBE hq
     }
-    // ---- interpolation adjoint: scatter to node means, gradient in x
+    // ---- interpolation adjoint.  The particle's node-mean adjoints
+    // (dL/dE, dL/dB at the particle times the trilinear weights of its 8
+    // corners) are left as one record in its base cell's bucket; a pass over
+    // cells sums them (k_cell_scatter).  Gradient of the weights in x here,
+    // with the corner values loaded again (cache hits) instead of kept live.
     const T wx[2] = {T(1) - f.d[0], f.d[0]}, wy[2] = {T(1) - f.d[1], f.d[1]}, wz[2] = {T(1) - f.d[2], f.d[2]};
     T gd[3] = {T(0), T(0), T(0)};
 #pragma unroll
@@ -598,46 +738,104 @@ __global__ void __launch_bounds__(DWG)
       for(int jc = 0; jc < 2; ++jc)
 #pragma unroll
         for(int kc = 0; kc < 2; ++kc) {
-          const T w = wx[ic] * wy[jc] * wz[kc];
           const int64_t node = cidx(b.n, f.I[0][ic + 1], f.I[1][jc + 1], f.I[2][kc + 1]);
+          T v[6];
+          load_node(NM8 + node * 8, v);
+          T sum = T(0);
 #pragma unroll
-          for(int c = 0; c < 3; ++c) {
-            atomic_add(&ME.c[c][node], gEE[c] * w);
-            atomic_add(&MB.c[c][node], gEB[c] * w);
-          }
+          for(int c = 0; c < 3; ++c) sum += gEE[c] * v[c] + gEB[c] * v[3 + c];
+          // d(trilinear weight)/dd: the weight with its own axis factor replaced by -1 / +1
+          gd[0] += (ic ? sum : -sum) * (wy[jc] * wz[kc]);
+          gd[1] += (jc ? sum : -sum) * (wx[ic] * wz[kc]);
+          gd[2] += (kc ? sum : -sum) * (wx[ic] * wy[jc]);
         }
+    if(ok) {
+      const int64_t key = cidx(b.n, f.I[0][1], f.I[1][1], f.I[2][1]);
+      T* o = rec + (int64_t(cstart[key]) + pslot[p]) * 12;
 #pragma unroll
-    for(int c = 0; c < 3; ++c) {
-#pragma unroll
-      for(int jc = 0; jc < 2; ++jc)
-#pragma unroll
-        for(int kc = 0; kc < 2; ++kc) {
-          const T w = wy[jc] * wz[kc];
-          gd[0] += (gEE[c] * (f.mE[c][1][jc][kc] - f.mE[c][0][jc][kc]) +
-                    gEB[c] * (f.mB[c][1][jc][kc] - f.mB[c][0][jc][kc])) * w;
-        }
-#pragma unroll
-      for(int ic = 0; ic < 2; ++ic)
-#pragma unroll
-        for(int kc = 0; kc < 2; ++kc) {
-          const T w = wx[ic] * wz[kc];
-          gd[1] += (gEE[c] * (f.mE[c][ic][1][kc] - f.mE[c][ic][0][kc]) +
-                    gEB[c] * (f.mB[c][ic][1][kc] - f.mB[c][ic][0][kc])) * w;
-        }
-#pragma unroll
-      for(int ic = 0; ic < 2; ++ic)
-#pragma unroll
-        for(int jc = 0; jc < 2; ++jc) {
-          const T w = wx[ic] * wy[jc];
-          gd[2] += (gEE[c] * (f.mE[c][ic][jc][1] - f.mE[c][ic][jc][0]) +
-                    gEB[c] * (f.mB[c][ic][jc][1] - f.mB[c][ic][jc][0])) * w;
-        }
+      for(int c = 0; c < 3; ++c) {
+        o[c] = gEE[c];
+        o[3 + c] = gEB[c];
+        o[6 + c] = f.d[c];
+      }
     }
+    if(!ok) continue;
 #pragma unroll
     for(int a = 0; a < 3; ++a) {
       lx_in.c[a][p] = gx[a] + gd[a];
       lu_in.c[a][p] = gvel[a];
     }
+  }
+}
+
+// bucket slots: particle p of base cell c gets slot pslot[p] in c's bucket
+template<class T>
+__global__ void __launch_bounds__(DWG)
+  k_cell_count(DBox<T> b, G3<T> xin, uint32_t* __restrict__ count, uint32_t* __restrict__ pslot)
+{
+  const int64_t np = b.off[b.nsp];
+  for(int64_t p = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; p < np; p += int64_t(gridDim.x) * blockDim.x) {
+    const T x[3] = {xin.c[0][p], xin.c[1][p], xin.c[2][p]};
+    if(!in_box(b, x)) {
+      pslot[p] = UINT32_MAX;
+      continue;
+    }
+    int I[3];
+#pragma unroll
+    for(int a = 0; a < 3; ++a) {
+      const uint32_t ii = static_cast<uint32_t>(x[a] + T(3));
+      I[a] = wrap1(wrap1(int(ii) - 3, b.n[a]), b.n[a]);
+    }
+    pslot[p] = atomicAdd(&count[cidx(b.n, I[0], I[1], I[2])], 1u);
+  }
+}
+
+// one thread per cell: sum its particles' records into the 8 corner nodes'
+// node-mean adjoints (48 atomics per occupied cell instead of per particle)
+template<class T>
+__global__ void __launch_bounds__(DWG)
+  k_cell_scatter(DBox<T> b, const T* __restrict__ rec, const uint32_t* __restrict__ cstart,
+                 const uint32_t* __restrict__ count, T* M8, int64_t ncell)
+{
+  for(int64_t c = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; c < ncell; c += int64_t(gridDim.x) * blockDim.x) {
+    const uint32_t cnt = count[c];
+    if(cnt == 0) continue;
+    const int64_t r0 = cstart[c];
+    T acc[8][6];
+#pragma unroll
+    for(int m = 0; m < 8; ++m)
+#pragma unroll
+      for(int q = 0; q < 6; ++q) acc[m][q] = T(0);
+    for(uint32_t r = 0; r < cnt; ++r) {
+      const T* o = rec + (r0 + r) * 12;
+      T g[6], d[3];
+#pragma unroll
+      for(int q = 0; q < 6; ++q) g[q] = o[q];
+#pragma unroll
+      for(int a = 0; a < 3; ++a) d[a] = o[6 + a];
+      const T wx[2] = {T(1) - d[0], d[0]}, wy[2] = {T(1) - d[1], d[1]}, wz[2] = {T(1) - d[2], d[2]};
+#pragma unroll
+      for(int ic = 0; ic < 2; ++ic)
+#pragma unroll
+        for(int jc = 0; jc < 2; ++jc)
+#pragma unroll
+          for(int kc = 0; kc < 2; ++kc) {
+            const T w = wx[ic] * wy[jc] * wz[kc];
+#pragma unroll
+            for(int q = 0; q < 6; ++q) acc[(ic * 2 + jc) * 2 + kc][q] += g[q] * w;
+          }
+    }
+    const int k = int(c % b.n[2]), j = int((c / b.n[2]) % b.n[1]), i = int(c / (int64_t(b.n[1]) * b.n[2]));
+#pragma unroll
+    for(int ic = 0; ic < 2; ++ic)
+#pragma unroll
+      for(int jc = 0; jc < 2; ++jc)
+#pragma unroll
+        for(int kc = 0; kc < 2; ++kc) {
+          const int64_t node = cidx(b.n, wrap1(i + ic, b.n[0]), wrap1(j + jc, b.n[1]), wrap1(k + kc, b.n[2]));
+#pragma unroll
+          for(int q = 0; q < 6; ++q) atomic_add(&M8[node * 8 + q], acc[(ic * 2 + jc) * 2 + kc][q]);
+        }
   }
 }
 
@@ -648,13 +846,22 @@ struct DiffImpl {
   rk_diff_desc d;
   int64_t ncell = 0, np = 0;
   size_t vb = 4;
-  void* scratch[7][3] = {};  // B1, J, Jt, LB2, LE, ME, MB
+  void* scratch[5][3] = {};  // B1, J, Jt, LB2, LE (planar)
+  void* j4 = nullptr;        // [cell][4] current (deposit) / its adjoint (gather)
+  void* m8 = nullptr;        // [cell][8] node-mean adjoints of E and B
+  void* nm8 = nullptr;       // [cell][8] node means of E and B1
+  uint32_t* ccount = nullptr;  // particles per base cell
+  uint32_t* cstart = nullptr;  // bucket offsets (exclusive sum of ccount)
+  uint32_t* pslot = nullptr;   // slot of each particle in its cell's bucket
+  void* rec = nullptr;         // [particle][12] interpolation-adjoint records, bucketed by cell
+  void* scan_tmp = nullptr;
+  size_t scan_bytes = 0;
   unsigned long long* err = nullptr;
 };
 
 namespace {
 
-enum { S_B1 = 0, S_J, S_JT, S_LB2, S_LE, S_ME, S_MB };
+enum { S_B1 = 0, S_J, S_JT, S_LB2, S_LE };
 
 template<class T>
 DBox<T> make_box(const DiffImpl& h)
@@ -693,14 +900,17 @@ int dfail(int code, const char* msg)
 }
 
 template<class T>
-int fill3(const DiffImpl& h, G3<T> g, hipStream_t st)
+int fill_n(T* p, int64_t n, hipStream_t st)
 {
-  const unsigned blocks = unsigned(std::min<int64_t>((h.ncell + DWG - 1) / DWG, 4096));
-  for(int c = 0; c < 3; ++c) {
-    hipLaunchKernelGGL(k_dfill<T>, dim3(blocks), dim3(DWG), 0, st, g.c[c], h.ncell, T(0));
-    RK_HIP(hipGetLastError());
-  }
+  const unsigned blocks = unsigned(std::max<int64_t>(1, std::min<int64_t>((n + DWG - 1) / DWG, 4096)));
+  hipLaunchKernelGGL(k_dfill<T>, dim3(blocks), dim3(DWG), 0, st, p, n, T(0));
+  RK_HIP(hipGetLastError());
   return RK_OK;
+}
+
+inline unsigned cblocks(const DiffImpl& h)
+{
+  return unsigned(std::max<int64_t>(1, std::min<int64_t>((h.ncell + DWG - 1) / DWG, 4096)));
 }
 
 inline dim3 grid_of(const DiffImpl& h) { return dim3(unsigned(h.d.n[1]), unsigned(h.d.n[0])); }
@@ -734,12 +944,18 @@ int step_t(DiffImpl& h, const rk_diff_state* in, rk_diff_state* out, hipStream_t
   // B1 = B + cfl/2 curlE(E)
   hipLaunchKernelGGL(k_dcurl_b<T>, grid_of(h), block_of(h), 0, st, b, g3s<T>(h, S_B1), B, E, half);
   RK_HIP(hipGetLastError());
-  if(int rc = fill3<T>(h, g3s<T>(h, S_J), st)) return rc;
+  T* J4 = static_cast<T*>(h.j4);
+  T* NM8 = static_cast<T*>(h.nm8);
+  if(int rc = fill_n<T>(J4, 4 * h.ncell, st)) return rc;
+  hipLaunchKernelGGL(k_node_means<T>, grid_of(h), block_of(h), 0, st, b, E, g3s<T>(h, S_B1), NM8);
+  RK_HIP(hipGetLastError());
   if(h.np > 0) {
-    hipLaunchKernelGGL(k_dpush<T>, dim3(pblocks(h)), dim3(DWG), 0, st, b, E, g3s<T>(h, S_B1), g3<T>(in->x),
-                       g3<T>(in->u), g3<T>(out->x), g3<T>(out->u), g3s<T>(h, S_J), h.err);
+    hipLaunchKernelGGL(k_dpush<T>, dim3(pblocks(h)), dim3(DWG), 0, st, b, NM8, g3<T>(in->x),
+                       g3<T>(in->u), g3<T>(out->x), g3<T>(out->u), J4, h.err);
     RK_HIP(hipGetLastError());
   }
+  hipLaunchKernelGGL(k_j4_split<T>, dim3(cblocks(h)), dim3(DWG), 0, st, J4, g3s<T>(h, S_J), h.ncell);
+  RK_HIP(hipGetLastError());
   int jf = S_J;
   if(int rc = filter_passes<T>(h, b, S_J, st, jf)) return rc;
   // B2 = B1 + cfl/2 curlE(E), written to out.B (E is still the input E)
@@ -776,17 +992,32 @@ int adjoint_t(DiffImpl& h, const rk_diff_state* in, const rk_diff_state* ao, rk_
     RK_HIP(hipMemcpyAsync(h.scratch[S_J][c], ao->E[c], h.vb * size_t(h.ncell), hipMemcpyDeviceToDevice, st));
   int lj = S_J;
   if(int rc = filter_passes<T>(h, b, S_J, st, lj)) return rc;
-  if(int rc = fill3<T>(h, g3s<T>(h, S_ME), st)) return rc;
-  if(int rc = fill3<T>(h, g3s<T>(h, S_MB), st)) return rc;
+  T* L4 = static_cast<T*>(h.j4);
+  T* M8 = static_cast<T*>(h.m8);
+  hipLaunchKernelGGL(k_j4_pack<T>, dim3(cblocks(h)), dim3(DWG), 0, st, g3s<T>(h, lj), L4, h.ncell);
+  RK_HIP(hipGetLastError());
+  if(int rc = fill_n<T>(M8, 8 * h.ncell, st)) return rc;
+  T* NM8 = static_cast<T*>(h.nm8);
+  hipLaunchKernelGGL(k_node_means<T>, grid_of(h), block_of(h), 0, st, b, E, g3s<T>(h, S_B1), NM8);
+  RK_HIP(hipGetLastError());
   if(h.np > 0) {
-    hipLaunchKernelGGL(k_dpush_adj<T>, dim3(pblocks(h)), dim3(DWG), 0, st, b, E, g3s<T>(h, S_B1), g3<T>(in->x),
-                       g3<T>(in->u), g3s<T>(h, lj), T(-1), g3<T>(ao->x), g3<T>(ao->u), g3<T>(ai->x), g3<T>(ai->u),
-                       g3s<T>(h, S_ME), g3s<T>(h, S_MB));
+    // bucket the particles by base cell: counts, offsets, then the records
+    RK_HIP(hipMemsetAsync(h.ccount, 0, sizeof(uint32_t) * size_t(h.ncell), st));
+    hipLaunchKernelGGL(k_cell_count<T>, dim3(pblocks(h)), dim3(DWG), 0, st, b, g3<T>(in->x), h.ccount, h.pslot);
+    RK_HIP(hipGetLastError());
+    size_t bytes = h.scan_bytes;
+    RK_HIP(hipcub::DeviceScan::ExclusiveSum(h.scan_tmp, bytes, h.ccount, h.cstart, int(h.ncell), st));
+    T* rec = static_cast<T*>(h.rec);
+    hipLaunchKernelGGL(k_dpush_adj<T>, dim3(pblocks(h)), dim3(DWG), 0, st, b, NM8, g3<T>(in->x),
+                       g3<T>(in->u), L4, T(-1), g3<T>(ao->x), g3<T>(ao->u), g3<T>(ai->x), g3<T>(ai->u), h.cstart,
+                       h.pslot, rec);
+    RK_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_cell_scatter<T>, dim3(cblocks(h)), dim3(DWG), 0, st, b, rec, h.cstart, h.ccount, M8,
+                       h.ncell);
     RK_HIP(hipGetLastError());
   }
   // node-mean adjoints: lambda_E += NM_E^T(ME), lambda_B1 = LB2 + NM_B^T(MB)
-  hipLaunchKernelGGL(k_nodemean_adj<T>, grid_of(h), block_of(h), 0, st, b, g3s<T>(h, S_LE), g3s<T>(h, S_LB2),
-                     g3s<T>(h, S_ME), g3s<T>(h, S_MB));
+  hipLaunchKernelGGL(k_nodemean_adj<T>, grid_of(h), block_of(h), 0, st, b, g3s<T>(h, S_LE), g3s<T>(h, S_LB2), M8);
   RK_HIP(hipGetLastError());
   // B1 = B + cfl/2 curlE(E): lambda_B = lambda_B1, lambda_E -= cfl/2 curlB(lambda_B1)
   for(int c = 0; c < 3; ++c)
@@ -839,8 +1070,19 @@ int rk_diff_create(rk_diff** out, const rk_diff_desc* desc)
     h->np += d.count[s];
   }
   hipError_t e = hipSuccess;
-  for(int i = 0; i < 7 && e == hipSuccess; ++i)
+  for(int i = 0; i < 5 && e == hipSuccess; ++i)
     for(int c = 0; c < 3 && e == hipSuccess; ++c) e = hipMalloc(&h->scratch[i][c], h->vb * size_t(h->ncell));
+  if(e == hipSuccess) e = hipMalloc(&h->j4, 4 * h->vb * size_t(h->ncell));
+  if(e == hipSuccess) e = hipMalloc(&h->m8, 8 * h->vb * size_t(h->ncell));
+  if(e == hipSuccess) e = hipMalloc(&h->nm8, 8 * h->vb * size_t(h->ncell));
+  if(e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&h->ccount), sizeof(uint32_t) * size_t(h->ncell));
+  if(e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&h->cstart), sizeof(uint32_t) * size_t(h->ncell));
+  if(e == hipSuccess)
+    e = hipMalloc(reinterpret_cast<void**>(&h->pslot), sizeof(uint32_t) * size_t(std::max<int64_t>(h->np, 1)));
+  if(e == hipSuccess) e = hipMalloc(&h->rec, 12 * h->vb * size_t(std::max<int64_t>(h->np, 1)));
+  if(e == hipSuccess)
+    e = hipcub::DeviceScan::ExclusiveSum(nullptr, h->scan_bytes, h->ccount, h->cstart, int(h->ncell));
+  if(e == hipSuccess) e = hipMalloc(&h->scan_tmp, std::max<size_t>(h->scan_bytes, 16));
   if(e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&h->err), sizeof(unsigned long long));
   if(e == hipSuccess) e = hipMemset(h->err, 0, sizeof(unsigned long long));
   if(e != hipSuccess) {
@@ -857,6 +1099,12 @@ int rk_diff_destroy(rk_diff* h)
   for(auto& s : h->scratch)
     for(auto& p : s)
       if(p) (void)hipFree(p);
+  if(h->j4) (void)hipFree(h->j4);
+  if(h->m8) (void)hipFree(h->m8);
+  if(h->nm8) (void)hipFree(h->nm8);
+  for(void* q : {static_cast<void*>(h->ccount), static_cast<void*>(h->cstart), static_cast<void*>(h->pslot), h->rec,
+                 h->scan_tmp})
+    if(q) (void)hipFree(q);
   if(h->err) (void)hipFree(h->err);
   delete h;
   return RK_OK;
