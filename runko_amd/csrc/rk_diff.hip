@@ -485,40 +485,6 @@ __device__ inline void zig_values(const T F[3], const T W[3], T v[7][3])
 }
 
 template<class T>
-__device__ inline void zig_store(const DBox<T>& b, T* J4, const int c[3][2], const T v[7][3])
-{
-#pragma unroll
-  for(int m = 0; m < 7; ++m) {
-    const int64_t q = cidx(b.n, c[0][ZOFF[m][0]], c[1][ZOFF[m][1]], c[2][ZOFF[m][2]]);
-#pragma unroll
-    for(int comp = 0; comp < 3; ++comp)
-      if(!(m == 1 && comp == 0) && !(m == 2 && comp == 1) && !(m == 3 && comp == 2) && !(m == 4 && comp != 0) &&
-         !(m == 5 && comp != 1) && !(m == 6 && comp != 2))
-        atomic_add(&J4[q * 4 + comp], v[m][comp]);
-  }
-}
-
-// both segments of a particle that stays in its cell land on the same 7
-// nodes: their sums take 12 atomics instead of 24
-template<class T>
-__device__ inline void zig_deposit(const DBox<T>& b, T* J4, const Zig<T>& z)
-{
-  T v1[7][3], v2[7][3];
-  zig_values(z.F1, z.W1, v1);
-  zig_values(z.F2, z.W2, v2);
-  if(z.fi1[0] == z.fi2[0] && z.fi1[1] == z.fi2[1] && z.fi1[2] == z.fi2[2]) {
-#pragma unroll
-    for(int m = 0; m < 7; ++m)
-#pragma unroll
-      for(int comp = 0; comp < 3; ++comp) v1[m][comp] = v1[m][comp] + v2[m][comp];
-    zig_store(b, J4, z.c1, v1);
-  } else {
-    zig_store(b, J4, z.c1, v1);
-    zig_store(b, J4, z.c2, v2);
-  }
-}
-
-template<class T>
 __device__ inline int species_of(const DBox<T>& b, int64_t p)
 {
   int s = 0;
@@ -528,8 +494,8 @@ __device__ inline int species_of(const DBox<T>& b, int64_t p)
 
 template<class T>
 __global__ void __launch_bounds__(DWG)
-  k_dpush(DBox<T> b, const T* __restrict__ NM8, G3<T> xin, G3<T> uin, G3<T> xout, G3<T> uout, T* J4,
-          unsigned long long* err)
+  k_dpush(DBox<T> b, const T* __restrict__ NM8, G3<T> xin, G3<T> uin, G3<T> xout, G3<T> uout,
+          uint32_t* __restrict__ count, uint32_t* __restrict__ slot2, unsigned long long* err)
 {
   const int64_t np = b.off[b.nsp];
   for(int64_t p = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; p < np; p += int64_t(gridDim.x) * blockDim.x) {
@@ -547,6 +513,8 @@ __global__ void __launch_bounds__(DWG)
         xout.c[a][p] = x[a];
         uout.c[a][p] = u[a];
       }
+      slot2[2 * p] = UINT32_MAX;
+      slot2[2 * p + 1] = UINT32_MAX;
       continue;
     }
     Fwd<T> f;
@@ -556,11 +524,98 @@ __global__ void __launch_bounds__(DWG)
     wrap_pos(b, f.xn, f.xw);
     Zig<T> z;
     zig_setup(b, f.xw, f.un, b.q[s], z);
-    zig_deposit(b, J4, z);
+    // a slot in the bucket of each segment's cell (one when both share it)
+    const bool same = z.fi1[0] == z.fi2[0] && z.fi1[1] == z.fi2[1] && z.fi1[2] == z.fi2[2];
+    slot2[2 * p] = atomicAdd(&count[cidx(b.n, z.c1[0][0], z.c1[1][0], z.c1[2][0])], 1u);
+    slot2[2 * p + 1] = same ? UINT32_MAX : atomicAdd(&count[cidx(b.n, z.c2[0][0], z.c2[1][0], z.c2[2][0])], 1u);
 #pragma unroll
     for(int a = 0; a < 3; ++a) {
       xout.c[a][p] = f.xw[a];
       uout.c[a][p] = f.un[a];
+    }
+  }
+}
+
+// the 12 structurally non-zero (node, component) values of zig_values, in
+// ZOFF order: node 0 x,y,z; node 1 y,z; node 2 x,z; node 3 x,y; node 4 x;
+// node 5 y; node 6 z
+template<class T>
+__device__ inline void pack12(const T v[7][3], T* o)
+{
+  o[0] = v[0][0];
+  o[1] = v[0][1];
+  o[2] = v[0][2];
+  o[3] = v[1][1];
+  o[4] = v[1][2];
+  o[5] = v[2][0];
+  o[6] = v[2][2];
+  o[7] = v[3][0];
+  o[8] = v[3][1];
+  o[9] = v[4][0];
+  o[10] = v[5][1];
+  o[11] = v[6][2];
+}
+
+__constant__ const int P12[12][2] = {{0, 0}, {0, 1}, {0, 2}, {1, 1}, {1, 2}, {2, 0},
+                                     {2, 2}, {3, 0}, {3, 1}, {4, 0}, {5, 1}, {6, 2}};
+
+// current records: each segment's 12 values in its cell's bucket, from the
+// pushed state (the zigzag geometry is recomputed from x'', u')
+template<class T>
+__global__ void __launch_bounds__(DWG)
+  k_dep_records(DBox<T> b, G3<T> xout, G3<T> uout, const uint32_t* __restrict__ cstart,
+                const uint32_t* __restrict__ slot2, T* __restrict__ rec)
+{
+  const int64_t np = b.off[b.nsp];
+  for(int64_t p = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; p < np; p += int64_t(gridDim.x) * blockDim.x) {
+    const uint32_t s1 = slot2[2 * p], s2 = slot2[2 * p + 1];
+    if(s1 == UINT32_MAX) continue;
+    const int s = species_of(b, p);
+    const T xw[3] = {xout.c[0][p], xout.c[1][p], xout.c[2][p]};
+    const T un[3] = {uout.c[0][p], uout.c[1][p], uout.c[2][p]};
+    Zig<T> z;
+    zig_setup(b, xw, un, b.q[s], z);
+    T v1[7][3], v2[7][3];
+    zig_values(z.F1, z.W1, v1);
+    zig_values(z.F2, z.W2, v2);
+    if(s2 == UINT32_MAX) {
+#pragma unroll
+      for(int m = 0; m < 7; ++m)
+#pragma unroll
+        for(int c = 0; c < 3; ++c) v1[m][c] = v1[m][c] + v2[m][c];
+    } else {
+      pack12(v2, rec + (int64_t(cstart[cidx(b.n, z.c2[0][0], z.c2[1][0], z.c2[2][0])]) + s2) * 12);
+    }
+    pack12(v1, rec + (int64_t(cstart[cidx(b.n, z.c1[0][0], z.c1[1][0], z.c1[2][0])]) + s1) * 12);
+  }
+}
+
+// one thread per cell: sum its segment records, 12 atomics into the
+// interleaved current at the cell's 7 zigzag nodes
+template<class T>
+__global__ void __launch_bounds__(DWG)
+  k_cell_deposit(DBox<T> b, const T* __restrict__ rec, const uint32_t* __restrict__ cstart,
+                 const uint32_t* __restrict__ count, T* J4, int64_t ncell)
+{
+  for(int64_t c = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; c < ncell; c += int64_t(gridDim.x) * blockDim.x) {
+    const uint32_t cnt = count[c];
+    if(cnt == 0) continue;
+    const int64_t r0 = cstart[c];
+    T acc[12];
+#pragma unroll
+    for(int q = 0; q < 12; ++q) acc[q] = T(0);
+    for(uint32_t r = 0; r < cnt; ++r) {
+      const T* o = rec + (r0 + r) * 12;
+#pragma unroll
+      for(int q = 0; q < 12; ++q) acc[q] += o[q];
+    }
+    const int k = int(c % b.n[2]), j = int((c / b.n[2]) % b.n[1]), i = int(c / (int64_t(b.n[1]) * b.n[2]));
+#pragma unroll
+    for(int q = 0; q < 12; ++q) {
+      const int m = P12[q][0];
+      const int64_t node = cidx(b.n, wrap1(i + ZOFF[m][0], b.n[0]), wrap1(j + ZOFF[m][1], b.n[1]),
+                                wrap1(k + ZOFF[m][2], b.n[2]));
+      atomic_add(&J4[node * 4 + P12[q][1]], acc[q]);
     }
   }
 }
@@ -852,8 +907,8 @@ struct DiffImpl {
   void* nm8 = nullptr;       // [cell][8] node means of E and B1
   uint32_t* ccount = nullptr;  // particles per base cell
   uint32_t* cstart = nullptr;  // bucket offsets (exclusive sum of ccount)
-  uint32_t* pslot = nullptr;   // slot of each particle in its cell's bucket
-  void* rec = nullptr;         // [particle][12] interpolation-adjoint records, bucketed by cell
+  uint32_t* pslot = nullptr;   // bucket slots: per particle (adjoint) or per zigzag segment (forward)
+  void* rec = nullptr;         // [record][12] bucketed by cell: current segments or interpolation adjoints
   void* scan_tmp = nullptr;
   size_t scan_bytes = 0;
   unsigned long long* err = nullptr;
@@ -950,8 +1005,19 @@ int step_t(DiffImpl& h, const rk_diff_state* in, rk_diff_state* out, hipStream_t
   hipLaunchKernelGGL(k_node_means<T>, grid_of(h), block_of(h), 0, st, b, E, g3s<T>(h, S_B1), NM8);
   RK_HIP(hipGetLastError());
   if(h.np > 0) {
-    hipLaunchKernelGGL(k_dpush<T>, dim3(pblocks(h)), dim3(DWG), 0, st, b, NM8, g3<T>(in->x),
-                       g3<T>(in->u), g3<T>(out->x), g3<T>(out->u), J4, h.err);
+    // push, then the current through cell buckets: slots, offsets, records, per-cell sums
+    RK_HIP(hipMemsetAsync(h.ccount, 0, sizeof(uint32_t) * size_t(h.ncell), st));
+    hipLaunchKernelGGL(k_dpush<T>, dim3(pblocks(h)), dim3(DWG), 0, st, b, NM8, g3<T>(in->x), g3<T>(in->u),
+                       g3<T>(out->x), g3<T>(out->u), h.ccount, h.pslot, h.err);
+    RK_HIP(hipGetLastError());
+    size_t bytes = h.scan_bytes;
+    RK_HIP(hipcub::DeviceScan::ExclusiveSum(h.scan_tmp, bytes, h.ccount, h.cstart, int(h.ncell), st));
+    T* rec = static_cast<T*>(h.rec);
+    hipLaunchKernelGGL(k_dep_records<T>, dim3(pblocks(h)), dim3(DWG), 0, st, b, g3<T>(out->x), g3<T>(out->u),
+                       h.cstart, h.pslot, rec);
+    RK_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_cell_deposit<T>, dim3(cblocks(h)), dim3(DWG), 0, st, b, rec, h.cstart, h.ccount, J4,
+                       h.ncell);
     RK_HIP(hipGetLastError());
   }
   hipLaunchKernelGGL(k_j4_split<T>, dim3(cblocks(h)), dim3(DWG), 0, st, J4, g3s<T>(h, S_J), h.ncell);
@@ -1078,8 +1144,8 @@ int rk_diff_create(rk_diff** out, const rk_diff_desc* desc)
   if(e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&h->ccount), sizeof(uint32_t) * size_t(h->ncell));
   if(e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&h->cstart), sizeof(uint32_t) * size_t(h->ncell));
   if(e == hipSuccess)
-    e = hipMalloc(reinterpret_cast<void**>(&h->pslot), sizeof(uint32_t) * size_t(std::max<int64_t>(h->np, 1)));
-  if(e == hipSuccess) e = hipMalloc(&h->rec, 12 * h->vb * size_t(std::max<int64_t>(h->np, 1)));
+    e = hipMalloc(reinterpret_cast<void**>(&h->pslot), 2 * sizeof(uint32_t) * size_t(std::max<int64_t>(h->np, 1)));
+  if(e == hipSuccess) e = hipMalloc(&h->rec, 2 * 12 * h->vb * size_t(std::max<int64_t>(h->np, 1)));
   if(e == hipSuccess)
     e = hipcub::DeviceScan::ExclusiveSum(nullptr, h->scan_bytes, h->ccount, h->cstart, int(h->ncell));
   if(e == hipSuccess) e = hipMalloc(&h->scan_tmp, std::max<size_t>(h->scan_bytes, 16));
