@@ -238,6 +238,31 @@ int rk_gauss_residual(rk_world* w, int32_t filter_passes, double* out, void* str
  * (reference behaviour undefined; clamped here).  Synchronous. */
 int rk_error_counters(rk_world* w, int64_t* out2, void* stream);
 
+/* ---- Snapshot packing (the "RNKO" v3 output files) ----------------------
+ * Device side of the reference's MPI-IO writers (src/runko/io/snapshots/,
+ * bound at src/runko/bindings/pyemf.c++:271-296); the host writes the
+ * 512-byte header (mpiio_header.h:55-103) and the arrays. */
+
+/* mpiio::FieldsWriter::pack_tile for every tile (mpiio_fields.c++:218-321):
+ * out = float [9 + min(nspecies, 5)][nz][ny][nx] in file order (z outermost),
+ * n_axis = tiles_axis * max(1, cells_axis / stride): E and B at the stride
+ * points, J summed over stride^3 cells, then per-species particle counts at
+ * floor((x - tile mins) / stride).  1 <= stride <= cells per tile. */
+int rk_snapshot_fields(rk_world* w, int32_t stride, int32_t nspecies, float* out, void* stream);
+
+/* mpiio::SpectraWriter::histogram_tile for every tile (mpiio_spectra.c++:130-209):
+ * out = float [4 * min(nspecies, 3)][tiles_z][tiles_y][tiles_x * nxt][nbins],
+ * nxt = max(1, cells_x / stride): per species log10|u| bins on [umin, umax]
+ * and beta_x, beta_y, beta_z bins on [-1, 1], values outside in the edge bins. */
+int rk_snapshot_spectra(rk_world* w, int32_t stride, int32_t nbins, float umin, float umax, int32_t nspecies,
+                        float* out, void* stream);
+
+/* pic::Tile::EB_linear_1st_interpolator at n global positions of one tile
+ * (emf/yee_lattice_interpolate_linear_1st.h:57-138, used by
+ * mpiio_particles.c++:226-251): E and B written to the six output arrays. */
+int rk_interpolate_at(rk_world* w, int32_t tile, int64_t n, const float* const pos[3], float* const E[3],
+                      float* const B[3], void* stream);
+
 /* ---- Differentiable periodic box --------------------------------------
  * No counterpart in the reference (runko has no differentiation).  One
  * periodic box of n[0] x n[1] x n[2] cells with the same discrete step as one

@@ -16,7 +16,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "librunko_amd.so")
-SOURCES = ["rk_api.hip", "rk_fields.hip", "rk_particles.hip", "rk_diag.hip", "rk_wall.hip", "rk_diff.hip"]
+SOURCES = ["rk_api.hip", "rk_fields.hip", "rk_particles.hip", "rk_diag.hip", "rk_wall.hip", "rk_diff.hip", "rk_snapshot.hip"]
 ARCH = os.environ.get("RUNKO_AMD_ARCH", "gfx950")
 
 FLAGS = [

@@ -128,6 +128,9 @@ def lib():
     L.rk_diff_step_adjoint.argtypes = [vp, ctypes.POINTER(DiffState), ctypes.POINTER(DiffState),
                                        ctypes.POINTER(DiffState), vp]
     L.rk_diff_error_count.argtypes = [vp, ctypes.POINTER(i64), vp]
+    L.rk_snapshot_fields.argtypes = [vp, i32, i32, vp, vp]
+    L.rk_snapshot_spectra.argtypes = [vp, i32, i32, ctypes.c_float, ctypes.c_float, i32, vp, vp]
+    L.rk_interpolate_at.argtypes = [vp, i32, i64, P3, P3, P3, vp]
     _lib = L
     return _lib
 
@@ -153,7 +156,7 @@ EXPORTED = [
     "rk_energies", "rk_gauss_residual", "rk_error_counters", "rk_push_e_add_current", "rk_profile_enable",
     "rk_profile_read", "rk_world_set_mailbox", "rk_world_set_current_staging", "rk_reflect_particles",
     "rk_apply_edge_bc", "rk_diff_create", "rk_diff_destroy", "rk_diff_step", "rk_diff_step_adjoint",
-    "rk_diff_error_count",
+    "rk_diff_error_count", "rk_snapshot_fields", "rk_snapshot_spectra", "rk_interpolate_at",
 ]
 
 PROF_CATEGORIES = ["fused_particle_step", "push", "deposit", "pack", "append", "fdtd", "filter", "ghost", "other"]

@@ -65,6 +65,9 @@ class Simulation:
         self._total_interval_time = 0
         self._pending = []
         self._warned = set()
+        self._emf_writer = None
+        self._prtcl_writers = {}
+        self._spectra_writer = None
         self._logger = logging.getLogger("runko_amd.Simulation")
         out = self._io_config["outdir"]
         self._io_config["kinetic_energy_path"] = f"{out}/average_kinetic_energy.txt"
@@ -115,9 +118,31 @@ class Simulation:
                 path = self._io_config["average_B_energy_density_path" if "B" in what else "average_E_energy_density_path"]
             with open(path, "a") as f:
                 f.write(line)
-        elif what in ("emf_snapshot", "prtcl_snapshot", "spectra_snapshot", "ram_usage"):
+        elif what in ("emf_snapshot", "prtcl_snapshot", "spectra_snapshot"):
+            # runko/simulation.py:96-146,257-276: writers built on first use
+            from .snapshots import MpiioFieldsWriter, MpiioParticlesWriter, MpiioSpectraWriter
+            io, g = self._io_config, self._tile_grid
+            dims = (g._Nx, g._NxMesh, g._Ny, g._NyMesh, g._Nz, g._NzMesh)
+            pathlib.Path(io["outdir"]).mkdir(parents=True, exist_ok=True)
+            if what == "emf_snapshot":
+                if self._emf_writer is None:
+                    self._emf_writer = MpiioFieldsWriter(io["outdir"], *dims, io["stride"], io.get("nspecies", 2))
+                self._emf_writer.write(g, self.lap)
+            elif what == "prtcl_snapshot":
+                if not self._prtcl_writers:
+                    self._prtcl_writers = {sp: MpiioParticlesWriter(io["outdir"], io.get("n_prtcls", 0), sp)
+                                           for sp in range(io.get("nspecies", 2))}
+                for wr in self._prtcl_writers.values():
+                    wr.write(g, self.lap)
+            else:
+                if self._spectra_writer is None:
+                    self._spectra_writer = MpiioSpectraWriter(
+                        io["outdir"], *dims, io.get("spectra_stride", io["stride"]), io["spectra_nbins"],
+                        io["spectra_umin"], io["spectra_umax"], io.get("nspecies", 2))
+                self._spectra_writer.write(g, self.lap)
+        elif what == "ram_usage":
             if what not in self._warned:
-                self._logger.warning(f"io_{what}: snapshot/diagnostic output is not part of runko_amd; skipped")
+                self._logger.warning("io_ram_usage: host memory diagnostics are not part of runko_amd; skipped")
                 self._warned.add(what)
         else:
             raise AttributeError(f"io_{what} is not supported IO type.")

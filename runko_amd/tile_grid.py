@@ -100,6 +100,11 @@ class TileGrid:
         self.world = world
         return world
 
+    @property
+    def _corgi_grid(self):
+        """The grid the snapshot writers take (the reference passes its corgi grid)."""
+        return self
+
     def configure_simulation(self, config):
         if getattr(config, "n_laps") is None:
             raise RuntimeError("Can not configure simulation without: n_laps")
@@ -111,8 +116,15 @@ class TileGrid:
             else:
                 break
         outdir = getattr(config, "io_outdir") or "runko-output"
-        io_config = dict(outdir=outdir, nspecies=nspecies,
-                         laps_in_timer_statistics=getattr(config, "io_n_laps_in_timer_stats"))
+        # runko/tile_grid.py:200-209
+        stride = getattr(config, "io_grid_stride") or 1
+        io_config = dict(outdir=outdir, nspecies=nspecies, stride=stride,
+                         n_prtcls=getattr(config, "n_sampled_prtcls") or 0,
+                         laps_in_timer_statistics=getattr(config, "io_n_laps_in_timer_stats"),
+                         spectra_nbins=getattr(config, "io_n_spectra_bins") or 200,
+                         spectra_umin=getattr(config, "io_spectra_umin") or 1e-4,
+                         spectra_umax=getattr(config, "io_spectra_umax") or 1e3,
+                         spectra_stride=getattr(config, "io_spectra_stride") or stride)
         mode = getattr(config, "execution") or "plain"
         return Simulation(self, Simulation._im_not_user, Nt=config.n_laps, io_config=io_config,
                           verbose=config.verbose, mode=mode)

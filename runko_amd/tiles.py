@@ -462,4 +462,8 @@ def make_modules():
     emf3 = _Namespace()
     emf3.Tile = EmfTile
     emf3.edge_bc = edge_bc
+    from .snapshots import MpiioFieldsWriter, MpiioParticlesWriter, MpiioSpectraWriter
+    emf3.MpiioFieldsWriter = MpiioFieldsWriter
+    emf3.MpiioParticlesWriter = MpiioParticlesWriter
+    emf3.MpiioSpectraWriter = MpiioSpectraWriter
     return pic3, emf3
