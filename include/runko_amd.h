@@ -122,6 +122,17 @@ int rk_world_set_mailbox(rk_world* w, int64_t per_brick, float* const pos[3], fl
  * into J.  Required by rk_particle_step_fused. */
 int rk_world_set_current_staging(rk_world* w, float* buf, int64_t stride);
 
+/* Numerical schemes of every tile (the configuration keys particle_pusher and
+ * field_propagator, src/runko/pic/tile.c++:74-86,337-350 and
+ * src/runko/emf/tile.c++:29-141,363-372): particle_pusher 0 boris,
+ * 1 higuera_cary (particle_higuera_cary.h:15-77), 2 faraday
+ * (particle_faraday.h:37-109); stencil = null for the fdtd2 B push, else 45
+ * floats [axis x,y,z][row 0..2][column 0..4], the StencilAxisCoeffs matrices
+ * with M[0][0] normalised (stencil_coefficients.h:42-60), for push_b_stencil
+ * (emf/yee_lattice_stencil.c++:17-297).  The fused particle step implements
+ * boris only (RK_ESTATE otherwise). */
+int rk_world_set_schemes(rk_world* w, int32_t particle_pusher, const float* stencil);
+
 /* Reflecting wall (conducting piston) normal to x, pic::reflector_wall
  * (src/runko/pic/reflector_wall.h:13-25): position, velocity beta, Lorentz factor. */
 typedef struct rk_reflector_wall {

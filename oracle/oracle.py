@@ -41,6 +41,8 @@ def lib():
             f.argtypes = [vp, ip]
             f.restype = ip
         L.orc_comm_local.argtypes = [vp, ip]
+        L.orc_set_pusher.argtypes = [vp, ip]
+        L.orc_set_stencil.argtypes = [vp, vp]
         L.orc_lap.argtypes = [vp, ctypes.c_long]
         L.orc_set_fields.argtypes = [vp, ip, vp]
         L.orc_get_fields_halo.argtypes = [vp, ip, vp]
@@ -89,6 +91,19 @@ class OracleWorld:
         if getattr(self, "_h", None):
             lib().orc_free(self._h)
             self._h = None
+
+    def set_pusher(self, name):
+        """boris, higuera_cary or faraday for every tile (pic/tile.c++:74-86)."""
+        _check(lib().orc_set_pusher(self._h, {"boris": 0, "higuera_cary": 1, "faraday": 2}[name]))
+
+    def set_stencil(self, coeffs):
+        """B propagator of every tile: [3 axes][3][5] float32 stencil matrices
+        (stencil_matrices), or None for fdtd2."""
+        if coeffs is None:
+            _check(lib().orc_set_stencil(self._h, None))
+            return
+        self._stc = np.ascontiguousarray(np.asarray(coeffs, dtype=np.float32).reshape(3, 3, 5))
+        _check(lib().orc_set_stencil(self._h, self._stc.ctypes.data))
 
     def tile_index(self, i, j, k):
         return (i * self.n_tiles[1] + j) * self.n_tiles[2] + k
@@ -232,3 +247,25 @@ def set_threads(n):
 
 def max_threads():
     return lib().orc_max_threads()
+
+
+STENCIL_KEYS = {(1, 0): "delta", (2, 0): "gamma", (0, 1): "beta_p1", (0, 2): "beta_p2", (1, 1): "beta2_p1",
+                (1, 2): "beta2_p2", (2, 1): "beta3_p1", (2, 2): "beta3_p2", (0, 3): "zeta_p1", (0, 4): "zeta_p2",
+                (1, 3): "zeta2_p1", (1, 4): "zeta2_p2", (2, 3): "zeta3_p1", (2, 4): "zeta3_p2"}
+
+
+def stencil_matrices(per_axis):
+    """[3][3][5] float32 stencil matrices from per-axis {name: value} dicts,
+    M[0][0] from the normalisation of emf/stencil_coefficients.h:42-60 (float)."""
+    f = np.float32
+    out = np.zeros((3, 3, 5), dtype=f)
+    for a in range(3):
+        M = out[a]
+        for (r, c), name in STENCIL_KEYS.items():
+            M[r, c] = f(per_axis[a].get(name, 0.0))
+        s = f(0)
+        s = f(s + f(3) * f(M[1, 0] + f(2) * f(f(f(M[1, 1] + M[1, 2]) + M[1, 3]) + M[1, 4])))
+        s = f(s + f(5) * f(M[2, 0] + f(2) * f(f(f(M[2, 1] + M[2, 2]) + M[2, 3]) + M[2, 4])))
+        s = f(s + f(2) * f(f(f(M[0, 1] + M[0, 2]) + M[0, 3]) + M[0, 4]))
+        M[0, 0] = f(f(1) - s)
+    return out

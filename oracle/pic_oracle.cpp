@@ -129,6 +129,11 @@ struct Tile {
   // reflector-wall correction current, src/runko/pic/reflector_wall.c++:243-273
   std::vector<T> corrJ[3];
   bool corr_pending = false;
+  // particle pusher (0 boris, 1 higuera_cary, 2 faraday; pic/tile.c++:74-86)
+  // and B propagator (stencil coefficients when stencil_on; emf/tile.c++:29-141)
+  int pusher = 0;
+  bool stencil_on = false;
+  float stc[3][3][5] {};
 };
 
 template<class T>
@@ -191,6 +196,75 @@ void push_half_b(Tile<T>& t, double cfl)
         const T DiEy = Ey[c + si] - Ey[c];
         Bz[c] = Bz[c] + dt * (DjEx - DiEy);
       }
+}
+
+// push_b_stencil, src/runko/emf/yee_lattice_stencil.c++:17-297 (dt = cfl/2):
+// D*_a F = sum over rows n = 0..2 and columns c = 0..4 (in that order) of
+// M_a[n][c] times the difference of F between axial offsets +n+1 and -n; column
+// 0 takes the axial line, columns 1 / 3 the pair at perp1 offsets +-1 / +-2,
+// columns 2 / 4 at perp2 offsets +-1 / +-2; perp1, perp2 cyclic:
+// x -> (y, z), y -> (z, x), z -> (x, y).
+template<class T>
+T stencil_derivative(const Lattice<T>& l, const std::vector<T>& F, int i, int j, int k, int a,
+                     const float M[3][5])
+{
+  const int p1 = (a + 1) % 3, p2 = (a + 2) % 3;
+  auto at = [&](int ax, int o1, int o2) {
+    int d[3] = {0, 0, 0};
+    d[a] += ax;
+    d[p1] += o1;
+    d[p2] += o2;
+    return F[l.at(i + d[0], j + d[1], k + d[2])];
+  };
+  T sum = T(0);
+  bool first = true;
+  for(int n = 0; n < 3; ++n) {
+    const int hi = n + 1, lo = -n;
+    for(int c = 0; c < 5; ++c) {
+      T diff;
+      if(c == 0) {
+        diff = at(hi, 0, 0) - at(lo, 0, 0);
+      } else {
+        const int o = c <= 2 ? 1 : 2;
+        const bool on1 = c == 1 || c == 3;
+        const T up = on1 ? at(hi, o, 0) + at(hi, -o, 0) : at(hi, 0, o) + at(hi, 0, -o);
+        const T dn = on1 ? at(lo, o, 0) + at(lo, -o, 0) : at(lo, 0, o) + at(lo, 0, -o);
+        diff = up - dn;
+      }
+      const T term = static_cast<T>(M[n][c]) * diff;
+      sum = first ? term : sum + term;
+      first = false;
+    }
+  }
+  return sum;
+}
+
+template<class T>
+void push_half_b_stencil(Tile<T>& t, double cfl)
+{
+  auto& l = t.lat;
+  const T dt = static_cast<T>(cfl / 2);
+  for(int i = H; i < H + l.n[0]; ++i)
+    for(int j = H; j < H + l.n[1]; ++j)
+      for(int k = H; k < H + l.n[2]; ++k) {
+        const std::size_t c = l.at(i, j, k);
+        const T DzEy = stencil_derivative(l, l.E[1], i, j, k, 2, t.stc[2]);
+        const T DyEz = stencil_derivative(l, l.E[2], i, j, k, 1, t.stc[1]);
+        const T DxEz = stencil_derivative(l, l.E[2], i, j, k, 0, t.stc[0]);
+        const T DzEx = stencil_derivative(l, l.E[0], i, j, k, 2, t.stc[2]);
+        const T DyEx = stencil_derivative(l, l.E[0], i, j, k, 1, t.stc[1]);
+        const T DxEy = stencil_derivative(l, l.E[1], i, j, k, 0, t.stc[0]);
+        l.B[0][c] = l.B[0][c] + dt * (DzEy - DyEz);
+        l.B[1][c] = l.B[1][c] + dt * (DxEz - DzEx);
+        l.B[2][c] = l.B[2][c] + dt * (DyEx - DxEy);
+      }
+}
+
+template<class T>
+void push_half_b_any(Tile<T>& t, double cfl)
+{
+  if(t.stencil_on) push_half_b_stencil(t, cfl);
+  else push_half_b(t, cfl);
 }
 
 // push_e_fdtd2, src/runko/emf/yee_lattice_fdtd2.c++:68-111 (dt = cfl)
@@ -407,6 +481,82 @@ void boris_one(const T cfl, const T qm, const T EBE[3], const T EBB[3], T pos[3]
   for(int c = 0; c < 3; ++c) pos[c] = pos[c] + (vel[c] * ginv2) * cfl;
 }
 
+// push_particles_higuera_cary, src/runko/pic/particle_higuera_cary.h:15-77
+// (scalar * vector is v(i) * s, toolbox::dot accumulates from 0 in index order)
+template<class T>
+void higuera_cary_one(const T cfl, const T qm, const T EBE[3], const T EBB[3], T pos[3], T vel[3])
+{
+  const T hqm = T(0.5) * qm;
+  const T cfl2 = cfl * cfl;
+  const T cinv = T(1) / cfl;
+  const T cinv2 = cinv * cinv;
+  T v0[3], E0[3], u0[3], Bt[3];
+  for(int c = 0; c < 3; ++c) v0[c] = vel[c] * cfl;
+  for(int c = 0; c < 3; ++c) E0[c] = EBE[c] * hqm;
+  for(int c = 0; c < 3; ++c) u0[c] = v0[c] + E0[c];
+  for(int c = 0; c < 3; ++c) Bt[c] = EBB[c] * hqm;
+  const T u0sq = dot3(u0, u0);
+  const T b2 = dot3(Bt, Bt);
+  const T bdotu = dot3(Bt, u0);
+  const T gmb = T(1) + u0sq * cinv2 - b2 * cinv2;
+  const T disc = gmb * gmb + T(4) * (b2 * cinv2 + bdotu * bdotu * cinv2);
+  const T ginv = T(1) / std::sqrt(T(0.5) * (gmb + std::sqrt(disc)));
+  const T gc = ginv * cinv;
+  T B0[3];
+  for(int c = 0; c < 3; ++c) B0[c] = Bt[c] * gc;
+  const T f = T(2) / (T(1) + gc * gc * b2);
+  T cr[3], u1[3], u2[3];
+  cross3(u0, B0, cr);
+  for(int c = 0; c < 3; ++c) u1[c] = (u0[c] + cr[c]) * f;
+  cross3(u1, B0, cr);
+  for(int c = 0; c < 3; ++c) u2[c] = (u0[c] + cr[c]) + E0[c];
+  const T ginv2 = cfl / std::sqrt(cfl2 + dot3(u2, u2));
+  for(int c = 0; c < 3; ++c) {
+    vel[c] = u2[c] * cinv;
+    pos[c] = pos[c] + u2[c] * ginv2;
+  }
+}
+
+// push_particles_faraday, src/runko/pic/particle_faraday.h:37-109
+template<class T>
+void faraday_one(const T cfl, const T qm, const T EBE[3], const T EBB[3], T pos[3], T vel[3])
+{
+  T v0[3];
+  for(int c = 0; c < 3; ++c) v0[c] = vel[c] * cfl;
+  const T gcfl = std::sqrt(cfl * cfl + dot3(v0, v0));
+  const T hq = T(0.5) * qm;
+  T u0[3];
+  for(int c = 0; c < 3; ++c) u0[c] = v0[c] + EBE[c] * hq;
+  const T geff_cfl = std::sqrt(cfl * cfl + dot3(u0, u0));
+  const T kappa = hq / geff_cfl;
+  T eps[3], beta[3];
+  for(int c = 0; c < 3; ++c) {
+    eps[c] = EBE[c] * kappa;
+    beta[c] = EBB[c] * kappa;
+  }
+  const T w0 = gcfl + dot3(eps, v0);
+  T cr[3], W[3];
+  cross3(v0, beta, cr);
+  for(int c = 0; c < 3; ++c) W[c] = ((v0[c] + eps[c] * gcfl) + cr[c]) + eps[c] * w0;
+  const T b2 = dot3(beta, beta);
+  const T f = T(1) / (T(1) + b2);
+  cross3(beta, W, cr);
+  const T bW = dot3(beta, W);
+  T Wr[3];
+  for(int c = 0; c < 3; ++c) Wr[c] = ((W[c] - cr[c]) + beta[c] * bW) * f;
+  const T bde = dot3(beta, eps);
+  cross3(beta, eps, cr);
+  T er[3];
+  for(int c = 0; c < 3; ++c) er[c] = ((eps[c] - cr[c]) + beta[c] * bde) * f;
+  const T D = T(1) - f * (dot3(eps, eps) + bde * bde);
+  const T sc = dot3(eps, Wr) / D;
+  T u2[3];
+  for(int c = 0; c < 3; ++c) u2[c] = Wr[c] + er[c] * sc;
+  for(int c = 0; c < 3; ++c) vel[c] = u2[c] / cfl;
+  const T ginv2 = cfl / std::sqrt(cfl * cfl + dot3(u2, u2));
+  for(int c = 0; c < 3; ++c) pos[c] = pos[c] + vel[c] * ginv2 * cfl;
+}
+
 template<class T>
 inline void tile_origo(const Tile<T>& t, T o[3])
 {
@@ -429,7 +579,9 @@ void push_particles(Tile<T>& t, double cfl_d)
       T vel[3] = {c.u[0][n], c.u[1][n], c.u[2][n]};
       T E[3], B[3];
       interpolate(t.lat, origo, pos, E, B);
-      boris_one(cfl, qm, E, B, pos, vel);
+      if(t.pusher == 1) higuera_cary_one(cfl, qm, E, B, pos, vel);
+      else if(t.pusher == 2) faraday_one(cfl, qm, E, B, pos, vel);
+      else boris_one(cfl, qm, E, B, pos, vel);
       for(int d = 0; d < 3; ++d) {
         c.x[d][n] = pos[d];
         c.u[d][n] = vel[d];
@@ -965,7 +1117,7 @@ void lap(World<T>& w, long lapno)
 #pragma omp parallel for schedule(dynamic)
     for(int t = 0; t < NT; ++t) f(w.tiles[t]);
   };
-  each([&](Tile<T>& t) { push_half_b(t, w.cfl); });
+  each([&](Tile<T>& t) { push_half_b_any(t, w.cfl); });
   comm_fields(w, 2);
   each([&](Tile<T>& t) { push_particles(t, w.cfl); });
   each([&](Tile<T>& t) { pack_outgoing(t); });
@@ -978,7 +1130,7 @@ void lap(World<T>& w, long lapno)
   comm_fields(w, 0);
   each([&](Tile<T>& t) { filter(w, t); });
   each([&](Tile<T>& t) { filter(w, t); });
-  each([&](Tile<T>& t) { push_half_b(t, w.cfl); });
+  each([&](Tile<T>& t) { push_half_b_any(t, w.cfl); });
   comm_fields(w, 2);
   each([&](Tile<T>& t) { push_e(t, w.cfl); });
   each([&](Tile<T>& t) { add_current(t); });
@@ -1073,6 +1225,29 @@ int orc_max_threads()
 #else
   return 1;
 #endif
+}
+
+// particle pusher of every tile: 0 boris, 1 higuera_cary, 2 faraday
+int orc_set_pusher(void* hp, int pusher)
+{
+  auto* h = static_cast<Handle*>(hp);
+  return guard([&] { dispatch(h, [&](auto& w) {
+    if(pusher < 0 || pusher > 2) throw std::runtime_error("unknown particle pusher");
+    for(auto& t : w.tiles) t.pusher = pusher;
+  }); });
+}
+
+// B propagator of every tile: coeffs = 45 floats [axis][row][column] (the
+// stencil's M matrices, M[0][0] already normalised), or null for fdtd2
+int orc_set_stencil(void* hp, const float* coeffs)
+{
+  auto* h = static_cast<Handle*>(hp);
+  return guard([&] { dispatch(h, [&](auto& w) {
+    for(auto& t : w.tiles) {
+      t.stencil_on = coeffs != nullptr;
+      if(coeffs) std::memcpy(t.stc, coeffs, sizeof(t.stc));
+    }
+  }); });
 }
 
 void orc_set_threads(int n)
@@ -1274,7 +1449,7 @@ int orc_get_outgoing(void* hp, int tile, double* out, std::uint64_t* ids, int* d
     });                                                                           \
   }
 
-TILE_METHOD(push_half_b, push_half_b(t, w.cfl))
+TILE_METHOD(push_half_b, push_half_b_any(t, w.cfl))
 TILE_METHOD(push_e, push_e(t, w.cfl))
 TILE_METHOD(add_current, add_current(t))
 TILE_METHOD(filter_current, filter(w, t))

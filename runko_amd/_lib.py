@@ -131,6 +131,7 @@ def lib():
     L.rk_snapshot_fields.argtypes = [vp, i32, i32, vp, vp]
     L.rk_snapshot_spectra.argtypes = [vp, i32, i32, ctypes.c_float, ctypes.c_float, i32, vp, vp]
     L.rk_interpolate_at.argtypes = [vp, i32, i64, P3, P3, P3, vp]
+    L.rk_world_set_schemes.argtypes = [vp, i32, vp]
     _lib = L
     return _lib
 
@@ -157,6 +158,7 @@ EXPORTED = [
     "rk_profile_read", "rk_world_set_mailbox", "rk_world_set_current_staging", "rk_reflect_particles",
     "rk_apply_edge_bc", "rk_diff_create", "rk_diff_destroy", "rk_diff_step", "rk_diff_step_adjoint",
     "rk_diff_error_count", "rk_snapshot_fields", "rk_snapshot_spectra", "rk_interpolate_at",
+    "rk_world_set_schemes",
 ]
 
 PROF_CATEGORIES = ["fused_particle_step", "push", "deposit", "pack", "append", "fdtd", "filter", "ghost", "other"]

@@ -348,7 +348,19 @@ int rk_push_half_b(rk_world* w, int32_t tile, void* stream)
   int r0[3], r1[3];
   interior(w, tile, r0, r1);
   ProfScope ps(w, 5, S(stream));
-  return launch_curl_b(w, r0, r1, static_cast<float>(w->d.cfl / 2), S(stream));
+  const float dt = static_cast<float>(w->d.cfl / 2);
+  if(w->stencil_on) return launch_curl_b_stencil(w, r0, r1, dt, S(stream));
+  return launch_curl_b(w, r0, r1, dt, S(stream));
+}
+
+int rk_world_set_schemes(rk_world* w, int32_t particle_pusher, const float* stencil)
+{
+  if(!w) return fail(RK_EINVAL, "null world");
+  if(particle_pusher < 0 || particle_pusher > 2) return fail(RK_EINVAL, "unknown particle pusher");
+  w->g.pusher = particle_pusher;
+  w->stencil_on = stencil != nullptr;
+  if(stencil) std::memcpy(w->stencil, stencil, sizeof(w->stencil));
+  return RK_OK;
 }
 
 int rk_push_e(rk_world* w, int32_t tile, void* stream)
@@ -484,6 +496,7 @@ int rk_particle_step_fused(rk_world* w, void* stream)
   if(!w->mbox.meta || !w->mig[0].meta || !w->mig[1].meta || !w->jstage)
     return fail(RK_EINVAL, "fused particle step: mailbox, migration or current staging buffers not set");
   if(w->filter_open) return fail(RK_ESTATE, "particle step during an open filter pass");
+  if(w->g.pusher != 0) return fail(RK_ESTATE, "the fused particle step implements the boris pusher only");
   hipStream_t s = S(stream);
   {
     ProfScope ps(w, 8, s);

@@ -5,6 +5,7 @@
 // compiled with -ffp-contract=off, so a*b+c stays two roundings):
 //   push_b_fdtd2  src/runko/emf/yee_lattice_fdtd2.c++:19-58
 //   push_e_fdtd2  src/runko/emf/yee_lattice_fdtd2.c++:68-111
+//   push_b_stencil src/runko/emf/yee_lattice_stencil.c++:17-297
 //   add_current   src/runko/emf/yee_lattice.c++:170-179
 //   binomial2     src/runko/emf/yee_lattice_current_filter_binomial2.c++:23-76
 //   binomial2_unrolled                                       :88-157
@@ -15,6 +16,7 @@
 #include "rk_internal.h"
 
 #include <cmath>
+#include <cstring>
 
 namespace rk {
 
@@ -55,6 +57,66 @@ __global__ void __launch_bounds__(BX* BY)
   const float DjEx = Ex[c + sj] - ex;
   const float DiEy = Ey[c + si] - ey;
   Bz[c] = Bz[c] + dt * (DjEx - DiEy);
+}
+
+// push_b_stencil, src/runko/emf/yee_lattice_stencil.c++:17-297: each
+// derivative D*_a F sums, over rows n = 0..2 and then columns c = 0..4, M_a[n][c]
+// times the difference of F between axial offsets +n+1 and -n: column 0 on
+// the axial line, columns 1 / 3 the pairs at perp1 offsets +-1 / +-2, columns
+// 2 / 4 at perp2 offsets +-1 / +-2 (perp1, perp2 cyclic: x -> (y, z),
+// y -> (z, x), z -> (x, y)); each pair is (+o) + (-o).
+struct StencilM {
+  float M[3][3][5];
+};
+
+__device__ inline float stencil_d(const float* __restrict__ F, int64_t c, const int64_t st[3], int a,
+                                  const float (&M)[3][5])
+{
+  const int64_t sa = st[a], s1 = st[(a + 1) % 3], s2 = st[(a + 2) % 3];
+  auto at = [&](int ax, int o1, int o2) { return F[c + ax * sa + o1 * s1 + o2 * s2]; };
+  float sum = 0.0f;
+#pragma unroll
+  for(int n = 0; n < 3; ++n) {
+    const int hi = n + 1, lo = -n;
+#pragma unroll
+    for(int col = 0; col < 5; ++col) {
+      float diff;
+      if(col == 0) {
+        diff = at(hi, 0, 0) - at(lo, 0, 0);
+      } else {
+        const int o = col <= 2 ? 1 : 2;
+        if(col == 1 || col == 3)
+          diff = (at(hi, o, 0) + at(hi, -o, 0)) - (at(lo, o, 0) + at(lo, -o, 0));
+        else
+          diff = (at(hi, 0, o) + at(hi, 0, -o)) - (at(lo, 0, o) + at(lo, 0, -o));
+      }
+      const float term = M[n][col] * diff;
+      sum = (n == 0 && col == 0) ? term : sum + term;
+    }
+  }
+  return sum;
+}
+
+__global__ void __launch_bounds__(BX* BY)
+  k_curl_b_stencil(Geo g, Region r, float dt, StencilM S, const float* __restrict__ Ex,
+                   const float* __restrict__ Ey, const float* __restrict__ Ez, float* __restrict__ Bx,
+                   float* __restrict__ By, float* __restrict__ Bz)
+{
+  const int k = r.r0[2] + int(blockIdx.x) * BX + int(threadIdx.x);
+  const int j = r.r0[1] + int(blockIdx.y) * BY + int(threadIdx.y);
+  const int i = r.r0[0] + int(blockIdx.z);
+  if(k >= r.r1[2] || j >= r.r1[1]) return;
+  const int64_t c = g.at(i, j, k);
+  const int64_t st[3] = {int64_t(g.L[1]) * g.L[2], int64_t(g.L[2]), 1};
+  const float DzEy = stencil_d(Ey, c, st, 2, S.M[2]);
+  const float DyEz = stencil_d(Ez, c, st, 1, S.M[1]);
+  Bx[c] = Bx[c] + dt * (DzEy - DyEz);
+  const float DxEz = stencil_d(Ez, c, st, 0, S.M[0]);
+  const float DzEx = stencil_d(Ex, c, st, 2, S.M[2]);
+  By[c] = By[c] + dt * (DxEz - DzEx);
+  const float DyEx = stencil_d(Ex, c, st, 1, S.M[1]);
+  const float DxEy = stencil_d(Ey, c, st, 0, S.M[0]);
+  Bz[c] = Bz[c] + dt * (DyEx - DxEy);
 }
 
 template<bool ADDJ>
@@ -332,6 +394,18 @@ int launch_curl_b(rk_world* w, const int r0[3], const int r1[3], float dt, hipSt
   if(empty(r)) return RK_OK;
   hipLaunchKernelGGL(k_curl_b, region_grid(r), dim3(BX, BY), 0, s, w->g, r, dt, w->E[0], w->E[1], w->E[2],
                      w->B[0], w->B[1], w->B[2]);
+  RK_HIP(hipGetLastError());
+  return RK_OK;
+}
+
+int launch_curl_b_stencil(rk_world* w, const int r0[3], const int r1[3], float dt, hipStream_t s)
+{
+  const Region r = make_region(r0, r1);
+  if(empty(r)) return RK_OK;
+  StencilM S;
+  std::memcpy(S.M, w->stencil, sizeof(S.M));
+  hipLaunchKernelGGL(k_curl_b_stencil, region_grid(r), dim3(BX, BY), 0, s, w->g, r, dt, S, w->E[0], w->E[1],
+                     w->E[2], w->B[0], w->B[1], w->B[2]);
   RK_HIP(hipGetLastError());
   return RK_OK;
 }

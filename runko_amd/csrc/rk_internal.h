@@ -65,6 +65,7 @@ struct Geo {
   double jscale;      // fixed-point scale of the fused current windows: 2^40 / (max|q| rounded up to 2^k)
   double jinv;        // 1 / jscale (exact)
   uint32_t br_magic[3];  // ceil(2^32 / br) for br > 1: c / br == umulhi(c, magic) for 0 <= c < 2^16
+  int pusher;            // 0 boris, 1 higuera_cary, 2 faraday (pic/tile.c++:74-86)
 
   __host__ __device__ int div_br(int c, int d) const
   {
@@ -157,6 +158,10 @@ struct rk_world {
   // reflector-wall correction current (world lattice, 3 components), added to
   // J and cleared by the next deposit of each pending tile
   float* corrJ[3] = {nullptr, nullptr, nullptr};
+  // B propagator: extended stencil (emf/yee_lattice_stencil.c++) when set,
+  // [axis][row][column] coefficient matrices with M[0][0] normalised
+  bool stencil_on = false;
+  float stencil[3][3][5] = {};
   std::vector<char> corr_pending;
   int corr_any = 0;
   // library scratch
@@ -219,6 +224,7 @@ int hip_fail(hipError_t e, const char* where);
 
 // launchers implemented in rk_fields.hip / rk_particles.hip
 int launch_curl_b(rk_world* w, const int r0[3], const int r1[3], float dt, hipStream_t s);
+int launch_curl_b_stencil(rk_world* w, const int r0[3], const int r1[3], float dt, hipStream_t s);
 int launch_curl_e(rk_world* w, const int r0[3], const int r1[3], float dt, bool add_j, hipStream_t s);
 int launch_add_current(rk_world* w, const int r0[3], const int r1[3], hipStream_t s);
 int launch_filter(rk_world* w, const int r0[3], const int r1[3], hipStream_t s);

@@ -6,6 +6,8 @@
 // Reference semantics followed (hel-astro-lab/runko v5.0.0):
 //   interpolation  src/runko/emf/yee_lattice_interpolate_linear_1st.h:29-138
 //   Boris          src/runko/pic/particle_boris.h:15-62
+//   Higuera-Cary   src/runko/pic/particle_higuera_cary.h:15-77 (unfused push only)
+//   Faraday        src/runko/pic/particle_faraday.h:37-109 (unfused push only)
 //   zigzag         src/runko/pic/particle_current_zigzag_1st.c++:225-339
 //   divide         src/runko/pic/particle.c++:198-348
 //   append/wrap    src/runko/pic/particle.h:449-572
@@ -740,9 +742,119 @@ __device__ inline int bin_total(const SpParams& sp, int b)
   return t;
 }
 
+__device__ inline float dot3p(const float a[3], const float b[3])
+{
+  float r = a[0] * b[0];
+  r = r + a[1] * b[1];
+  r = r + a[2] * b[2];
+  return r;
+}
+
+__device__ inline void cross3p(const float a[3], const float b[3], float r[3])
+{
+  r[0] = a[1] * b[2] - a[2] * b[1];
+  r[1] = -a[0] * b[2] + a[2] * b[0];
+  r[2] = a[0] * b[1] - a[1] * b[0];
+}
+
+// push_particles_higuera_cary, src/runko/pic/particle_higuera_cary.h:34-73,
+// expression order as written there (scalar * vector is v(i) * s)
+__device__ inline void higuera_cary(const Geo& g, float qm, const float E[3], const float B[3], float pos[3],
+                                    float vel[3])
+{
+  const float cfl = g.cfl;
+  const float hqm = 0.5f * qm;
+  const float cfl2 = cfl * cfl;
+  const float cinv = 1.0f / cfl;
+  const float cinv2 = cinv * cinv;
+  float E0[3], u0[3], Bt[3];
+#pragma unroll
+  for(int c = 0; c < 3; ++c) {
+    E0[c] = E[c] * hqm;
+    u0[c] = vel[c] * cfl + E0[c];
+    Bt[c] = B[c] * hqm;
+  }
+  const float u0sq = dot3p(u0, u0);
+  const float b2 = dot3p(Bt, Bt);
+  const float bdotu = dot3p(Bt, u0);
+  const float gmb = 1.0f + u0sq * cinv2 - b2 * cinv2;
+  const float disc = gmb * gmb + 4.0f * (b2 * cinv2 + bdotu * bdotu * cinv2);
+  const float ginv = 1.0f / sqrtf(0.5f * (gmb + sqrtf(disc)));
+  const float gc = ginv * cinv;
+  float B0[3];
+#pragma unroll
+  for(int c = 0; c < 3; ++c) B0[c] = Bt[c] * gc;
+  const float f = 2.0f / (1.0f + gc * gc * b2);
+  float cr[3], u1[3], u2[3];
+  cross3p(u0, B0, cr);
+#pragma unroll
+  for(int c = 0; c < 3; ++c) u1[c] = (u0[c] + cr[c]) * f;
+  cross3p(u1, B0, cr);
+#pragma unroll
+  for(int c = 0; c < 3; ++c) u2[c] = (u0[c] + cr[c]) + E0[c];
+  const float ginv2 = cfl / sqrtf(cfl2 + dot3p(u2, u2));
+#pragma unroll
+  for(int c = 0; c < 3; ++c) {
+    vel[c] = u2[c] * cinv;
+    pos[c] = pos[c] + u2[c] * ginv2;
+  }
+}
+
+// push_particles_faraday, src/runko/pic/particle_faraday.h:54-104 (Cayley
+// transform of the field bivector, Sherman-Morrison reduction)
+__device__ inline void faraday(const Geo& g, float qm, const float E[3], const float B[3], float pos[3],
+                               float vel[3])
+{
+  const float cfl = g.cfl;
+  float v0[3];
+#pragma unroll
+  for(int c = 0; c < 3; ++c) v0[c] = vel[c] * cfl;
+  const float gcfl = sqrtf(cfl * cfl + dot3p(v0, v0));
+  const float hq = 0.5f * qm;
+  float u0[3];
+#pragma unroll
+  for(int c = 0; c < 3; ++c) u0[c] = v0[c] + E[c] * hq;
+  const float geff_cfl = sqrtf(cfl * cfl + dot3p(u0, u0));
+  const float kappa = hq / geff_cfl;
+  float eps[3], beta[3];
+#pragma unroll
+  for(int c = 0; c < 3; ++c) {
+    eps[c] = E[c] * kappa;
+    beta[c] = B[c] * kappa;
+  }
+  const float w0 = gcfl + dot3p(eps, v0);
+  float cr[3], W[3];
+  cross3p(v0, beta, cr);
+#pragma unroll
+  for(int c = 0; c < 3; ++c) W[c] = ((v0[c] + eps[c] * gcfl) + cr[c]) + eps[c] * w0;
+  const float b2 = dot3p(beta, beta);
+  const float f = 1.0f / (1.0f + b2);
+  cross3p(beta, W, cr);
+  const float bW = dot3p(beta, W);
+  float Wr[3];
+#pragma unroll
+  for(int c = 0; c < 3; ++c) Wr[c] = ((W[c] - cr[c]) + beta[c] * bW) * f;
+  const float bde = dot3p(beta, eps);
+  cross3p(beta, eps, cr);
+  float er[3];
+#pragma unroll
+  for(int c = 0; c < 3; ++c) er[c] = ((eps[c] - cr[c]) + beta[c] * bde) * f;
+  const float D = 1.0f - f * (dot3p(eps, eps) + bde * bde);
+  const float sc = dot3p(eps, Wr) / D;
+  float u2[3];
+#pragma unroll
+  for(int c = 0; c < 3; ++c) u2[c] = Wr[c] + er[c] * sc;
+#pragma unroll
+  for(int c = 0; c < 3; ++c) vel[c] = u2[c] / cfl;
+  const float ginv2 = cfl / sqrtf(cfl * cfl + dot3p(u2, u2));
+#pragma unroll
+  for(int c = 0; c < 3; ++c) pos[c] = pos[c] + vel[c] * ginv2 * cfl;
+}
+
 // ---------------------------------------------------------------------------
 // kernels
 // ---------------------------------------------------------------------------
+template<int PUSHER>
 __global__ void __launch_bounds__(WG)
   k_push(Geo g, SpParams sp, FieldPtrs f, int bin0, int nbins, uint32_t* counters)
 {
@@ -778,7 +890,12 @@ __global__ void __launch_bounds__(WG)
       float E[3], Bf[3];
       gather(g, f, W, lds, Bk.toffh[0] + ci[0], Bk.toffh[1] + ci[1], Bk.toffh[2] + ci[2], dd, E, Bf,
              &counters[C_STENCIL]);
-      boris(g, qm, E, Bf, pos, vel);
+      if constexpr(PUSHER == 1)
+        higuera_cary(g, qm, E, Bf, pos, vel);
+      else if constexpr(PUSHER == 2)
+        faraday(g, qm, E, Bf, pos, vel);
+      else
+        boris(g, qm, E, Bf, pos, vel);
 #pragma unroll
       for(int d = 0; d < 3; ++d) {
         sp.pos[s][d][q] = pos[d];
@@ -1709,7 +1826,8 @@ int launch_push(rk_world* w, int tile, hipStream_t s)
 {
   int b0, nb;
   bin_range(w, tile, b0, nb);
-  hipLaunchKernelGGL(k_push, dim3(xcd_grid(nb)), dim3(WG), means_lds(w->g), s, w->g, make_sp(w), make_fields(w), b0,
+  auto* kern = w->g.pusher == 1 ? k_push<1> : (w->g.pusher == 2 ? k_push<2> : k_push<0>);
+  hipLaunchKernelGGL(kern, dim3(xcd_grid(nb)), dim3(WG), means_lds(w->g), s, w->g, make_sp(w), make_fields(w), b0,
                      nb, w->counters);
   RK_HIP(hipGetLastError());
   return RK_OK;

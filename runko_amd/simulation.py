@@ -177,7 +177,7 @@ class Simulation:
                     w.settle_overflow()
             return
         self._flush()
-        if op == "push":
+        if op == "push" and self._world.pusher == "boris":  # the fused step implements boris
             self._pending = ["push"]
         else:
             self._run_unfused(op)

@@ -65,6 +65,7 @@ class TileGrid:
         world = World(t0.n, self.n_tiles, (0, 0, 0), self.n_tiles, periodic=True, cfl=t0.cfl,
                       charges=t0._charges, masses=t0._masses, current_filter=t0.current_filter or "binomial2",
                       brick=brick)
+        world.set_schemes(getattr(t0, "particle_pusher", "boris"), t0._stencil)
         torch.cuda.synchronize()
         # fields
         for t, tile in enumerate(tiles):

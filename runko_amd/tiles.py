@@ -21,7 +21,37 @@ from . import _lib
 from . import _lib as L
 from .world import HALO, World, _stream
 
-SUPPORTED_PUSHERS = ("boris",)
+SUPPORTED_PUSHERS = ("boris", "higuera_cary", "faraday")
+SUPPORTED_PROPAGATORS = ("fdtd2", "stencil")
+
+# StencilAxisCoeffs entries read from the configuration (emf/tile.c++:118-135):
+# (row, column) -> key suffix; M[0][0] follows from the normalisation
+_STENCIL_KEYS = (((1, 0), "delta"), ((2, 0), "gamma"), ((0, 1), "beta_p1"), ((0, 2), "beta_p2"),
+                 ((1, 1), "beta2_p1"), ((1, 2), "beta2_p2"), ((2, 1), "beta3_p1"), ((2, 2), "beta3_p2"),
+                 ((0, 3), "zeta_p1"), ((0, 4), "zeta_p2"), ((1, 3), "zeta2_p1"), ((1, 4), "zeta2_p2"),
+                 ((2, 3), "zeta3_p1"), ((2, 4), "zeta3_p2"))
+
+
+def stencil_from_config(conf):
+    """[3][3][5] float32 matrices of the stencil B push: each coefficient from
+    stencil_<axis>_<name>, else stencil_<name>, else 0 (emf/tile.c++:103-141);
+    M[0][0] = StencilAxisCoeffs::alpha() (stencil_coefficients.h:42-60), in
+    float arithmetic."""
+    f = np.float32
+    out = np.zeros((3, 3, 5), dtype=f)
+    for a, ax in enumerate("xyz"):
+        M = out[a]
+        for (r, c), name in _STENCIL_KEYS:
+            v = conf.get(f"stencil_{ax}_{name}")
+            if v is None:
+                v = conf.get(f"stencil_{name}")
+            M[r, c] = f(float(v)) if v is not None else f(0)
+        s = f(0)
+        s = f(s + f(3) * f(M[1, 0] + f(2) * f(f(f(M[1, 1] + M[1, 2]) + M[1, 3]) + M[1, 4])))
+        s = f(s + f(5) * f(M[2, 0] + f(2) * f(f(f(M[2, 1] + M[2, 2]) + M[2, 3]) + M[2, 4])))
+        s = f(s + f(1) * f(2) * f(f(f(M[0, 1] + M[0, 2]) + M[0, 3]) + M[0, 4]))
+        M[0, 0] = f(f(1) - s)
+    return out
 SUPPORTED_INTERPOLATORS = ("linear_1st", "linear_1st_unrolled")
 SUPPORTED_DEPOSITERS = ("zigzag", "zigzag_1st", "zigzag_1st_atomic")
 SUPPORTED_FILTERS = ("binomial2", "binomial2_unrolled")
@@ -137,8 +167,10 @@ class EmfTile:
             raise RuntimeError(f"Yee Lattice extents ({n[0]}, {n[1]}, {n[2]}) are assumed to be at least halo size: 3")
         self.cfl = float(conf.require("cfl"))
         fp = conf.require("field_propagator")
-        if fp != "fdtd2":
+        if fp not in SUPPORTED_PROPAGATORS:
             raise RuntimeError(f"{fp} is not supported field propagator.")
+        self.field_propagator = fp
+        self._stencil = stencil_from_config(conf) if fp == "stencil" else None
         cf = conf.get("current_filter")
         if cf is not None and cf not in SUPPORTED_FILTERS:
             raise RuntimeError(f"{cf} is not supported current filter.")
@@ -156,6 +188,7 @@ class EmfTile:
         self._world = World(self.n, (1, 1, 1), idx, tiles, periodic=False, cfl=self.cfl,
                             charges=self._charges, masses=self._masses,
                             current_filter=cf or "binomial2")
+        self._world.set_schemes(getattr(self, "particle_pusher", "boris"), self._stencil)
         self._t = 0
         self._edge_bcs = []
         self._walls = []

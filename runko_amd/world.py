@@ -116,8 +116,21 @@ class World:
         self.mbox = None
         self._mbox_per_brick = 0
         self.jstage = None
+        self.pusher = "boris"
+        self.stencil = None
 
     # ------------------------------------------------------------------ plumbing
+    PUSHERS = {"boris": 0, "higuera_cary": 1, "faraday": 2}
+
+    def set_schemes(self, pusher="boris", stencil=None):
+        """Particle pusher of every tile and the B propagator (stencil
+        matrices [3][3][5] float32, or None for fdtd2)."""
+        if pusher not in self.PUSHERS:
+            raise RuntimeError(f"{pusher} is not supported particle pusher.")
+        self.pusher = pusher
+        self.stencil = None if stencil is None else np.ascontiguousarray(stencil, dtype=np.float32).reshape(3, 3, 5)
+        ptr = None if self.stencil is None else self.stencil.ctypes.data
+        L.check(L.lib().rk_world_set_schemes(self.h, self.PUSHERS[pusher], ptr))
     def __del__(self):
         h = getattr(self, "h", None)
         if h:

@@ -7,13 +7,16 @@ import runko_amd as runko
 CFL = 0.45
 
 
-def make_config(nt, n, charges, masses, filt="binomial2", cfl=CFL, execution="plain"):
+def make_config(nt, n, charges, masses, filt="binomial2", cfl=CFL, execution="plain", pusher="boris",
+                stencil=None):
     c = runko.Configuration(None)
     c.n_tiles = list(nt)
     c.n_cells_per_tile = list(n)
     c.cfl = cfl
-    c.field_propagator = "fdtd2"
-    c.particle_pusher = "boris"
+    c.field_propagator = "fdtd2" if stencil is None else "stencil"
+    for key, v in (stencil or {}).items():
+        setattr(c, key, v)
+    c.particle_pusher = pusher
     c.field_interpolator = "linear_1st"
     c.current_depositer = "zigzag_1st_atomic"
     c.current_filter = filt
@@ -48,8 +51,9 @@ def random_inputs(nt, n, nsp, seed, ppc=4, field_amp=0.05, umax=1.5):
     return tiles
 
 
-def build_gpu(nt, n, charges, masses, inputs, filt="binomial2", execution="plain", comm_prelude=True):
-    conf = make_config(nt, n, charges, masses, filt=filt, execution=execution)
+def build_gpu(nt, n, charges, masses, inputs, filt="binomial2", execution="plain", comm_prelude=True, pusher="boris",
+              stencil=None):
+    conf = make_config(nt, n, charges, masses, filt=filt, execution=execution, pusher=pusher, stencil=stencil)
     grid = runko.TileGrid(conf)
     for idx in grid.local_tile_indices():
         F, parts = inputs[idx]
@@ -70,8 +74,23 @@ def build_gpu(nt, n, charges, masses, inputs, filt="binomial2", execution="plain
     return conf, grid, sim
 
 
-def build_oracle(orc, nt, n, charges, masses, inputs, prec=32, filt="binomial2", comm_prelude=True):
+def build_oracle(orc, nt, n, charges, masses, inputs, prec=32, filt="binomial2", comm_prelude=True, pusher="boris",
+                 stencil=None):
+    """stencil: the configuration keys (stencil_<name>, stencil_<axis>_<name>)."""
     w = orc.OracleWorld(nt, n, CFL, charges, masses, prec=prec, current_filter=filt)
+    w.set_pusher(pusher)
+    if stencil is not None:
+        per_axis = []
+        for ax in "xyz":
+            d = {}
+            for key, v in stencil.items():
+                name = key[len("stencil_"):]
+                if name.startswith(ax + "_"):
+                    d[name[2:]] = v
+                elif name[:2] not in ("x_", "y_", "z_"):
+                    d.setdefault(name, v)
+            per_axis.append(d)
+        w.set_stencil(orc.stencil_matrices(per_axis))
     for idx, (F, parts) in inputs.items():
         t = w.tile_index(*idx)
         w.set_fields(t, F)

@@ -147,3 +147,25 @@ def test_moving_injector_stripe_respects_wall_and_period():
     # left edge pulled back by beta_flow * n_inj * cfl = 1.8, clamped at the wall
     assert t.calls == [(0, 4.0, 7.0), (1, 4.0, 7.0)] * 2
     assert inj.injloc == 7.0
+
+
+def test_stencil_config_matches_oracle_restatement():
+    # the product's reading of stencil_* keys (emf/tile.c++:103-141) against the oracle's
+    import oracle
+    from runko_amd.tiles import _Conf, stencil_from_config
+
+    class C:
+        pass
+
+    c = C()
+    c.stencil_delta = -0.06
+    c.stencil_gamma = 0.004
+    c.stencil_beta_p1 = 0.011
+    c.stencil_x_delta = -0.05
+    c.stencil_y_beta2_p2 = 0.002
+    c.stencil_z_zeta3_p1 = 0.0004
+    got = stencil_from_config(_Conf(c))
+    per_axis = [dict(delta=-0.05, gamma=0.004, beta_p1=0.011),
+                dict(delta=-0.06, gamma=0.004, beta_p1=0.011, beta2_p2=0.002),
+                dict(delta=-0.06, gamma=0.004, beta_p1=0.011, zeta3_p1=0.0004)]
+    assert np.array_equal(got, oracle.stencil_matrices(per_axis))
