@@ -211,6 +211,64 @@ def run_cpu_baseline(inputs, seconds):
                        f"{laps} full laps ({dt:.1f} s), same step order as the GPU run")
 
 
+def run_shock(args):
+    """Reflecting-wall shock: nx x ny x nz cells in tile^3 tiles, gamma 10 cold
+    flow along -x, 8 ppc/species upstream, density ramped 4x toward the wall
+    (512 cells), theta 1e-4, sigma 0.1 with B perpendicular to the flow.  The
+    pic-shock lap (edge conditions, reflect, the rest of the PIC step) in
+    the batched form (one launch per method over every tile) and in the plain
+    per-tile form (one call per tile per method, each method synchronous),
+    on the same world, back to back."""
+    import gc
+
+    import numpy as np
+    import torch
+
+    from runko_amd.init import shock_world
+    from runko_amd.stepper import PlainShockLap, ShockLap
+
+    gc.collect()
+    torch.cuda.empty_cache()
+    nx, ny, nz = args.shock_cells
+    t0 = time.perf_counter()
+    w, walls, bcs, par = shock_world(nx=nx, ny=ny, nz=nz, tile=args.tile, ppc=8, theta=1e-4, gamma_flow=10.0,
+                                     sigma=0.1, seed=3)
+    npart = w.total_particles()
+    per_tile = np.array([w.tile_count(t, 0) + w.tile_count(t, 1) for t in range(w.ntiles)])
+    occupied = per_tile[per_tile > 0]
+    log(f"[bench] shock {nx}x{ny}x{nz}, {w.ntiles} tiles, {npart} particles, tile occupancy "
+        f"{occupied.min()}..{occupied.max()} (+{int((per_tile == 0).sum())} empty), "
+        f"setup {time.perf_counter() - t0:.1f}s")
+    res = dict(workload=f"reflecting-wall shock, {nx}x{ny}x{nz} cells in {args.tile}^3 tiles, gamma 10 flow in -x, "
+                        f"8 ppc/species upstream, 4x density ramp over 512 cells at the wall, theta 1e-4, "
+                        f"sigma 0.1 perpendicular B, conducting + upstream edge conditions, FP32",
+               particles=npart, tiles=w.ntiles, tile_particles_min=int(occupied.min()),
+               tile_particles_max=int(occupied.max()), tiles_empty=int((per_tile == 0).sum()))
+    runs = [("batched", ShockLap(w, walls, bcs), args.shock_steps)]
+    if args.plain_steps > 0:
+        runs.append(("plain_per_tile", PlainShockLap(w, walls, bcs), max(1, args.shock_plain_steps)))
+    for name, lap, steps in runs:
+        lap()
+        lap.settle()
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        for i in range(steps):
+            lap()
+            lap.settle()
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t1
+        assert w.total_particles() == npart, "particle count changed (shock)"
+        res[name] = dict(steps=steps, ms_per_step=1e3 * dt / steps, particle_steps_per_s=npart * steps / dt)
+        log(f"[bench] shock {name}: {1e3 * dt / steps:.2f} ms/step, {npart * steps / dt:.3e} p-s/s")
+    if "plain_per_tile" in res:
+        res["batched_vs_plain"] = res["plain_per_tile"]["ms_per_step"] / res["batched"]["ms_per_step"]
+    res["error_counters"] = [int(v) for v in w.error_counters()]
+    del runs, w
+    gc.collect()
+    torch.cuda.empty_cache()
+    return res
+
+
 def run_differentiation(args):
     """Differentiation workload: periodic pair plasma on a cells^3 box, ppc
     particles/cell/species, theta 0.3, initial dB from 12 seeded Fourier modes
@@ -311,6 +369,12 @@ def main():
     ap.add_argument("--adjoint-ppc", type=int, default=16)
     ap.add_argument("--adjoint-steps", type=int, default=200)
     ap.add_argument("--adjoint-every", type=int, default=20)
+    ap.add_argument("--no-shock", action="store_true", help="skip the reflecting-wall shock workload")
+    ap.add_argument("--only-shock", action="store_true",
+                    help="run only the shock workload and print its result object")
+    ap.add_argument("--shock-cells", type=int, nargs=3, default=[2048, 128, 128], metavar=("NX", "NY", "NZ"))
+    ap.add_argument("--shock-steps", type=int, default=20)
+    ap.add_argument("--shock-plain-steps", type=int, default=3)
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="also run plain copies of MIB MiB (scale for rocprof byte counters)")
     args = ap.parse_args()
@@ -323,6 +387,9 @@ def main():
         log(f"[bench] adjoint/forward {d['adjoint_over_forward']:.2f} (rollout with recompute), "
             f"step kernels {d['forward_ms_per_step']:.2f} / {d['adjoint_ms_per_step']:.2f} ms")
         print(json.dumps(d), flush=True)
+        return
+    if args.only_shock:
+        print(json.dumps(run_shock(args)), flush=True)
         return
     if args.copy_probe:
         run_copy_probe(args.copy_probe)
@@ -349,6 +416,8 @@ def main():
             graph_step_roofline_frac=step_bytes / (gph["ms_per_step"] * 1e-3) / 1e9 / HBM_PEAK_GBS)
         log(f"[bench] 64^3: graph {gph['particle_steps_per_s']:.3e}, batched {b['particle_steps_per_s']:.3e}, "
             f"plain {p['particle_steps_per_s']:.3e} p-s/s")
+    if not args.no_shock:
+        extra["shock"] = run_shock(args)
     if not args.no_adjoint:
         extra["differentiation"] = run_differentiation(args)
         d = extra["differentiation"]

@@ -12,7 +12,12 @@ pools directly on the device:
     torch's Philox generator (statistically, not bitwise, equal),
   * ids (tile_tag << 40) | ordinal, ordinals counted brick by brick,
   * a guide field B0 along z plus n x n divergence-free sinusoidal dBx, dBy
-    modes with seeded random phases, normalised to rms(dB) = dB_over_B0 * B0.
+    modes with seeded random phases, normalised to rms(dB) = dB_over_B0 * B0;
+  * for the reflecting-wall shock: a cold flow along -x (rest-frame
+    Juttner-Synge momenta boosted by the flow's Lorentz factor; the boost
+    does not re-weight the sample as the reference's flipping method does,
+    which matters only for warm flows), a density ramp in front of the wall,
+    the upstream perpendicular field and its motional E.
 """
 
 import math
@@ -168,3 +173,128 @@ def turbulence_world(N=512, tile=32, ppc=16, theta=0.3, sigma=10.0, cfl=0.45, c_
     w.comm_local(2)
     torch.cuda.synchronize()
     return w, dict(q0=q0, q1=q1, B0=B0)
+
+
+def shock_parameters(cfl, c_omp, ppc, gamma_flow, sigma, m0=1.0, m1=1.0):
+    """Charges and upstream field of projects/pic-shock/pic.py:44-69 (field
+    perpendicular to the flow, along z): (q0, q1, B0, Ey) with the motional
+    E_y = -beta B0 of a flow along -x."""
+    beta = math.sqrt(1.0 - 1.0 / gamma_flow**2)
+    oppc = 2 * ppc
+    omp = cfl / c_omp
+    q0 = -(omp**2 * gamma_flow) / (0.5 * oppc * (1.0 + m0 / m1))
+    q1 = abs(q0)
+    m0hat, m1hat = m0 * abs(q0), m1 * abs(q1)
+    B0 = math.sqrt(gamma_flow * oppc * 0.5 * cfl**2 * m0hat * (1.0 + m0hat / m1hat) * sigma)
+    return q0, q1, B0, -beta * B0
+
+
+def boost_flow(u, gamma_flow):
+    """Rest-frame 4-velocities [3, n] boosted to a frame moving along -x with
+    Lorentz factor gamma_flow: u_x' = G (u_x - beta g), u_y, u_z unchanged."""
+    beta = math.sqrt(1.0 - 1.0 / gamma_flow**2)
+    g = torch.sqrt(1.0 + (u * u).sum(0))
+    out = u.clone()
+    out[0] = gamma_flow * (u[0] - beta * g)
+    return out
+
+
+def fill_shock_plasma(world, ppc, theta, gamma_flow, walloc, ramp_len, ramp_factor, seed=1, chunk_bins=2048):
+    """Cold flow along -x in front of a wall at x = walloc: ppc particles per
+    cell per species upstream, ramp_factor * ppc at the wall falling linearly
+    to ppc over ramp_len cells (a compressed layer, so tiles near the wall hold
+    several times the upstream occupancy), none behind the wall.  Positions
+    uniform in each cell (x >= walloc), species 1 on top of species 0, rest
+    frame momenta from the Juttner-Synge law at theta boosted by gamma_flow."""
+    dev = world.dev
+    br = world.brick
+    N = world.N
+    xc = np.arange(N[0], dtype=np.float64)
+    factor = 1.0 + (ramp_factor - 1.0) * np.clip(1.0 - (xc - walloc) / ramp_len, 0.0, 1.0)
+    cnt_x = np.where(xc + 1.0 <= walloc, 0, np.rint(ppc * factor)).astype(np.int64)
+    nbd = [world.n[a] // br[a] for a in range(3)]
+    # particles per brick: bricks are tile-major, layout_right inside the tile
+    b = np.arange(world.nbricks)
+    tile = b // world.bpt
+    lb = b % world.bpt
+    tcx = tile // (world.nt[1] * world.nt[2])
+    lbx = lb // (nbd[1] * nbd[2])
+    x0 = (tcx + world.toff[0]) * world.n[0] + lbx * br[0]
+    csum = np.concatenate([[0], np.cumsum(cnt_x)])
+    demand = (csum[x0 + br[0]] - csum[x0]) * br[1] * br[2]
+    for s in range(world.nsp):
+        world.presize(s, demand)
+    gen = torch.Generator(device=dev)
+    gen.manual_seed(int(seed))
+    cl = torch.stack(torch.meshgrid(*(torch.arange(v, device=dev) for v in br), indexing="ij"), -1).reshape(-1, 3)
+    cnt_x_t = torch.from_numpy(cnt_x).to(dev)
+    nbt = torch.tensor(nbd, device=dev)
+    for b0 in range(0, world.nbricks, chunk_bins):
+        b1 = min(world.nbricks, b0 + chunk_bins)
+        bb = torch.arange(b0, b1, device=dev)
+        t = bb // world.bpt
+        lbb = bb % world.bpt
+        lbc = torch.stack([lbb // (nbt[1] * nbt[2]), (lbb // nbt[2]) % nbt[1], lbb % nbt[2]], -1)
+        tc = torch.stack([t // (world.nt[1] * world.nt[2]), (t // world.nt[2]) % world.nt[1], t % world.nt[2]], -1)
+        origin = (tc + torch.tensor(world.toff, device=dev)) * torch.tensor(world.n, device=dev) + \
+            lbc * torch.tensor(br, device=dev)
+        cells = (origin[:, None, :] + cl[None, :, :]).reshape(-1, 3)      # [nb*cells, 3], brick-major
+        per_cell = cnt_x_t[cells[:, 0]]
+        total = int(per_cell.sum().item())
+        if total == 0:
+            continue
+        base = torch.repeat_interleave(cells, per_cell, dim=0).to(torch.float64)
+        pos = base + torch.rand((total, 3), generator=gen, device=dev, dtype=torch.float64)
+        pos[:, 0] = torch.clamp(pos[:, 0], min=walloc)
+        # keep each float32 position inside its cell (and so inside its brick)
+        upper = (base + 1.0).to(torch.float32)
+        pos = pos.to(torch.float32)
+        pos = torch.where(pos >= upper, torch.nextafter(upper, torch.zeros_like(upper)), pos)
+        pos = pos.t().contiguous()
+        brick_of = torch.repeat_interleave(bb, torch.as_tensor(demand[b0:b1], device=dev))
+        first = torch.zeros(b1 - b0 + 1, dtype=torch.int64, device=dev)
+        first[1:] = torch.cumsum(torch.as_tensor(demand[b0:b1], device=dev), 0)
+        slot = torch.arange(total, device=dev) - first[brick_of - b0]
+        # ids (tile_tag << 40) | ordinal, ordinal = (brick in tile << 20) | slot (< 2^20 per brick)
+        ids = ((brick_of // world.bpt) << 40) | ((brick_of % world.bpt) << 20) | slot
+        for s in range(world.nsp):
+            pool = world.pools[s]
+            dst = pool.start[brick_of] + slot
+            u = boost_flow(juttner_momenta(total, theta, gen, dev).to(torch.float64), gamma_flow).to(torch.float32)
+            for d in range(3):
+                pool.pos[d][dst] = pos[d]
+                pool.vel[d][dst] = u[d]
+            pool.id[dst] = ids
+    for s in range(world.nsp):
+        world.count[s].copy_(torch.as_tensor(demand, dtype=torch.int32, device=dev))
+    torch.cuda.synchronize()
+
+
+def shock_world(nx=2048, ny=128, nz=128, tile=32, ppc=8, theta=1e-4, gamma_flow=10.0, sigma=0.1, cfl=0.45,
+                c_omp=10.0, walloc=15.0, ramp_len=512.0, ramp_factor=4.0, seed=1, brick=None, mig_fraction=0.12):
+    """Reflecting-wall shock set-up: nx x ny x nz cells in tile^3 tiles, a cold
+    gamma_flow flow along -x with a perpendicular field (B0 along z, motional
+    E_y), a wall at walloc with the conducting and upstream edge conditions
+    of projects/pic-shock/pic.py:124-133.  Returns the world, the walls and
+    the edge conditions (as rk structs) and the parameters."""
+    from . import _lib as L
+    nt = (nx // tile, ny // tile, nz // tile)
+    q0, q1, B0, Ey = shock_parameters(cfl, c_omp, ppc, gamma_flow, sigma)
+    w = World((tile,) * 3, nt, (0, 0, 0), nt, periodic=True, cfl=cfl, charges=[q0, q1], masses=[1.0, 1.0],
+              current_filter="binomial2", brick=brick)
+    inner = (slice(HALO, HALO + w.N[0]), slice(HALO, HALO + w.N[1]), slice(HALO, HALO + w.N[2]))
+    w.F.zero_()
+    w.F[0][1][inner] = float(Ey)
+    w.F[1][2][inner] = float(B0)
+    fill_shock_plasma(w, ppc, theta, gamma_flow, walloc, ramp_len, ramp_factor, seed=seed)
+    w.ensure_migration(max(1 << 16, int(mig_fraction * w.total_particles())))
+    w.comm_local(1)
+    w.comm_local(2)
+    torch.cuda.synchronize()
+    wall = L.ReflectorWall(float(np.float32(walloc)), 0.0, 1.0)
+    Lx = float(nx)
+    conducting = L.EdgeBC(0, 0, float(np.float32(walloc)), (0.0, 0.0, 0.0), (0.0, 0.0, 0.0), (0.0, 0.0, 0.0),
+                          0b110, 0, 0b111)
+    upstream = L.EdgeBC(0, 1, float(np.float32(Lx - 5)), (0.0, float(np.float32(Ey)), 0.0),
+                        (0.0, 0.0, float(np.float32(B0))), (0.0, 0.0, 0.0), 0b111, 0b111, 0b111)
+    return w, [wall], [conducting, upstream], dict(q0=q0, q1=q1, B0=B0, Ey=Ey)

@@ -7,11 +7,22 @@ without the lap-function dispatch of Simulation:
 * PlainLap: the same lap in the per-tile form of the reference driver, one call
   per tile per method, each method synchronous over all tiles (Simulation
   "plain").
+* ShockLap / PlainShockLap: the reflecting-wall shock lap of
+  projects/pic-shock/pic.py:250-299 (edge conditions after every field
+  update, reflect between push and pack, the wall advanced per lap; IO and
+  the injector excluded), batched (one launch per method over every tile;
+  the particle phase unfused, since the reflect sits inside it) and per tile.
 """
 
+import ctypes
+
+from . import _lib as L
+
+import numpy as np
 import torch
 
 from .tools import comm_mode
+from .world import _stream
 
 
 class BatchedLap:
@@ -84,23 +95,31 @@ class GraphLap:
     Every filter pass swaps the roles of J and the filter target, so with an
     odd number of passes consecutive laps run on swapped buffers: two graphs
     are captured (even and odd laps) and replayed alternately, starting with
-    the even one.  Capture records kernels only (the library allocates nothing
+    the even one.  `lap` may instead be a batched ShockLap whose walls stand
+    still (the reflect launches are fixed by the wall positions; run one
+    eager lap first, which allocates the wall's correction buffers).  Capture records kernels only (the library allocates nothing
     and never synchronises inside these calls; kernel timing must be off).
     After every replay settle() checks the overflow counter (synchronous), as
     for BatchedLap.
     """
 
-    def __init__(self, world, filter_passes=3):
+    def __init__(self, world, filter_passes=3, lap=None):
         self.w = world
-        self.lap = BatchedLap(world, filter_passes)
-        if world.mbox is None or world.jstage is None:
-            world.ensure_mailbox()
+        if lap is None:
+            lap = BatchedLap(world, filter_passes)
+            if world.mbox is None or world.jstage is None:
+                world.ensure_mailbox()
+        elif not isinstance(lap, ShockLap) or lap.tiles is not None:
+            raise ValueError("only BatchedLap and the batched ShockLap are captured")
+        elif any(float(wall.betawall) != 0.0 for wall in lap.walls):
+            raise ValueError("a moving wall changes the reflect launch every lap; replay it eagerly")
+        self.lap = lap
         torch.cuda.synchronize()
         self.graphs = []
         side = torch.cuda.Stream()
         side.wait_stream(torch.cuda.current_stream())
         with torch.cuda.stream(side):
-            for _ in range(2 if filter_passes % 2 else 1):
+            for _ in range(2 if self.lap.filter_passes % 2 else 1):
                 g = torch.cuda.CUDAGraph()
                 with torch.cuda.graph(g, stream=side):
                     self.lap()
@@ -116,3 +135,78 @@ class GraphLap:
     def settle(self):
         if self.w.pending_overflow():
             self.w.settle_overflow()
+
+
+class ShockLap:
+    """walls: list of rk_reflector_wall; bcs: list of rk_edge_bc (all tiles)."""
+
+    def __init__(self, world, walls, bcs, filter_passes=3, tiles=None):
+        self.w = world
+        self.walls = list(walls)
+        self.bcs = list(bcs)
+        self.filter_passes = filter_passes
+        self.tiles = tiles  # None: one launch per method over every tile
+
+    def _targets(self):
+        return [-1] if self.tiles is None else range(self.w.ntiles)
+
+    def _each(self, name):
+        for t in self._targets():
+            self.w.call(name, t)
+        if self.tiles is not None:
+            torch.cuda.synchronize()
+
+    def _bcs(self, mode):
+        for t in self._targets():
+            for bc in self.bcs:
+                L.check(L.lib().rk_apply_edge_bc(self.w.h, t, ctypes.byref(bc), int(mode), _stream()))
+        if self.tiles is not None:
+            torch.cuda.synchronize()
+
+    def _reflect(self):
+        arr = (L.ReflectorWall * len(self.walls))(*self.walls)
+        for t in self._targets():
+            L.check(L.lib().rk_reflect_particles(self.w.h, t, arr, len(self.walls), _stream()))
+        if self.tiles is not None:
+            torch.cuda.synchronize()
+
+    def __call__(self):
+        w = self.w
+        self._each("rk_push_half_b")
+        self._bcs(comm_mode.emf_B)
+        w.comm_local(int(comm_mode.emf_B))
+        self._each("rk_push_particles")
+        self._reflect()
+        self._each("rk_pack_outgoing_particles")
+        w.comm_local(int(comm_mode.pic_particle))
+        self._each("rk_deposit_current")
+        w.comm_local(int(comm_mode.emf_J_exchange))
+        w.comm_local(int(comm_mode.emf_J))
+        self._bcs(comm_mode.emf_J)
+        for p in range(self.filter_passes):
+            if p > 0 and p % 3 == 0:
+                w.comm_local(int(comm_mode.emf_J))
+            self._each("rk_filter_current")
+        self._bcs(comm_mode.emf_J)
+        self._each("rk_push_half_b")
+        self._bcs(comm_mode.emf_B)
+        w.comm_local(int(comm_mode.emf_B))
+        self._each("rk_push_e")
+        self._bcs(comm_mode.emf_E)
+        self._each("rk_add_current")
+        self._bcs(comm_mode.emf_E)
+        w.comm_local(int(comm_mode.emf_E))
+        cfl = np.float32(w.cfl)
+        for wall in self.walls:
+            wall.walloc = float(np.float32(wall.walloc) + np.float32(wall.betawall) * cfl)
+        if self.tiles is not None:
+            torch.cuda.synchronize()
+
+    def settle(self):
+        if self.w.pending_overflow():
+            self.w.settle_overflow()
+
+
+class PlainShockLap(ShockLap):
+    def __init__(self, world, walls, bcs, filter_passes=3):
+        super().__init__(world, walls, bcs, filter_passes, tiles="all")
