@@ -186,6 +186,16 @@ int rk_sort_particles(rk_world* w, int32_t tile, void* stream);
  * tile.  No counterpart in the reference (its driver issues those four). */
 int rk_particle_step_fused(rk_world* w, void* stream);
 
+/* The fused particle phase with reflecting walls: the tile methods
+ * push_particles, reflect_particles (every wall in registration order on the
+ * tiles it lies in, src/runko/pic/reflector_wall.c++:253-289),
+ * pack_outgoing_particles, comm_local(pic_particle), deposit_current of the
+ * reference's shock lap (projects/pic-shock/pic.py:259-270) in one launch;
+ * the correction currents of the reflections are added to J before it
+ * returns.  n_walls = 0 is rk_particle_step_fused.  At most 8 walls.  No
+ * counterpart in the reference (its driver issues those five). */
+int rk_particle_step_fused_walls(rk_world* w, const rk_reflector_wall* walls, int32_t n_walls, void* stream);
+
 /* E += cfl curl B followed by E -= J in one pass (same two roundings per
  * node as rk_push_e then rk_add_current).  Batched form of that pair. */
 int rk_push_e_add_current(rk_world* w, int32_t tile, void* stream);

@@ -3,6 +3,7 @@
 // behave like the reference's tiles, communication, overflow handling and
 // readback.
 #include "rk_internal.h"
+#include "rk_reflect.h"
 
 #include <algorithm>
 #include <cmath>
@@ -102,6 +103,18 @@ int read_counter(rk_world* w, int which, uint32_t& v, hipStream_t s)
 {
   RK_HIP(hipMemcpyAsync(&v, &w->counters[which], sizeof(uint32_t), hipMemcpyDeviceToHost, s));
   RK_HIP(hipStreamSynchronize(s));
+  return RK_OK;
+}
+
+// Record buffers and the overflow list may only be replaced while no
+// overflow record is pending: the list indexes the current buffers.
+int check_no_pending(rk_world* w)
+{
+  if(!w->counters) return RK_OK;
+  uint32_t v = 0;
+  RK_HIP(hipDeviceSynchronize());
+  RK_HIP(hipMemcpy(&v, &w->counters[C_OVF], sizeof(uint32_t), hipMemcpyDeviceToHost));
+  if(v) return fail(RK_ESTATE, "overflow records pending: repack and retry the append before replacing record buffers");
   return RK_OK;
 }
 
@@ -285,6 +298,7 @@ int rk_world_set_migration(rk_world* w, int32_t set, int64_t capacity, float* co
 {
   if(!w || set < 0 || set > 1) return fail(RK_EINVAL, "invalid record set");
   if(capacity >= (int64_t(1) << 30)) return fail(RK_EINVAL, "record set capacity must be < 2^30");
+  if(int rc = check_no_pending(w)) return rc;
   auto& m = w->mig[set];
   m.capacity = capacity;
   for(int d = 0; d < 3; ++d) {
@@ -317,6 +331,7 @@ int rk_world_set_mailbox(rk_world* w, int64_t per_brick, float* const pos[3], fl
                          uint32_t* meta)
 {
   if(!w) return fail(RK_EINVAL, "null world");
+  if(int rc = check_no_pending(w)) return rc;
   const int64_t cap = per_brick * int64_t(w->g.nbricks);
   if(per_brick <= 0 || cap >= (int64_t(1) << 30)) return fail(RK_EINVAL, "mailbox capacity must be in (0, 2^30)");
   auto& m = w->mbox;
@@ -488,15 +503,23 @@ int rk_sort_particles(rk_world* w, int32_t tile, void* stream)
   return launch_sort(w, tile, S(stream));
 }
 
-int rk_particle_step_fused(rk_world* w, void* stream)
+int rk_particle_step_fused(rk_world* w, void* stream) { return rk_particle_step_fused_walls(w, nullptr, 0, stream); }
+
+int rk_particle_step_fused_walls(rk_world* w, const rk_reflector_wall* walls, int32_t n_walls, void* stream)
 {
   if(int rc = check_tile(w, -1)) return rc;
+  if(n_walls < 0 || n_walls > MAX_WALLS) return fail(RK_EINVAL, "at most 8 reflector walls");
+  if(n_walls > 0 && !walls) return fail(RK_EINVAL, "null wall array");
   if(int rc = check_particles(w)) return rc;
   if(!w->g.periodic) return fail(RK_EINVAL, "the fused particle step needs a periodic tile grid");
   if(!w->mbox.meta || !w->mig[0].meta || !w->mig[1].meta || !w->jstage)
     return fail(RK_EINVAL, "fused particle step: mailbox, migration or current staging buffers not set");
   if(w->filter_open) return fail(RK_ESTATE, "particle step during an open filter pass");
   if(w->g.pusher != 0) return fail(RK_ESTATE, "the fused particle step implements the boris pusher only");
+  WallSet ws {};
+  for(int i = 0; i < n_walls; ++i) ws.w[ws.n++] = walls[i];
+  if(ws.n > 0)
+    if(int rc = mark_wall_tiles(w, ws)) return rc;
   hipStream_t s = S(stream);
   {
     ProfScope ps(w, 8, s);
@@ -505,7 +528,7 @@ int rk_particle_step_fused(rk_world* w, void* stream)
   w->j_accumulating = true;
   {
     ProfScope ps(w, 0, s);
-    if(int rc = launch_fused(w, s)) return rc;
+    if(int rc = launch_fused(w, s, &ws)) return rc;
   }
   {
     ProfScope ps(w, 2, s);

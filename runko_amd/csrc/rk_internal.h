@@ -243,8 +243,10 @@ int verify_fast_division(float cfl, float rcp, uint64_t mismatches[2]);
 int upload_mig_table(rk_world* w);
 int launch_reset_counter(rk_world* w, int which, hipStream_t s);
 int apply_corrections(rk_world* w, int tile, hipStream_t s);  // rk_wall.hip
+struct WallSet;                                               // rk_reflect.h
+int mark_wall_tiles(rk_world* w, const WallSet& ws);          // rk_wall.hip
 int launch_sort(rk_world* w, int tile, hipStream_t s);
-int launch_fused(rk_world* w, hipStream_t s);
+int launch_fused(rk_world* w, hipStream_t s, const WallSet* walls = nullptr);
 int kinetic_energies(rk_world* w, int tile, double* out, hipStream_t s);
 int launch_inject(rk_world* w, int tile, int s, int64_t n, const float* const pos[3],
                   const float* const vel[3], const uint64_t* id, hipStream_t st);

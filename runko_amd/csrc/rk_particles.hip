@@ -25,6 +25,7 @@
 // atomic per wave (ballot + popcount) and their slots are refilled from the
 // bin tail (swap-remove) so bins stay compact.
 #include "rk_internal.h"
+#include "rk_reflect.h"
 
 #include <algorithm>
 
@@ -1082,6 +1083,10 @@ __global__ void __launch_bounds__(WG)
     if(list) {
       st = int(list[e] >> 30);
       r = list[e] & 0x3fffffffu;
+      if(st > 2 || int64_t(r) >= m.cap[st]) {  // not a record of the current buffers
+        atomicAdd(&counters[C_MIGLOST], 1u);
+        continue;
+      }
     }
     append_record(g, sp, m, st, r, counters, ovf, ovf_cap);
   }
@@ -1356,10 +1361,15 @@ struct NbrTab {
   float wlo[3], whi[3], wlen[3];  // periodic box
 };
 
-template<int BR, int FAST>
+// WALL: reflecting walls between the push and the classification
+// (reflector_wall.c++:253-289 inside the step): the walls that lie in the
+// brick's tile reflect or park each pushed particle, reflections add their
+// correction current to the world buffer Jc, parked particles leave the bin
+// without a record and deposit nothing.
+template<int BR, int FAST, int WALL = 0>
 __global__ void __launch_bounds__(FWG) __attribute__((amdgpu_waves_per_eu(4)))
   k_fused(Geo g, SpParams sp, FieldPtrs f, MigPtrs m, const MigPtrs* mtab, int nbins, uint32_t* counters,
-          float* jstage, int64_t jstage_stride)
+          float* jstage, int64_t jstage_stride, WallSet walls, float* Jc0, float* Jc1, float* Jc2)
 {
   extern __shared__ double lds_dyn[];
   __shared__ int wcnt[2 * (FWG / 64)];
@@ -1398,6 +1408,8 @@ __global__ void __launch_bounds__(FWG) __attribute__((amdgpu_waves_per_eu(4)))
     }
   }
   __syncthreads();
+  // walls in this brick's tile (uniform over the block)
+  const unsigned wmask = WALL ? walls_in_tile(walls, Bk.tmin[0], Bk.tmax[0], g.cfl) : 0u;
   int nrec = 0;  // records in this brick's mailbox (uniform over the block)
   for(int s = 0; any && s < sp.nsp; ++s) {
     // bin start and count are uniform: keep them (and the bin's array bases)
@@ -1436,7 +1448,7 @@ __global__ void __launch_bounds__(FWG) __attribute__((amdgpu_waves_per_eu(4)))
     }
     for(int base = 0; base < count; base += FWG) {
       const int t = base + int(threadIdx.x);
-      bool lv0 = false, lv1 = false;
+      bool lv0 = false, lv1 = false, dead = false;
       float pos[3] = {npos[0], npos[1], npos[2]}, vel[3] = {nvel[0], nvel[1], nvel[2]};
       const uint64_t id = nid;
       int dir = 13;
@@ -1464,6 +1476,13 @@ __global__ void __launch_bounds__(FWG) __attribute__((amdgpu_waves_per_eu(4)))
           gather(g, f, MeanWin {{0, 0, 0}, {0, 0, 0}}, nullptr, Bk.toffh[0] + ci[0], Bk.toffh[1] + ci[1],
                  Bk.toffh[2] + ci[2], dd, E, Bf, &counters[C_STENCIL]);
         boris<FAST>(g, qm, E, Bf, pos, vel);
+        if(WALL && wmask) {
+          float* const Jc[3] = {Jc0, Jc1, Jc2};
+          dead = !reflect_particle(g, walls, wmask, g.cfl, Bk.origo, Bk.toffh, Jc, charge, pos, vel,
+                                   &counters[C_STENCIL]);
+        }
+      }
+      if(t < count && !dead) {
         // classification against the tile (pack_outgoing) and the brick
         dir = tile_dir(pos, Bk.tmin, Bk.tmax);
         float x2[3];
@@ -1517,13 +1536,16 @@ __global__ void __launch_bounds__(FWG) __attribute__((amdgpu_waves_per_eu(4)))
           atomicAdd(&counters[C_MIGLOST], 1u);
         }
       }
-      const bool stays = t < count && (!hole || kept);
+      const bool stays = t < count && !dead && (!hole || kept);
       // survivor's slot: wr + lanes before it in the chunk - leavers before
       // it that departed (r counts all leavers, rk the kept ones)
       int kept_tot = 0;
       int rk = 0;
       if(int64_t(nrec) + tot > m.per_brick) rk = block_rank1<FWG>(hole && kept, wcnt, par, kept_tot);
-      const int dst = wr + int(threadIdx.x) - (r - rk);
+      // parked particles before this lane (they are not leavers)
+      int dead_tot = 0, rd = 0;
+      if(WALL && wmask) rd = block_rank1<FWG>(dead, wcnt, par, dead_tot);
+      const int dst = wr + int(threadIdx.x) - (r - rk) - rd;
       if(t < count && hole && !kept) {
         // records keep the unwrapped position; the append applies the wrap
         const uint32_t meta = lv0 ? meta_pack(s, 0, dir, uint32_t(b)) : meta_pack(s, 1, 13, uint32_t(b));
@@ -1540,7 +1562,7 @@ __global__ void __launch_bounds__(FWG) __attribute__((amdgpu_waves_per_eu(4)))
         U2[d] = vel[2];
         if(d != uint32_t(t)) ID[d] = id;
       }
-      wr += min(FWG, count - base) - (tot - kept_tot);
+      wr += min(FWG, count - base) - (tot - kept_tot) - dead_tot;
       nrec += (int64_t(nrec) + tot <= m.per_brick) ? tot : int(m.per_brick - nrec);
     }
     __syncthreads();
@@ -1877,20 +1899,30 @@ int launch_reset_counter(rk_world* w, int which, hipStream_t s)
   return RK_OK;
 }
 
-int launch_fused(rk_world* w, hipStream_t s)
+int launch_fused(rk_world* w, hipStream_t s, const WallSet* walls)
 {
   const Geo& g = w->g;
   const size_t dyn = means_lds(g) + cur_lds(g);
   const dim3 grid(xcd_grid(g.nbricks));
+  WallSet ws {};
+  if(walls) ws = *walls;
   auto go = [&](auto kern) {
     hipLaunchKernelGGL(kern, grid, dim3(FWG), dyn, s, g, make_sp(w), make_fields(w), make_mig(w),
                        static_cast<const MigPtrs*>(w->mig_table), g.nbricks, w->counters, w->jstage,
-                       w->jstage_stride);
+                       w->jstage_stride, ws, w->corrJ[0], w->corrJ[1], w->corrJ[2]);
   };
   const bool cube8 = g.br[0] == 8 && g.br[1] == 8 && g.br[2] == 8;
   const bool cube4 = g.br[0] == 4 && g.br[1] == 4 && g.br[2] == 4;
   const int lvl = g.fast_div ? (g.fast_sqrt ? 2 : 1) : -1;
-  if(cube8) lvl == 2 ? go(k_fused<8, 2>) : (lvl == 1 ? go(k_fused<8, 1>) : go(k_fused<8, -1>));
+  if(ws.n > 0) {
+    if(!w->corrJ[0]) {
+      set_error("wall correction buffer not set");
+      return RK_EINVAL;
+    }
+    if(cube8) lvl == 2 ? go(k_fused<8, 2, 1>) : (lvl == 1 ? go(k_fused<8, 1, 1>) : go(k_fused<8, -1, 1>));
+    else if(cube4) lvl == 2 ? go(k_fused<4, 2, 1>) : (lvl == 1 ? go(k_fused<4, 1, 1>) : go(k_fused<4, -1, 1>));
+    else go(k_fused<0, 0, 1>);
+  } else if(cube8) lvl == 2 ? go(k_fused<8, 2>) : (lvl == 1 ? go(k_fused<8, 1>) : go(k_fused<8, -1>));
   else if(cube4) lvl == 2 ? go(k_fused<4, 2>) : (lvl == 1 ? go(k_fused<4, 1>) : go(k_fused<4, -1>));
   else go(k_fused<0, 0>);
   RK_HIP(hipGetLastError());

@@ -15,6 +15,7 @@
 // atomics); deposit_current of each affected tile adds its window of that
 // buffer to J and clears it.
 #include "rk_internal.h"
+#include "rk_reflect.h"
 
 #include <algorithm>
 #include <vector>
@@ -24,12 +25,6 @@ namespace rk {
 namespace {
 
 constexpr int WWG = 256;
-constexpr int MAX_WALLS = 8;
-
-struct WallSet {
-  rk_reflector_wall w[MAX_WALLS];
-  int n;
-};
 
 struct SpeciesPtrs {
   float* pos[RK_MAX_SPECIES][3];
@@ -58,71 +53,8 @@ SpeciesPtrs species_ptrs(const rk_world* w)
   return p;
 }
 
-__device__ inline float dot3w(const float a[3], const float b[3])
-{
-  float r = a[0] * b[0];
-  r = r + a[1] * b[1];
-  r = r + a[2] * b[2];
-  return r;
-}
-
-// zigzag deposit of one sub-trajectory x1 -> x2 (tile-lattice coordinates)
-// into the correction buffer, reflector_wall.c++:36-118
-__device__ void zigzag_single(const Geo& g, float* const Jc[3], const int toffh[3], const float x1[3],
-                              const float x2[3], float charge, uint32_t* err)
-{
-  const float one = 1.0f;
-  float fi1[3], fi2[3], xr[3], F1[3], F2[3], W1[3], W2[3];
-#pragma unroll
-  for(int d = 0; d < 3; ++d) {
-    fi1[d] = floorf(x1[d]);
-    fi2[d] = floorf(x2[d]);
-  }
-#pragma unroll
-  for(int d = 0; d < 3; ++d) {
-    const float a = fminf(fi1[d], fi2[d]) + one;
-    const float b = fmaxf(fmaxf(fi1[d], fi2[d]), 0.5f * (x1[d] + x2[d]));
-    xr[d] = fminf(a, b);
-  }
-#pragma unroll
-  for(int d = 0; d < 3; ++d) {
-    F1[d] = charge * (xr[d] - x1[d]);
-    F2[d] = charge * (x2[d] - xr[d]);
-    W1[d] = 0.5f * (x1[d] + xr[d]) - fi1[d];
-    W2[d] = 0.5f * (x2[d] + xr[d]) - fi2[d];
-  }
-  const int a1 = int(uint32_t(fi1[0])), b1 = int(uint32_t(fi1[1])), c1 = int(uint32_t(fi1[2]));
-  const int a2 = int(uint32_t(fi2[0])), b2 = int(uint32_t(fi2[1])), c2 = int(uint32_t(fi2[2]));
-  auto st = [&](int i, int j, int k, float jx, float jy, float jz) {
-    const int I = toffh[0] + i, J = toffh[1] + j, K = toffh[2] + k;
-    if(I < 0 || J < 0 || K < 0 || I >= g.L[0] || J >= g.L[1] || K >= g.L[2]) {
-      atomicAdd(err, 1u);
-      return;
-    }
-    const uint32_t q = g.at(I, J, K);
-    unsafeAtomicAdd(&Jc[0][q], jx);
-    unsafeAtomicAdd(&Jc[1][q], jy);
-    unsafeAtomicAdd(&Jc[2][q], jz);
-  };
-  const float Fx1 = F1[0], Fy1 = F1[1], Fz1 = F1[2], Fx2 = F2[0], Fy2 = F2[1], Fz2 = F2[2];
-  const float Wx1 = W1[0], Wy1 = W1[1], Wz1 = W1[2], Wx2 = W2[0], Wy2 = W2[1], Wz2 = W2[2];
-  st(a1, b1, c1, Fx1 * (one - Wy1) * (one - Wz1), Fy1 * (one - Wx1) * (one - Wz1), Fz1 * (one - Wx1) * (one - Wy1));
-  st(a2, b2, c2, Fx2 * (one - Wy2) * (one - Wz2), Fy2 * (one - Wx2) * (one - Wz2), Fz2 * (one - Wx2) * (one - Wy2));
-  st(a1 + 1, b1, c1, 0.0f, Fy1 * Wx1 * (one - Wz1), Fz1 * Wx1 * (one - Wy1));
-  st(a2 + 1, b2, c2, 0.0f, Fy2 * Wx2 * (one - Wz2), Fz2 * Wx2 * (one - Wy2));
-  st(a1, b1 + 1, c1, Fx1 * Wy1 * (one - Wz1), 0.0f, Fz1 * (one - Wx1) * Wy1);
-  st(a2, b2 + 1, c2, Fx2 * Wy2 * (one - Wz2), 0.0f, Fz2 * (one - Wx2) * Wy2);
-  st(a1, b1, c1 + 1, Fx1 * (one - Wy1) * Wz1, Fy1 * (one - Wx1) * Wz1, 0.0f);
-  st(a2, b2, c2 + 1, Fx2 * (one - Wy2) * Wz2, Fy2 * (one - Wx2) * Wz2, 0.0f);
-  st(a1, b1 + 1, c1 + 1, Fx1 * Wy1 * Wz1, 0.0f, 0.0f);
-  st(a2, b2 + 1, c2 + 1, Fx2 * Wy2 * Wz2, 0.0f, 0.0f);
-  st(a1 + 1, b1, c1 + 1, 0.0f, Fy1 * Wx1 * Wz1, 0.0f);
-  st(a2 + 1, b2, c2 + 1, 0.0f, Fy2 * Wx2 * Wz2, 0.0f);
-  st(a1 + 1, b1 + 1, c1, 0.0f, 0.0f, Fz1 * Wx1 * Wy1);
-  st(a2 + 1, b2 + 1, c2, 0.0f, 0.0f, Fz2 * Wx2 * Wy2);
-}
-
-// One block per bin of the tile; walls in registration order.
+// One block per bin of the tile; every particle through the tile's walls in
+// registration order.
 __global__ void __launch_bounds__(WWG)
   k_reflect(Geo g, SpeciesPtrs sp, WallSet walls, int b0, float cfl, float origo0, float origo1, float origo2,
             int toffh0, int toffh1, int toffh2, float* Jc0, float* Jc1, float* Jc2, uint32_t* counters)
@@ -131,70 +63,22 @@ __global__ void __launch_bounds__(WWG)
   const float origo[3] = {origo0, origo1, origo2};
   const int toffh[3] = {toffh0, toffh1, toffh2};
   float* const Jc[3] = {Jc0, Jc1, Jc2};
-  const float EPS = 1e-10f;
-  const float one = 1.0f;
-  const float c = cfl;
-  for(int wi = 0; wi < walls.n; ++wi) {
-    const float walloc = walls.w[wi].walloc, betawall = walls.w[wi].betawall, gammawall = walls.w[wi].gammawall;
-    const float walloc0 = walloc - betawall * c;
-    for(int s = 0; s < sp.nsp; ++s) {
-      const int64_t start = sp.start[s][b];
-      const int count = int(sp.count[s][b]);
-      const float charge = sp.charge[s];
-      for(int t = int(threadIdx.x); t < count; t += WWG) {
-        const int64_t q = start + t;
-        if(sp.id[s][q] == DEAD_ID) continue;
-        const float pos1[3] = {sp.pos[s][0][q], sp.pos[s][1][q], sp.pos[s][2][q]};
-        const float u[3] = {sp.vel[s][0][q], sp.vel[s][1][q], sp.vel[s][2][q]};
-        const float gam = sqrtf(one + dot3w(u, u));
-        const float invgam = one / gam;
-        float pos0[3];
+  const unsigned all = (1u << walls.n) - 1u;
+  for(int s = 0; s < sp.nsp; ++s) {
+    const int64_t start = sp.start[s][b];
+    const int count = int(sp.count[s][b]);
+    const float charge = sp.charge[s];
+    for(int t = int(threadIdx.x); t < count; t += WWG) {
+      const int64_t q = start + t;
+      if(sp.id[s][q] == DEAD_ID) continue;
+      float pos[3] = {sp.pos[s][0][q], sp.pos[s][1][q], sp.pos[s][2][q]};
+      float u[3] = {sp.vel[s][0][q], sp.vel[s][1][q], sp.vel[s][2][q]};
+      if(pos[0] >= walls_max_x(walls)) continue;  // in front of every wall
+      const bool alive = reflect_particle(g, walls, all, cfl, origo, toffh, Jc, charge, pos, u, &counters[C_STENCIL]);
 #pragma unroll
-        for(int d = 0; d < 3; ++d) pos0[d] = pos1[d] - c * invgam * u[d];
-        const float mask_skip = (pos1[0] >= walloc) ? one : 0.0f;
-        if(mask_skip == one) continue;  // in front of the wall: untouched (all masks zero)
-        const float mask_close = (walloc0 - pos0[0] <= c) ? one : 0.0f;
-        const float denom = betawall * c - c * u[0] * invgam;
-        const float dt = fabsf((pos0[0] - walloc0) / (denom + EPS));
-        const float mask_crossed = (dt <= one) ? one : 0.0f;
-        const float mask_refl = (one - mask_skip) * mask_close * mask_crossed;
-        const float mask_park = (one - mask_skip) - mask_refl;
-        if(mask_refl > 0.0f) {
-          float pcol[3];
-#pragma unroll
-          for(int d = 0; d < 3; ++d) pcol[d] = pos0[d] + c * dt * invgam * u[d];
-          const float ux_new =
-            gammawall * gammawall * gam * (2.0f * betawall - u[0] * invgam * (one + betawall * betawall));
-          const float un[3] = {ux_new, u[1], u[2]};
-          const float gam_new = sqrtf(one + dot3w(un, un));
-          const float invgam_new = one / gam_new;
-          const float dt_refl = fminf(one, fabsf((pos1[0] - pcol[0]) / (pos1[0] - pos0[0] + EPS)));
-          float prefl[3], p1l[3], f0[3], f1[3], r0[3], r1[3];
-#pragma unroll
-          for(int d = 0; d < 3; ++d) {
-            prefl[d] = pcol[d] + c * dt_refl * invgam_new * un[d];
-            p1l[d] = pos1[d] - origo[d];
-          }
-#pragma unroll
-          for(int d = 0; d < 3; ++d) {
-            f0[d] = p1l[d] + mask_refl * (pos0[d] - pos1[d]);
-            f1[d] = p1l[d] + mask_refl * (pcol[d] - pos1[d]);
-          }
-          zigzag_single(g, Jc, toffh, f0, f1, mask_refl * charge, &counters[C_STENCIL]);
-#pragma unroll
-          for(int d = 0; d < 3; ++d) {
-            const float x1d = prefl[d] - c * invgam_new * un[d];
-            r0[d] = p1l[d] + mask_refl * (x1d - pos1[d]);
-            r1[d] = p1l[d] + mask_refl * (pcol[d] - pos1[d]);
-          }
-          zigzag_single(g, Jc, toffh, r0, r1, mask_refl * (-charge), &counters[C_STENCIL]);
-#pragma unroll
-          for(int d = 0; d < 3; ++d) sp.pos[s][d][q] = (one - mask_refl) * pos1[d] + mask_refl * prefl[d];
-          sp.vel[s][0][q] = (one - mask_refl) * u[0] + mask_refl * ux_new;
-        }
-        if(mask_park > 0.5f) sp.id[s][q] = DEAD_ID;
-      }
-      __syncthreads();
+      for(int d = 0; d < 3; ++d) sp.pos[s][d][q] = pos[d];
+      sp.vel[s][0][q] = u[0];
+      if(!alive) sp.id[s][q] = DEAD_ID;
     }
   }
 }
@@ -300,6 +184,38 @@ int fail_w(int code, const char* msg)
 
 }  // namespace
 
+// the world buffer of wall correction currents, allocated (zeroed) on first use
+int ensure_corrections(rk_world* w)
+{
+  if(w->corrJ[0]) return RK_OK;
+  const size_t nb = sizeof(float) * size_t(w->g.nodes());
+  for(int c = 0; c < 3; ++c) {
+    RK_HIP(hipMalloc(reinterpret_cast<void**>(&w->corrJ[c]), nb));
+    RK_HIP(hipMemset(w->corrJ[c], 0, nb));
+  }
+  w->corr_pending.assign(size_t(w->g.ntiles()), 0);
+  return RK_OK;
+}
+
+// tiles that a wall of ws lies in get their corrections added by the next
+// apply_corrections (the fused step's reflect stage writes them)
+int mark_wall_tiles(rk_world* w, const WallSet& ws)
+{
+  if(int rc = ensure_corrections(w)) return rc;
+  const Geo& g = w->g;
+  for(int t = 0; t < g.ntiles(); ++t) {
+    int tc[3];
+    g.tile_coords(t, tc);
+    const float tmin = g.tile_min(tc[0], 0);
+    const float tmax = float((g.toff[0] + tc[0] + 1) * g.n[0]);
+    if(walls_in_tile(ws, tmin, tmax, g.cfl)) {
+      w->corr_pending[size_t(t)] = 1;
+      w->corr_any = 1;
+    }
+  }
+  return RK_OK;
+}
+
 int apply_corrections(rk_world* w, int tile, hipStream_t s)
 {
   if(!w->corr_any) return RK_OK;
@@ -337,14 +253,7 @@ int rk_reflect_particles(rk_world* w, int32_t tile, const rk_reflector_wall* wal
   for(int s = 0; s < w->d.n_species; ++s)
     if(!w->sp[s].id) return fail_w(RK_EINVAL, "particle pools not set");
   hipStream_t st = reinterpret_cast<hipStream_t>(stream);
-  if(!w->corrJ[0]) {
-    const size_t nb = sizeof(float) * size_t(g.nodes());
-    for(int c = 0; c < 3; ++c) {
-      RK_HIP(hipMalloc(reinterpret_cast<void**>(&w->corrJ[c]), nb));
-      RK_HIP(hipMemset(w->corrJ[c], 0, nb));
-    }
-    w->corr_pending.assign(size_t(g.ntiles()), 0);
-  }
+  if(int rc = ensure_corrections(w)) return rc;
   const float cfl = g.cfl;
   const SpeciesPtrs sp = species_ptrs(w);
   const int t0 = tile < 0 ? 0 : tile, t1 = tile < 0 ? g.ntiles() : tile + 1;

@@ -41,7 +41,8 @@ _RK_TILE = {"push_particles": "rk_push_particles", "sort_particles": "rk_sort_pa
             "pack_outgoing_particles": "rk_pack_outgoing_particles", "deposit_current": "rk_deposit_current",
             "push_half_b": "rk_push_half_b", "push_e": "rk_push_e", "add_current": "rk_add_current",
             "filter_current": "rk_filter_current"}
-_NEXT = {"push": ("pack",), "pack": ("comm",), "comm": ("sort", "deposit"), "sort": ("deposit",)}
+_NEXT = {"push": ("pack", "reflect"), "reflect": ("pack",), "pack": ("comm",), "comm": ("sort", "deposit"),
+         "sort": ("deposit",)}
 
 
 class Simulation:
@@ -64,6 +65,7 @@ class Simulation:
         self._prev_elapsed_wall_time = 0
         self._total_interval_time = 0
         self._pending = []
+        self._pending_walls = None
         self._warned = set()
         self._emf_writer = None
         self._prtcl_writers = {}
@@ -160,6 +162,10 @@ class Simulation:
             w.call("rk_sort_particles", -1)
         elif op == "deposit":
             w.call("rk_deposit_current", -1)
+        elif op == "reflect":
+            # one launch per tile that a wall overlaps, issued by the library
+            arr = self._pending_walls
+            L.check(L.lib().rk_reflect_particles(w.h, -1, arr, len(arr), _stream()))
 
     def _flush(self):
         ops, self._pending = self._pending, []
@@ -170,9 +176,10 @@ class Simulation:
         if self._pending and op in _NEXT.get(self._pending[-1], ()):
             self._pending.append(op)
             if op == "deposit":
+                walls = list(self._pending_walls) if "reflect" in self._pending else None
                 self._pending = []
                 w = self._world
-                w.particle_step_fused()
+                w.particle_step_fused(walls)
                 if w.pending_overflow():
                     w.settle_overflow()
             return
@@ -207,13 +214,12 @@ class Simulation:
                           "sort_particles": "sort", "deposit_current": "deposit"}[m]
                     self._particle_op(op)
                 elif batched and m == "reflect_particles" and self._same_on_all_tiles("_walls"):
-                    # one launch per tile that a wall overlaps, issued by the library
-                    self._flush()
-                    tiles = self._tile_grid.tiles()
-                    walls = tiles[0]._walls
+                    # joins a pending fused particle phase (push -> reflect ->
+                    # pack -> comm -> deposit), else runs at once
+                    walls = self._tile_grid.tiles()[0]._walls
                     if walls:
-                        arr = _walls_array(walls)
-                        L.check(L.lib().rk_reflect_particles(self._world.h, -1, arr, len(walls), _stream()))
+                        self._pending_walls = _walls_array(walls)
+                        self._particle_op("reflect")
                 else:
                     self._flush()
                     for tile in self.local_tiles():

@@ -95,10 +95,11 @@ class GraphLap:
     Every filter pass swaps the roles of J and the filter target, so with an
     odd number of passes consecutive laps run on swapped buffers: two graphs
     are captured (even and odd laps) and replayed alternately, starting with
-    the even one.  `lap` may instead be a batched ShockLap whose walls stand
-    still (the reflect launches are fixed by the wall positions; run one
-    eager lap first, which allocates the wall's correction buffers).  Capture records kernels only (the library allocates nothing
-    and never synchronises inside these calls; kernel timing must be off).
+    the even one.  `lap` may instead be a fused ShockLap whose walls stand
+    still (the walls travel as launch arguments) and which has run one eager
+    lap (that lap allocates the wall correction buffer).  Capture records
+    kernels only (the library allocates nothing and never synchronises inside
+    these calls; kernel timing must be off).
     After every replay settle() checks the overflow counter (synchronous), as
     for BatchedLap.
     """
@@ -107,13 +108,19 @@ class GraphLap:
         self.w = world
         if lap is None:
             lap = BatchedLap(world, filter_passes)
-            if world.mbox is None or world.jstage is None:
-                world.ensure_mailbox()
-        elif not isinstance(lap, ShockLap) or lap.tiles is not None:
-            raise ValueError("only BatchedLap and the batched ShockLap are captured")
+        elif not isinstance(lap, ShockLap) or not lap.fused:
+            raise ValueError("only BatchedLap and the fused ShockLap are captured (the separate particle "
+                             "methods check their counters synchronously)")
         elif any(float(wall.betawall) != 0.0 for wall in lap.walls):
             raise ValueError("a moving wall changes the reflect launch every lap; replay it eagerly")
+        elif lap.laps_run == 0:
+            raise ValueError("run one eager lap of the ShockLap before capturing it")
         self.lap = lap
+        if world.mbox is None or world.jstage is None:
+            world.ensure_mailbox()
+        self._capture()
+
+    def _capture(self):
         torch.cuda.synchronize()
         self.graphs = []
         side = torch.cuda.Stream()
@@ -127,25 +134,40 @@ class GraphLap:
         torch.cuda.current_stream().wait_stream(side)
         torch.cuda.synchronize()
         self.k = 0
+        self.version = self.w.layout_version
 
     def __call__(self):
         self.graphs[self.k].replay()
         self.k = (self.k + 1) % len(self.graphs)
 
     def settle(self):
+        """After a repack (or a grown mailbox) the graphs hold replaced
+        buffers: they are captured again from the current state.  J between
+        laps is scratch (every lap rebuilds it), so recapturing after an odd
+        number of replays changes nothing but which buffer the next lap's J
+        starts in."""
         if self.w.pending_overflow():
             self.w.settle_overflow()
+        if self.w.layout_version != self.version:
+            del self.graphs
+            self._capture()
 
 
 class ShockLap:
-    """walls: list of rk_reflector_wall; bcs: list of rk_edge_bc (all tiles)."""
+    """walls: list of rk_reflector_wall; bcs: list of rk_edge_bc (all tiles).
 
-    def __init__(self, world, walls, bcs, filter_passes=3, tiles=None):
+    fused (batched form, Boris pusher): push, reflect, pack, comm
+    pic_particle and deposit run as the fused particle step with the walls
+    inside it; otherwise as separate launches."""
+
+    def __init__(self, world, walls, bcs, filter_passes=3, tiles=None, fused=True):
         self.w = world
         self.walls = list(walls)
         self.bcs = list(bcs)
         self.filter_passes = filter_passes
         self.tiles = tiles  # None: one launch per method over every tile
+        self.fused = fused and tiles is None and world.pusher == "boris"
+        self.laps_run = 0
 
     def _targets(self):
         return [-1] if self.tiles is None else range(self.w.ntiles)
@@ -175,11 +197,14 @@ class ShockLap:
         self._each("rk_push_half_b")
         self._bcs(comm_mode.emf_B)
         w.comm_local(int(comm_mode.emf_B))
-        self._each("rk_push_particles")
-        self._reflect()
-        self._each("rk_pack_outgoing_particles")
-        w.comm_local(int(comm_mode.pic_particle))
-        self._each("rk_deposit_current")
+        if self.fused:
+            w.particle_step_fused(self.walls)
+        else:
+            self._each("rk_push_particles")
+            self._reflect()
+            self._each("rk_pack_outgoing_particles")
+            w.comm_local(int(comm_mode.pic_particle))
+            self._each("rk_deposit_current")
         w.comm_local(int(comm_mode.emf_J_exchange))
         w.comm_local(int(comm_mode.emf_J))
         self._bcs(comm_mode.emf_J)
@@ -199,6 +224,7 @@ class ShockLap:
         cfl = np.float32(w.cfl)
         for wall in self.walls:
             wall.walloc = float(np.float32(wall.walloc) + np.float32(wall.betawall) * cfl)
+        self.laps_run += 1
         if self.tiles is not None:
             torch.cuda.synchronize()
 
@@ -209,4 +235,4 @@ class ShockLap:
 
 class PlainShockLap(ShockLap):
     def __init__(self, world, walls, bcs, filter_passes=3):
-        super().__init__(world, walls, bcs, filter_passes, tiles="all")
+        super().__init__(world, walls, bcs, filter_passes, tiles="all", fused=False)

@@ -106,6 +106,9 @@ class World:
                                             L.p3(self.F[3])))
         self.nbricks = L.lib().rk_world_num_bricks(self.h)
         self.bpt = self.nbricks // (self.nt[0] * self.nt[1] * self.nt[2])
+        # bumped whenever a buffer the kernels take as an argument is replaced
+        # (captured HIP graphs hold the old one)
+        self.layout_version = 0
         self.count = [torch.zeros((self.nbricks,), dtype=torch.int32, device=self.dev) for _ in range(self.nsp)]
         self.pools = [_Pool(self.nbricks, np.zeros(self.nbricks, dtype=np.int64), self.dev) for _ in range(self.nsp)]
         for s in range(self.nsp):
@@ -142,6 +145,7 @@ class World:
             self.h = None
 
     def _bind_species(self, s):
+        self.layout_version += 1
         p = self.pools[s]
         L.check(L.lib().rk_world_set_species(self.h, s, p.capacity, L.p3(p.pos), L.p3(p.vel), p.id.data_ptr(),
                                              p.start.data_ptr(), p.cap.data_ptr(), self.count[s].data_ptr()))
@@ -160,6 +164,7 @@ class World:
                                                    rec["id"].data_ptr(), rec["meta"].data_ptr()))
             self.mig[st] = rec
             self._mig_cap[st] = c
+            self.layout_version += 1
 
     def ensure_mailbox(self, fraction=0.12):
         """Per-brick mailboxes for the fused particle step, sized to `fraction`
@@ -170,6 +175,7 @@ class World:
             self.jstage = torch.empty((self.nbricks, stride), dtype=torch.float32, device=self.dev)
             torch.cuda.synchronize()
             L.check(L.lib().rk_world_set_current_staging(self.h, self.jstage.data_ptr(), stride))
+            self.layout_version += 1
         caps = sum(p.cap.to(torch.int64) for p in self.pools)
         per = int(math.ceil(fraction * int(caps.max().item()))) if self.nsp else 0
         per = max(64, (per + 31) // 32 * 32)
@@ -185,6 +191,7 @@ class World:
                                              rec["meta"].data_ptr()))
         self.mbox = rec
         self._mbox_per_brick = per
+        self.layout_version += 1
 
     @property
     def ntiles(self):
@@ -231,10 +238,16 @@ class World:
             return
         L.check(rc)
 
-    def particle_step_fused(self):
+    def particle_step_fused(self, walls=None):
+        """walls: optional sequence of rk_reflector_wall structs, applied to
+        the tiles they lie in between the push and the classification."""
         if self._mbox_per_brick == 0:
             self.ensure_mailbox()
-        L.check(L.lib().rk_particle_step_fused(self.h, _stream()))
+        if walls:
+            arr = (L.ReflectorWall * len(walls))(*walls)
+            L.check(L.lib().rk_particle_step_fused_walls(self.h, arr, len(walls), _stream()))
+        else:
+            L.check(L.lib().rk_particle_step_fused(self.h, _stream()))
 
     def pending_overflow(self):
         n = ctypes.c_int64(0)
@@ -258,6 +271,10 @@ class World:
             L.check(L.lib().rk_retry_append(self.h, _stream()))
         if self.pending_overflow():
             raise RuntimeError("runko_amd: particle bins still overflowing after repack")
+        # the mailboxes follow the grown bins only now: the pending records
+        # lived in them (and in the overflow list) until the retry above
+        if self._mbox_per_brick:
+            self.ensure_mailbox()
 
     def _repack(self, s, newcaps):
         newp = _Pool(self.nbricks, newcaps, self.dev)
@@ -265,8 +282,7 @@ class World:
                                   newp.start.data_ptr(), newp.cap.data_ptr(), _stream()))
         torch.cuda.synchronize()
         self.pools[s] = newp
-        if self._mbox_per_brick:
-            self.ensure_mailbox()
+        self.layout_version += 1
 
     def presize(self, s, demand):
         """Lay out species s with bins sized for `demand` particles per bin

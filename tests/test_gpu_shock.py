@@ -1,8 +1,11 @@
-"""The reflecting-wall shock as a workload: the world-level shock lap
-(runko_amd.stepper.ShockLap, the lap of projects/pic-shock/pic.py:250-299
-issued as one launch per method over every tile) against the Simulation's
-shock lap, whose tile-by-tile form the wall tests compare with the oracle;
-and the on-GPU shock set-up (cold flow along -x, density ramp at the wall)."""
+"""The reflecting-wall shock as a workload: the reflect inside the fused
+particle step against the oracle (particles bit-exact, including the parked
+ones it removes), the world-level shock lap (runko_amd.stepper.ShockLap,
+the lap of projects/pic-shock/pic.py:250-299 issued as one launch per
+method over every tile, fused or not, eager or replayed from HIP graphs)
+against the Simulation's shock lap, whose tile-by-tile form the wall tests
+compare with the oracle; and the on-GPU shock set-up (cold flow along -x,
+density ramp at the wall)."""
 
 import math
 
@@ -14,9 +17,9 @@ import runko_amd as runko
 from runko_amd import _lib as L
 from runko_amd import stepper
 from runko_amd.init import shock_parameters, shock_world
-from gpu_common import gpu_fields, gpu_particles
+from gpu_common import build_gpu, build_oracle, gpu_fields, gpu_particles
 from helpers import rel_err
-from test_gpu_wall import shock_lap
+from test_gpu_wall import M, Q, assert_particles_equal, shock_lap, wall_inputs
 from test_gpu_wall import shock_world as wall_test_world
 
 pytestmark = pytest.mark.gpu
@@ -34,8 +37,52 @@ def world_particles(w, s):
     return sts[o], ids[o]
 
 
-@pytest.mark.parametrize("beta", [0.0, 0.2])
-def test_stepper_shock_lap_matches_simulation(beta):
+@pytest.mark.parametrize("beta", [0.0, 0.25])
+def test_fused_reflect_parity_with_oracle(orc, beta):
+    # particles start on both sides of the wall: in the fused step some
+    # reflect, some are parked and removed, the rest are untouched
+    nt, n = (2, 2, 1), (8, 8, 8)
+    walloc = 5.25
+    inputs = wall_inputs(nt, n, seed=21)
+    conf, grid, sim = build_gpu(nt, n, Q, M, inputs, execution="batched")
+    w = build_oracle(orc, nt, n, Q, M, inputs)
+    gam = 1.0 / np.sqrt(1.0 - beta**2)
+    wall = runko.pic.threeD.reflector_wall(walloc=walloc, betawall=beta, gammawall=gam)
+    for tile in grid.tiles():
+        tile.register_reflector_wall(wall)
+    cm = runko.tools.comm_mode
+
+    def lap(x):
+        x.prtcl_push()
+        x.prtcl_reflect_particles()
+        x.prtcl_pack_outgoing()
+        x.comm_local(cm.pic_particle)
+        x.prtcl_deposit_current()
+        x.comm_local(cm.emf_J_exchange)
+        x.comm_local(cm.emf_J)
+
+    L.check(L.lib().rk_profile_enable(grid.world.h, 1))
+    sim.for_one_lap(lap)
+    ms = np.zeros(9)
+    cnt = np.zeros(9, dtype=np.int64)
+    L.check(L.lib().rk_profile_read(grid.world.h, ms.ctypes.data, cnt.ctypes.data))
+    L.check(L.lib().rk_profile_enable(grid.world.h, 0))
+    assert cnt[0] == 1 and cnt[8] == 1  # one fused launch (and its J fill); no separate reflect
+    w.push_particles()
+    w.reflect_particles(-1, [(wall.walloc, wall.betawall, wall.gammawall)])
+    w.pack_outgoing()
+    w.comm_local("pic_particle")
+    w.deposit_current()
+    w.comm_local("emf_J_exchange")
+    w.comm_local("emf_J")
+    n_before = sum(len(st) for _, parts in inputs.values() for st in parts)
+    assert grid.world.total_particles() < n_before  # some were parked
+    assert_particles_equal(grid, w)
+    assert rel_err(gpu_fields(grid)[6:9], w.global_fields()[6:9]) < 1e-6
+
+
+@pytest.mark.parametrize("beta,fused", [(0.0, True), (0.2, True), (0.2, False)])
+def test_stepper_shock_lap_matches_simulation(beta, fused):
     nt, n = (3, 2, 1), (8, 8, 8)
     walloc = 4.0
     res = []
@@ -48,7 +95,8 @@ def test_stepper_shock_lap_matches_simulation(beta):
             wall_now = tile._walls[0].walloc
         else:
             walls = [L.ReflectorWall(wl.walloc, wl.betawall, wl.gammawall) for wl in tile._walls]
-            lap = stepper.ShockLap(grid.world, walls, [bc._struct() for bc in tile._edge_bcs])
+            lap = stepper.ShockLap(grid.world, walls, [bc._struct() for bc in tile._edge_bcs], fused=fused)
+            assert lap.fused == fused
             for _ in range(5):
                 lap()
                 lap.settle()
@@ -103,12 +151,17 @@ def test_shock_world_initial_state():
     assert torch.all(E[1][inner] == np.float32(Ey)) and torch.all(B[2][inner] == np.float32(B0))
 
 
-def test_shock_world_batched_matches_plain_and_keeps_particles():
+@pytest.mark.parametrize("form", ["fused", "graph"])
+def test_shock_world_batched_matches_plain_and_keeps_particles(form):
     runs = []
-    for cls in (stepper.ShockLap, stepper.PlainShockLap):
+    for kind in (form, "plain"):
         w, walls, bcs, par = small_shock(seed=7)
         n0 = w.total_particles()
-        lap = cls(w, walls, bcs)
+        lap = stepper.PlainShockLap(w, walls, bcs) if kind == "plain" else stepper.ShockLap(w, walls, bcs)
+        lap()
+        lap.settle()
+        if kind == "graph":
+            lap = stepper.GraphLap(w, lap=lap)
         for _ in range(4):
             lap()
             lap.settle()
