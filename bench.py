@@ -215,17 +215,20 @@ def run_shock(args):
     """Reflecting-wall shock: nx x ny x nz cells in tile^3 tiles, gamma 10 cold
     flow along -x, 8 ppc/species upstream, density ramped 4x toward the wall
     (512 cells), theta 1e-4, sigma 0.1 with B perpendicular to the flow.  The
-    pic-shock lap (edge conditions, reflect, the rest of the PIC step) in
-    the batched form (one launch per method over every tile) and in the plain
-    per-tile form (one call per tile per method, each method synchronous),
-    on the same world, back to back."""
+    pic-shock lap (edge conditions, reflect, the rest of the PIC step), on
+    the same world, back to back, in four forms: replayed from captured HIP
+    graphs (fused particle step with the wall inside), the same eagerly, the
+    batched form with the particle methods as separate launches, and the
+    plain per-tile form (one call per tile per method, each method
+    synchronous).  Bins that fill up are repacked inside the timed region
+    (counted in `repacks`)."""
     import gc
 
     import numpy as np
     import torch
 
     from runko_amd.init import shock_world
-    from runko_amd.stepper import PlainShockLap, ShockLap
+    from runko_amd.stepper import GraphLap, PlainShockLap, ShockLap
 
     gc.collect()
     torch.cuda.empty_cache()
@@ -244,13 +247,20 @@ def run_shock(args):
                         f"sigma 0.1 perpendicular B, conducting + upstream edge conditions, FP32",
                particles=npart, tiles=w.ntiles, tile_particles_min=int(occupied.min()),
                tile_particles_max=int(occupied.max()), tiles_empty=int((per_tile == 0).sum()))
-    runs = [("batched", ShockLap(w, walls, bcs), args.shock_steps)]
+    fused = ShockLap(w, walls, bcs)
+    fused()
+    fused.settle()
+    runs = [("graph", lambda: GraphLap(w, lap=fused), args.shock_steps),
+            ("batched_fused", lambda: fused, args.shock_steps),
+            ("batched_unfused", lambda: ShockLap(w, walls, bcs, fused=False), max(1, args.shock_steps // 4))]
     if args.plain_steps > 0:
-        runs.append(("plain_per_tile", PlainShockLap(w, walls, bcs), max(1, args.shock_plain_steps)))
-    for name, lap, steps in runs:
+        runs.append(("plain_per_tile", lambda: PlainShockLap(w, walls, bcs), max(1, args.shock_plain_steps)))
+    for name, make, steps in runs:
+        lap = make()
         lap()
         lap.settle()
         torch.cuda.synchronize()
+        r0 = w.repacks
         t1 = time.perf_counter()
         for i in range(steps):
             lap()
@@ -258,12 +268,16 @@ def run_shock(args):
         torch.cuda.synchronize()
         dt = time.perf_counter() - t1
         assert w.total_particles() == npart, "particle count changed (shock)"
-        res[name] = dict(steps=steps, ms_per_step=1e3 * dt / steps, particle_steps_per_s=npart * steps / dt)
-        log(f"[bench] shock {name}: {1e3 * dt / steps:.2f} ms/step, {npart * steps / dt:.3e} p-s/s")
+        res[name] = dict(steps=steps, ms_per_step=1e3 * dt / steps, particle_steps_per_s=npart * steps / dt,
+                         repacks=w.repacks - r0)
+        log(f"[bench] shock {name}: {1e3 * dt / steps:.2f} ms/step, {npart * steps / dt:.3e} p-s/s, "
+            f"{w.repacks - r0} repacks")
+        del lap
     if "plain_per_tile" in res:
-        res["batched_vs_plain"] = res["plain_per_tile"]["ms_per_step"] / res["batched"]["ms_per_step"]
+        for name in ("graph", "batched_fused", "batched_unfused"):
+            res[f"{name}_vs_plain"] = res["plain_per_tile"]["ms_per_step"] / res[name]["ms_per_step"]
     res["error_counters"] = [int(v) for v in w.error_counters()]
-    del runs, w
+    del runs, fused, w
     gc.collect()
     torch.cuda.empty_cache()
     return res
@@ -373,7 +387,7 @@ def main():
     ap.add_argument("--only-shock", action="store_true",
                     help="run only the shock workload and print its result object")
     ap.add_argument("--shock-cells", type=int, nargs=3, default=[2048, 128, 128], metavar=("NX", "NY", "NZ"))
-    ap.add_argument("--shock-steps", type=int, default=20)
+    ap.add_argument("--shock-steps", type=int, default=40)
     ap.add_argument("--shock-plain-steps", type=int, default=3)
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="also run plain copies of MIB MiB (scale for rocprof byte counters)")

@@ -109,6 +109,7 @@ class World:
         # bumped whenever a buffer the kernels take as an argument is replaced
         # (captured HIP graphs hold the old one)
         self.layout_version = 0
+        self.repacks = 0
         self.count = [torch.zeros((self.nbricks,), dtype=torch.int32, device=self.dev) for _ in range(self.nsp)]
         self.pools = [_Pool(self.nbricks, np.zeros(self.nbricks, dtype=np.int64), self.dev) for _ in range(self.nsp)]
         for s in range(self.nsp):
@@ -256,25 +257,27 @@ class World:
 
     def settle_overflow(self):
         """Repack every species whose bins overflowed, then re-append the
-        pending records.  Synchronous."""
+        pending records.  Synchronous.  A full bin grows to its demand plus
+        slack and at least 1.5x, so a bin that keeps filling (a compressed
+        layer) is repacked a logarithmic number of times."""
         for _ in range(4):
             if self.pending_overflow() == 0:
-                return
+                break
             for s in range(self.nsp):
                 demand = self.count[s].cpu().numpy().view(np.uint32).astype(np.int64)
                 caps = self.pools[s].cap.cpu().numpy().view(np.uint32).astype(np.int64)
-                if np.all(demand <= caps):
+                full = demand > caps
+                if not np.any(full):
                     continue
-                newcaps = np.maximum(caps, np.ceil(demand * SLACK).astype(np.int64) + PAD)
+                grown = np.maximum(np.ceil(demand * SLACK).astype(np.int64) + PAD, (3 * caps + 1) // 2)
+                newcaps = np.where(full, grown, caps)
                 newcaps = (newcaps + 31) // 32 * 32
                 self._repack(s, newcaps)
             L.check(L.lib().rk_retry_append(self.h, _stream()))
         if self.pending_overflow():
             raise RuntimeError("runko_amd: particle bins still overflowing after repack")
-        # the mailboxes follow the grown bins only now: the pending records
-        # lived in them (and in the overflow list) until the retry above
-        if self._mbox_per_brick:
-            self.ensure_mailbox()
+        # the mailboxes keep their size (leavers past it spill into the shared
+        # record sets); replacing them here would orphan pending records
 
     def _repack(self, s, newcaps):
         newp = _Pool(self.nbricks, newcaps, self.dev)
@@ -283,6 +286,7 @@ class World:
         torch.cuda.synchronize()
         self.pools[s] = newp
         self.layout_version += 1
+        self.repacks += 1
 
     def presize(self, s, demand):
         """Lay out species s with bins sized for `demand` particles per bin
