@@ -157,14 +157,16 @@ __device__ inline int brick_in_tile(const Geo& g, int tile, const float origo[3]
   return tile * g.bpt + (lb[0] * g.nb[1] + lb[1]) * g.nb[2] + lb[2];
 }
 
-// XCD-aware block -> brick map: consecutive bricks (which share halo lines in
-// L2) go to the same XCD (blocks are dealt round-robin over 8 XCDs).  The grid
-// has 8*ceil(n/8) blocks; blocks mapped past n exit.
-__device__ inline int xcd_brick(int blk, int nblk)
+// XCD-aware block -> brick map.  Blocks are dealt round-robin over the 8 XCDs;
+// XCD x takes the groups x, x + 8, x + 16, ... of `span` consecutive bricks, so
+// bricks that share halo lines in L2 stay on one XCD while every XCD gets an
+// even share of a world whose density varies along x (span = whole tiles, an
+// eighth of a tile column).  span = ceil(n/8) gives each XCD one contiguous
+// eighth.  The grid is xcd_grid(n, span); blocks mapped past n exit.
+__device__ inline int xcd_brick(int blk, int span)
 {
-  const int per = (nblk + 7) / 8;
-  const int b = (blk % 8) * per + blk / 8;
-  return b;
+  const int x = blk % 8, j = blk / 8;
+  return ((j / span) * 8 + x) * span + j % span;
 }
 
 // ---------------------------------------------------------------------------
@@ -857,10 +859,10 @@ __device__ inline void faraday(const Geo& g, float qm, const float E[3], const f
 // ---------------------------------------------------------------------------
 template<int PUSHER>
 __global__ void __launch_bounds__(WG)
-  k_push(Geo g, SpParams sp, FieldPtrs f, int bin0, int nbins, uint32_t* counters)
+  k_push(Geo g, SpParams sp, FieldPtrs f, int bin0, int nbins, int span, uint32_t* counters)
 {
   extern __shared__ float lds[];
-  const int b = bin0 + xcd_brick(int(blockIdx.x), nbins);
+  const int b = bin0 + xcd_brick(int(blockIdx.x), span);
   if(b >= bin0 + nbins) return;
   if(bin_total(sp, b) == 0) return;
   const Brick Bk = brick_of(g, b);
@@ -907,11 +909,11 @@ __global__ void __launch_bounds__(WG)
 }
 
 __global__ void __launch_bounds__(WG)
-  k_deposit(Geo g, SpParams sp, FieldPtrs f, int bin0, int nbins, uint32_t* counters)
+  k_deposit(Geo g, SpParams sp, FieldPtrs f, int bin0, int nbins, int span, uint32_t* counters)
 {
   extern __shared__ double ldsd[];
   double* lds = ldsd;
-  const int b = bin0 + xcd_brick(int(blockIdx.x), nbins);
+  const int b = bin0 + xcd_brick(int(blockIdx.x), span);
   if(b >= bin0 + nbins) return;
   if(bin_total(sp, b) == 0) return;
   const Brick Bk = brick_of(g, b);
@@ -1368,13 +1370,13 @@ struct NbrTab {
 // without a record and deposit nothing.
 template<int BR, int FAST, int WALL = 0>
 __global__ void __launch_bounds__(FWG) __attribute__((amdgpu_waves_per_eu(4)))
-  k_fused(Geo g, SpParams sp, FieldPtrs f, MigPtrs m, const MigPtrs* mtab, int nbins, uint32_t* counters,
+  k_fused(Geo g, SpParams sp, FieldPtrs f, MigPtrs m, const MigPtrs* mtab, int nbins, int span, uint32_t* counters,
           float* jstage, int64_t jstage_stride, WallSet walls, float* Jc0, float* Jc1, float* Jc2)
 {
   extern __shared__ double lds_dyn[];
   __shared__ int wcnt[2 * (FWG / 64)];
   __shared__ NbrTab NT;
-  const int b = xcd_brick(int(blockIdx.x), nbins);
+  const int b = xcd_brick(int(blockIdx.x), span);
   if(b >= nbins) return;
   const Shape<BR> S(g);
   const Brick Bk = brick_of(g, b);
@@ -1788,8 +1790,23 @@ void bin_range(const rk_world* w, int tile, int& b0, int& nb)
   }
 }
 
-// grid for xcd_brick(): 8 * ceil(n/8) blocks so the permutation covers [0, n)
-unsigned xcd_grid(int n) { return unsigned((n + 7) / 8 * 8); }
+// span of xcd_brick() for a launch over bricks [b0, b0 + n): groups of whole
+// tiles, an eighth of a tile column (tiles sharing an x index) per group, when
+// the launch covers the world; else one contiguous eighth per XCD
+int xcd_span(const Geo& g, int n)
+{
+  const int eighth = (n + 7) / 8;
+  if(n != g.nbricks) return std::max(eighth, 1);
+  const int tiles = std::max(1, g.nt[1] * g.nt[2] / 8);
+  return std::max(1, std::min(tiles * g.bpt, eighth));
+}
+
+// grid for xcd_brick(): every group of every XCD, so the map covers [0, n)
+unsigned xcd_grid(int n, int span)
+{
+  const int groups = (n + span - 1) / span;
+  return unsigned((groups + 7) / 8 * 8 * span);
+}
 
 // Compare the fast forms of x/cfl, cfl/sqrt(X), 2/t and 1/sqrt(t) with the
 // IEEE operations over every float of their fast ranges; mismatches summed.
@@ -1849,8 +1866,9 @@ int launch_push(rk_world* w, int tile, hipStream_t s)
   int b0, nb;
   bin_range(w, tile, b0, nb);
   auto* kern = w->g.pusher == 1 ? k_push<1> : (w->g.pusher == 2 ? k_push<2> : k_push<0>);
-  hipLaunchKernelGGL(kern, dim3(xcd_grid(nb)), dim3(WG), means_lds(w->g), s, w->g, make_sp(w), make_fields(w), b0,
-                     nb, w->counters);
+  const int span = xcd_span(w->g, nb);
+  hipLaunchKernelGGL(kern, dim3(xcd_grid(nb, span)), dim3(WG), means_lds(w->g), s, w->g, make_sp(w), make_fields(w),
+                     b0, nb, span, w->counters);
   RK_HIP(hipGetLastError());
   return RK_OK;
 }
@@ -1859,8 +1877,9 @@ int launch_deposit(rk_world* w, int tile, hipStream_t s)
 {
   int b0, nb;
   bin_range(w, tile, b0, nb);
-  hipLaunchKernelGGL(k_deposit, dim3(xcd_grid(nb)), dim3(WG), cur_lds(w->g), s, w->g, make_sp(w), make_fields(w),
-                     b0, nb, w->counters);
+  const int span = xcd_span(w->g, nb);
+  hipLaunchKernelGGL(k_deposit, dim3(xcd_grid(nb, span)), dim3(WG), cur_lds(w->g), s, w->g, make_sp(w),
+                     make_fields(w), b0, nb, span, w->counters);
   RK_HIP(hipGetLastError());
   return RK_OK;
 }
@@ -1903,12 +1922,13 @@ int launch_fused(rk_world* w, hipStream_t s, const WallSet* walls)
 {
   const Geo& g = w->g;
   const size_t dyn = means_lds(g) + cur_lds(g);
-  const dim3 grid(xcd_grid(g.nbricks));
+  const int span = xcd_span(g, g.nbricks);
+  const dim3 grid(xcd_grid(g.nbricks, span));
   WallSet ws {};
   if(walls) ws = *walls;
   auto go = [&](auto kern) {
     hipLaunchKernelGGL(kern, grid, dim3(FWG), dyn, s, g, make_sp(w), make_fields(w), make_mig(w),
-                       static_cast<const MigPtrs*>(w->mig_table), g.nbricks, w->counters, w->jstage,
+                       static_cast<const MigPtrs*>(w->mig_table), g.nbricks, span, w->counters, w->jstage,
                        w->jstage_stride, ws, w->corrJ[0], w->corrJ[1], w->corrJ[2]);
   };
   const bool cube8 = g.br[0] == 8 && g.br[1] == 8 && g.br[2] == 8;

@@ -243,7 +243,16 @@ def fill_shock_plasma(world, ppc, theta, gamma_flow, walloc, ramp_len, ramp_fact
         total = int(per_cell.sum().item())
         if total == 0:
             continue
-        base = torch.repeat_interleave(cells, per_cell, dim=0).to(torch.float64)
+        # interleaved slot order inside every bin: slot p holds the particle
+        # of rank p // (cells per brick) of cell p mod (cells per brick) where
+        # the cell has one, so the lanes of a wave sit in different cells
+        cpb = cl.shape[0]
+        ci = torch.repeat_interleave(torch.arange(cells.shape[0], device=dev), per_cell)
+        cfirst = torch.cumsum(per_cell, 0) - per_cell
+        rank = torch.arange(total, device=dev) - cfirst[ci]
+        key = (ci // cpb) * (int(per_cell.max().item()) * cpb) + rank * cpb + ci % cpb
+        ci = ci[torch.argsort(key)]
+        base = cells[ci].to(torch.float64)
         pos = base + torch.rand((total, 3), generator=gen, device=dev, dtype=torch.float64)
         pos[:, 0] = torch.clamp(pos[:, 0], min=walloc)
         # keep each float32 position inside its cell (and so inside its brick)
