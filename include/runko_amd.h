@@ -316,6 +316,16 @@ typedef struct rk_diff_state {
 typedef struct rk_diff rk_diff;
 
 int rk_diff_create(rk_diff** out, const rk_diff_desc* desc);
+
+/* An ensemble of independent periodic boxes stepped together: every kernel of
+ * a step covers all boxes in one launch.  No counterpart in the reference.
+ * Boxes may differ in sides, species, counts, charges and masses; precision,
+ * cfl and filter passes are common.  A state holds the boxes one after
+ * another: the fields of box i start at cell sum_{j<i} n0 n1 n2 of box j (each
+ * box [n0][n1][n2] inside), its particles at sum_{j<i} particles of box j.
+ * rk_diff_step / rk_diff_step_adjoint / rk_diff_error_count / rk_diff_destroy
+ * take the handle as for one box (one box = an ensemble of one). */
+int rk_diff_create_ensemble(rk_diff** out, const rk_diff_desc* descs, int32_t n_boxes);
 int rk_diff_destroy(rk_diff* d);
 
 /* One step: out = step(in).  out may alias in. */

@@ -124,6 +124,7 @@ def lib():
     L.rk_gauss_residual.argtypes = [vp, i32, vp, vp]
     L.rk_error_counters.argtypes = [vp, vp, vp]
     L.rk_diff_create.argtypes = [ctypes.POINTER(vp), ctypes.POINTER(DiffDesc)]
+    L.rk_diff_create_ensemble.argtypes = [ctypes.POINTER(vp), ctypes.POINTER(DiffDesc), i32]
     L.rk_diff_destroy.argtypes = [vp]
     L.rk_diff_step.argtypes = [vp, ctypes.POINTER(DiffState), ctypes.POINTER(DiffState), vp]
     L.rk_diff_step_adjoint.argtypes = [vp, ctypes.POINTER(DiffState), ctypes.POINTER(DiffState),
@@ -157,7 +158,7 @@ EXPORTED = [
     "rk_repack", "rk_retry_append", "rk_tile_particle_count", "rk_tile_particles", "rk_outgoing",
     "rk_energies", "rk_gauss_residual", "rk_error_counters", "rk_push_e_add_current", "rk_profile_enable",
     "rk_profile_read", "rk_world_set_mailbox", "rk_world_set_current_staging", "rk_reflect_particles",
-    "rk_apply_edge_bc", "rk_diff_create", "rk_diff_destroy", "rk_diff_step", "rk_diff_step_adjoint",
+    "rk_apply_edge_bc", "rk_diff_create", "rk_diff_create_ensemble", "rk_diff_destroy", "rk_diff_step", "rk_diff_step_adjoint",
     "rk_diff_error_count", "rk_snapshot_fields", "rk_snapshot_spectra", "rk_interpolate_at",
     "rk_world_set_schemes",
 ]

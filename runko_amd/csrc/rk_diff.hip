@@ -58,6 +58,23 @@ struct G3 {
   T* c[3];
 };
 
+// one box of an ensemble: its geometry and where its cells and particles
+// start in the concatenated arrays (a single box is an ensemble of one)
+template<class T>
+struct DEnt {
+  DBox<T> b;
+  int64_t c0, p0;
+};
+
+template<class T>
+__device__ inline G3<T> sh(G3<T> g, int64_t o)
+{
+#pragma unroll
+  for(int c = 0; c < 3; ++c)
+    if(g.c[c]) g.c[c] += o;
+  return g;
+}
+
 __host__ __device__ inline int64_t cidx(const int n[3], int i, int j, int k)
 {
   return (int64_t(i) * n[1] + j) * n[2] + k;
@@ -110,8 +127,14 @@ __device__ inline void atomic_add(T* p, T v)
 // out = in + dt * curlE(E)   (push_b_fdtd2 order; out may alias in)
 template<class T>
 __global__ void __launch_bounds__(DWG)
-  k_dcurl_b(DBox<T> b, G3<T> out, G3<T> in, G3<T> E, T dt)
+  k_dcurl_b(const DEnt<T>* __restrict__ tab, G3<T> out, G3<T> in, G3<T> E, T dt)
 {
+  const DEnt<T> en = tab[blockIdx.z];
+  const DBox<T>& b = en.b;
+  if(int(blockIdx.x) >= b.n[1] || int(blockIdx.y) >= b.n[0]) return;
+  out = sh(out, en.c0);
+  in = sh(in, en.c0);
+  E = sh(E, en.c0);
   const int j = int(blockIdx.x), i = int(blockIdx.y);
   const int n0 = b.n[0], n1 = b.n[1], n2 = b.n[2];
   const int ip = wrap1(i + 1, n0), jp = wrap1(j + 1, n1);
@@ -134,8 +157,15 @@ __global__ void __launch_bounds__(DWG)
 // add_current order; out may alias in, not B)
 template<class T>
 __global__ void __launch_bounds__(DWG)
-  k_dcurl_e(DBox<T> b, G3<T> out, G3<T> in, G3<T> B, T dt, G3<T> J)
+  k_dcurl_e(const DEnt<T>* __restrict__ tab, G3<T> out, G3<T> in, G3<T> B, T dt, G3<T> J)
 {
+  const DEnt<T> en = tab[blockIdx.z];
+  const DBox<T>& b = en.b;
+  if(int(blockIdx.x) >= b.n[1] || int(blockIdx.y) >= b.n[0]) return;
+  out = sh(out, en.c0);
+  in = sh(in, en.c0);
+  B = sh(B, en.c0);
+  J = sh(J, en.c0);
   const int j = int(blockIdx.x), i = int(blockIdx.y);
   const int n0 = b.n[0], n1 = b.n[1], n2 = b.n[2];
   const int im = wrap1(i - 1, n0), jm = wrap1(j - 1, n1);
@@ -164,8 +194,13 @@ __global__ void __launch_bounds__(DWG)
 
 // binomial2, 27 taps in layout_right order (yee_lattice_current_filter_binomial2.c++:23-76)
 template<class T>
-__global__ void __launch_bounds__(DWG) k_dfilter(DBox<T> b, G3<T> out, G3<T> in)
+__global__ void __launch_bounds__(DWG) k_dfilter(const DEnt<T>* __restrict__ tab, G3<T> out, G3<T> in)
 {
+  const DEnt<T> en = tab[blockIdx.z];
+  const DBox<T>& b = en.b;
+  if(int(blockIdx.x) >= b.n[1] || int(blockIdx.y) >= b.n[0]) return;
+  out = sh(out, en.c0);
+  in = sh(in, en.c0);
   const int j = int(blockIdx.x), i = int(blockIdx.y);
   const int n0 = b.n[0], n1 = b.n[1], n2 = b.n[2];
   const T w1 = T(1. / 64.), w2 = T(2. / 64.), w4 = T(4. / 64.), w8 = T(8. / 64.);
@@ -205,8 +240,15 @@ __global__ void __launch_bounds__(DWG) k_dfill(T* p, int64_t n, T v)
 // + Bx[N-y-z]) / 4, so Bx[n] receives the mean of MB_x over n, n+y, n+z, n+y+z.
 // M8 holds per node ME_x, ME_y, ME_z, MB_x, MB_y, MB_z (and two pad slots).
 template<class T>
-__global__ void __launch_bounds__(DWG) k_nodemean_adj(DBox<T> b, G3<T> LE, G3<T> LB, const T* __restrict__ M8)
+__global__ void __launch_bounds__(DWG)
+  k_nodemean_adj(const DEnt<T>* __restrict__ tab, G3<T> LE, G3<T> LB, const T* __restrict__ M8)
 {
+  const DEnt<T> en = tab[blockIdx.z];
+  const DBox<T>& b = en.b;
+  if(int(blockIdx.x) >= b.n[1] || int(blockIdx.y) >= b.n[0]) return;
+  LE = sh(LE, en.c0);
+  LB = sh(LB, en.c0);
+  M8 += 8 * en.c0;
   const int j = int(blockIdx.x), i = int(blockIdx.y);
   const int ip = wrap1(i + 1, b.n[0]), jp = wrap1(j + 1, b.n[1]);
   for(int k = int(threadIdx.x); k < b.n[2]; k += int(blockDim.x)) {
@@ -278,8 +320,14 @@ __device__ inline bool in_box(const DBox<T>& b, const T x[3])
 // Bx[N-y-z]))) / 4: the expressions of the per-corner computation, so the
 // particles read the same values they would compute.
 template<class T>
-__global__ void __launch_bounds__(DWG) k_node_means(DBox<T> b, G3<T> E, G3<T> B, T* __restrict__ NM8)
+__global__ void __launch_bounds__(DWG) k_node_means(const DEnt<T>* __restrict__ tab, G3<T> E, G3<T> B, T* __restrict__ NM8)
 {
+  const DEnt<T> en = tab[blockIdx.z];
+  const DBox<T>& b = en.b;
+  if(int(blockIdx.x) >= b.n[1] || int(blockIdx.y) >= b.n[0]) return;
+  E = sh(E, en.c0);
+  B = sh(B, en.c0);
+  NM8 += 8 * en.c0;
   const int j = int(blockIdx.x), i = int(blockIdx.y);
   const int im = wrap1(i - 1, b.n[0]), jm = wrap1(j - 1, b.n[1]);
   for(int k = int(threadIdx.x); k < b.n[2]; k += int(blockDim.x)) {
@@ -494,9 +542,18 @@ __device__ inline int species_of(const DBox<T>& b, int64_t p)
 
 template<class T>
 __global__ void __launch_bounds__(DWG)
-  k_dpush(DBox<T> b, const T* __restrict__ NM8, G3<T> xin, G3<T> uin, G3<T> xout, G3<T> uout,
+  k_dpush(const DEnt<T>* __restrict__ tab, const T* __restrict__ NM8, G3<T> xin, G3<T> uin, G3<T> xout, G3<T> uout,
           uint32_t* __restrict__ count, uint32_t* __restrict__ slot2, unsigned long long* err)
 {
+  const DEnt<T> en = tab[blockIdx.z];
+  const DBox<T>& b = en.b;
+  NM8 += 8 * en.c0;
+  xin = sh(xin, en.p0);
+  uin = sh(uin, en.p0);
+  xout = sh(xout, en.p0);
+  uout = sh(uout, en.p0);
+  count += en.c0;
+  slot2 += 2 * en.p0;
   const int64_t np = b.off[b.nsp];
   for(int64_t p = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; p < np; p += int64_t(gridDim.x) * blockDim.x) {
     const int s = species_of(b, p);
@@ -563,9 +620,15 @@ __constant__ const int P12[12][2] = {{0, 0}, {0, 1}, {0, 2}, {1, 1}, {1, 2}, {2,
 // pushed state (the zigzag geometry is recomputed from x'', u')
 template<class T>
 __global__ void __launch_bounds__(DWG)
-  k_dep_records(DBox<T> b, G3<T> xout, G3<T> uout, const uint32_t* __restrict__ cstart,
+  k_dep_records(const DEnt<T>* __restrict__ tab, G3<T> xout, G3<T> uout, const uint32_t* __restrict__ cstart,
                 const uint32_t* __restrict__ slot2, T* __restrict__ rec)
 {
+  const DEnt<T> en = tab[blockIdx.z];
+  const DBox<T>& b = en.b;
+  xout = sh(xout, en.p0);
+  uout = sh(uout, en.p0);
+  cstart += en.c0;
+  slot2 += 2 * en.p0;
   const int64_t np = b.off[b.nsp];
   for(int64_t p = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; p < np; p += int64_t(gridDim.x) * blockDim.x) {
     const uint32_t s1 = slot2[2 * p], s2 = slot2[2 * p + 1];
@@ -594,9 +657,15 @@ __global__ void __launch_bounds__(DWG)
 // interleaved current at the cell's 7 zigzag nodes
 template<class T>
 __global__ void __launch_bounds__(DWG)
-  k_cell_deposit(DBox<T> b, const T* __restrict__ rec, const uint32_t* __restrict__ cstart,
-                 const uint32_t* __restrict__ count, T* J4, int64_t ncell)
+  k_cell_deposit(const DEnt<T>* __restrict__ tab, const T* __restrict__ rec, const uint32_t* __restrict__ cstart,
+                 const uint32_t* __restrict__ count, T* J4)
 {
+  const DEnt<T> en = tab[blockIdx.z];
+  const DBox<T>& b = en.b;
+  const int64_t ncell = int64_t(b.n[0]) * b.n[1] * b.n[2];
+  cstart += en.c0;
+  count += en.c0;
+  J4 += 4 * en.c0;
   for(int64_t c = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; c < ncell; c += int64_t(gridDim.x) * blockDim.x) {
     const uint32_t cnt = count[c];
     if(cnt == 0) continue;
@@ -665,10 +734,22 @@ __device__ inline void cross_adj(const T a[3], const T b[3], const T gc[3], T ga
 
 template<class T>
 __global__ void __launch_bounds__(DWG)
-  k_dpush_adj(DBox<T> b, const T* __restrict__ NM8, G3<T> xin, G3<T> uin, const T* L4, T jsign, G3<T> lx_out,
-              G3<T> lu_out, G3<T> lx_in, G3<T> lu_in, const uint32_t* __restrict__ cstart,
+  k_dpush_adj(const DEnt<T>* __restrict__ tab, const T* __restrict__ NM8, G3<T> xin, G3<T> uin, const T* L4, T jsign,
+              G3<T> lx_out, G3<T> lu_out, G3<T> lx_in, G3<T> lu_in, const uint32_t* __restrict__ cstart,
               const uint32_t* __restrict__ pslot, T* __restrict__ rec)
 {
+  const DEnt<T> en = tab[blockIdx.z];
+  const DBox<T>& b = en.b;
+  NM8 += 8 * en.c0;
+  xin = sh(xin, en.p0);
+  uin = sh(uin, en.p0);
+  L4 += 4 * en.c0;
+  lx_out = sh(lx_out, en.p0);
+  lu_out = sh(lu_out, en.p0);
+  lx_in = sh(lx_in, en.p0);
+  lu_in = sh(lu_in, en.p0);
+  cstart += en.c0;
+  pslot += en.p0;
   const int64_t np = b.off[b.nsp];
   const T cfl = b.cfl;
   // every lane of a wave runs every iteration (the scatter below reduces
@@ -826,8 +907,13 @@ __global__ void __launch_bounds__(DWG)
 // bucket slots: particle p of base cell c gets slot pslot[p] in c's bucket
 template<class T>
 __global__ void __launch_bounds__(DWG)
-  k_cell_count(DBox<T> b, G3<T> xin, uint32_t* __restrict__ count, uint32_t* __restrict__ pslot)
+  k_cell_count(const DEnt<T>* __restrict__ tab, G3<T> xin, uint32_t* __restrict__ count, uint32_t* __restrict__ pslot)
 {
+  const DEnt<T> en = tab[blockIdx.z];
+  const DBox<T>& b = en.b;
+  xin = sh(xin, en.p0);
+  count += en.c0;
+  pslot += en.p0;
   const int64_t np = b.off[b.nsp];
   for(int64_t p = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; p < np; p += int64_t(gridDim.x) * blockDim.x) {
     const T x[3] = {xin.c[0][p], xin.c[1][p], xin.c[2][p]};
@@ -849,9 +935,15 @@ __global__ void __launch_bounds__(DWG)
 // node-mean adjoints (48 atomics per occupied cell instead of per particle)
 template<class T>
 __global__ void __launch_bounds__(DWG)
-  k_cell_scatter(DBox<T> b, const T* __restrict__ rec, const uint32_t* __restrict__ cstart,
-                 const uint32_t* __restrict__ count, T* M8, int64_t ncell)
+  k_cell_scatter(const DEnt<T>* __restrict__ tab, const T* __restrict__ rec, const uint32_t* __restrict__ cstart,
+                 const uint32_t* __restrict__ count, T* M8)
 {
+  const DEnt<T> en = tab[blockIdx.z];
+  const DBox<T>& b = en.b;
+  const int64_t ncell = int64_t(b.n[0]) * b.n[1] * b.n[2];
+  cstart += en.c0;
+  count += en.c0;
+  M8 += 8 * en.c0;
   for(int64_t c = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; c < ncell; c += int64_t(gridDim.x) * blockDim.x) {
     const uint32_t cnt = count[c];
     if(cnt == 0) continue;
@@ -898,8 +990,13 @@ __global__ void __launch_bounds__(DWG)
 
 // ---------------------------------------------------------------- host side
 struct DiffImpl {
-  rk_diff_desc d;
-  int64_t ncell = 0, np = 0;
+  rk_diff_desc d;                    // the first box (precision, cfl, filter passes are common)
+  std::vector<rk_diff_desc> boxes;   // every box of the ensemble
+  std::vector<int64_t> c0, p0;       // first cell / particle of each box in the concatenated arrays
+  int nmax[3] = {0, 0, 0};           // largest side per axis (the field kernels' grid)
+  int64_t npmax = 0, ncmax = 0;      // largest box's particles and cells
+  void* tab = nullptr;               // device DEnt<T>[boxes]
+  int64_t ncell = 0, np = 0;         // totals
   size_t vb = 4;
   void* scratch[5][3] = {};  // B1, J, Jt, LB2, LE (planar)
   void* j4 = nullptr;        // [cell][4] current (deposit) / its adjoint (gather)
@@ -919,20 +1016,36 @@ namespace {
 enum { S_B1 = 0, S_J, S_JT, S_LB2, S_LE };
 
 template<class T>
-DBox<T> make_box(const DiffImpl& h)
+DBox<T> make_box(const rk_diff_desc& d)
 {
   DBox<T> b {};
-  for(int a = 0; a < 3; ++a) b.n[a] = h.d.n[a];
-  b.nsp = h.d.n_species;
+  for(int a = 0; a < 3; ++a) b.n[a] = d.n[a];
+  b.nsp = d.n_species;
   b.off[0] = 0;
   for(int s = 0; s < b.nsp; ++s) {
-    b.off[s + 1] = b.off[s] + h.d.count[s];
-    b.q[s] = static_cast<T>(h.d.charge[s]);
-    const double sg = (0.0 < h.d.charge[s]) - (h.d.charge[s] < 0.0);
-    b.qm[s] = static_cast<T>(sg / h.d.mass[s]);
+    b.off[s + 1] = b.off[s] + d.count[s];
+    b.q[s] = static_cast<T>(d.charge[s]);
+    const double sg = (0.0 < d.charge[s]) - (d.charge[s] < 0.0);
+    b.qm[s] = static_cast<T>(sg / d.mass[s]);
   }
-  b.cfl = static_cast<T>(h.d.cfl);
+  b.cfl = static_cast<T>(d.cfl);
   return b;
+}
+
+template<class T>
+hipError_t upload_table(DiffImpl& h)
+{
+  std::vector<DEnt<T>> t(h.boxes.size());
+  for(size_t i = 0; i < t.size(); ++i) t[i] = DEnt<T> {make_box<T>(h.boxes[i]), h.c0[i], h.p0[i]};
+  hipError_t e = hipMalloc(&h.tab, sizeof(DEnt<T>) * t.size());
+  if(e == hipSuccess) e = hipMemcpy(h.tab, t.data(), sizeof(DEnt<T>) * t.size(), hipMemcpyHostToDevice);
+  return e;
+}
+
+template<class T>
+const DEnt<T>* table(const DiffImpl& h)
+{
+  return static_cast<const DEnt<T>*>(h.tab);
 }
 
 template<class T>
@@ -963,21 +1076,32 @@ int fill_n(T* p, int64_t n, hipStream_t st)
   return RK_OK;
 }
 
+// flat passes over every cell of every box
 inline unsigned cblocks(const DiffImpl& h)
 {
   return unsigned(std::max<int64_t>(1, std::min<int64_t>((h.ncell + DWG - 1) / DWG, 4096)));
 }
 
-inline dim3 grid_of(const DiffImpl& h) { return dim3(unsigned(h.d.n[1]), unsigned(h.d.n[0])); }
-inline dim3 block_of(const DiffImpl& h) { return dim3(unsigned(std::min(h.d.n[2], DWG))); }
-inline unsigned pblocks(const DiffImpl& h)
+// per-box launches: blockIdx.z is the box; x (and y) are sized for the largest box
+inline dim3 grid_of(const DiffImpl& h)
 {
-  return unsigned(std::max<int64_t>(1, std::min<int64_t>((h.np + DWG - 1) / DWG, int64_t(1) << 20)));
+  return dim3(unsigned(h.nmax[1]), unsigned(h.nmax[0]), unsigned(h.boxes.size()));
+}
+inline dim3 block_of(const DiffImpl& h) { return dim3(unsigned(std::min(h.nmax[2], DWG))); }
+inline dim3 pgrid(const DiffImpl& h)
+{
+  const int64_t per = std::max<int64_t>(1, std::min<int64_t>((h.npmax + DWG - 1) / DWG, int64_t(1) << 20));
+  return dim3(unsigned(per), 1, unsigned(h.boxes.size()));
+}
+inline dim3 cgrid(const DiffImpl& h)
+{
+  const int64_t per = std::max<int64_t>(1, std::min<int64_t>((h.ncmax + DWG - 1) / DWG, 4096));
+  return dim3(unsigned(per), 1, unsigned(h.boxes.size()));
 }
 
 // Jf = F^P(J) in the J / Jt pair; returns the scratch slot holding the result
 template<class T>
-int filter_passes(const DiffImpl& h, const DBox<T>& b, int src, hipStream_t st, int& result)
+int filter_passes(const DiffImpl& h, const DEnt<T>* b, int src, hipStream_t st, int& result)
 {
   int a = src, o = src == S_J ? S_JT : S_J;
   for(int pass = 0; pass < h.d.filter_passes; ++pass) {
@@ -992,7 +1116,7 @@ int filter_passes(const DiffImpl& h, const DBox<T>& b, int src, hipStream_t st, 
 template<class T>
 int step_t(DiffImpl& h, const rk_diff_state* in, rk_diff_state* out, hipStream_t st)
 {
-  const DBox<T> b = make_box<T>(h);
+  const DEnt<T>* b = table<T>(h);
   const T half = static_cast<T>(h.d.cfl / 2), full = static_cast<T>(h.d.cfl);
   const G3<T> none {{nullptr, nullptr, nullptr}};
   const G3<T> E = g3<T>(in->E), B = g3<T>(in->B);
@@ -1007,17 +1131,16 @@ int step_t(DiffImpl& h, const rk_diff_state* in, rk_diff_state* out, hipStream_t
   if(h.np > 0) {
     // push, then the current through cell buckets: slots, offsets, records, per-cell sums
     RK_HIP(hipMemsetAsync(h.ccount, 0, sizeof(uint32_t) * size_t(h.ncell), st));
-    hipLaunchKernelGGL(k_dpush<T>, dim3(pblocks(h)), dim3(DWG), 0, st, b, NM8, g3<T>(in->x), g3<T>(in->u),
+    hipLaunchKernelGGL(k_dpush<T>, pgrid(h), dim3(DWG), 0, st, b, NM8, g3<T>(in->x), g3<T>(in->u),
                        g3<T>(out->x), g3<T>(out->u), h.ccount, h.pslot, h.err);
     RK_HIP(hipGetLastError());
     size_t bytes = h.scan_bytes;
     RK_HIP(hipcub::DeviceScan::ExclusiveSum(h.scan_tmp, bytes, h.ccount, h.cstart, int(h.ncell), st));
     T* rec = static_cast<T*>(h.rec);
-    hipLaunchKernelGGL(k_dep_records<T>, dim3(pblocks(h)), dim3(DWG), 0, st, b, g3<T>(out->x), g3<T>(out->u),
+    hipLaunchKernelGGL(k_dep_records<T>, pgrid(h), dim3(DWG), 0, st, b, g3<T>(out->x), g3<T>(out->u),
                        h.cstart, h.pslot, rec);
     RK_HIP(hipGetLastError());
-    hipLaunchKernelGGL(k_cell_deposit<T>, dim3(cblocks(h)), dim3(DWG), 0, st, b, rec, h.cstart, h.ccount, J4,
-                       h.ncell);
+    hipLaunchKernelGGL(k_cell_deposit<T>, cgrid(h), dim3(DWG), 0, st, b, rec, h.cstart, h.ccount, J4);
     RK_HIP(hipGetLastError());
   }
   hipLaunchKernelGGL(k_j4_split<T>, dim3(cblocks(h)), dim3(DWG), 0, st, J4, g3s<T>(h, S_J), h.ncell);
@@ -1038,7 +1161,7 @@ int step_t(DiffImpl& h, const rk_diff_state* in, rk_diff_state* out, hipStream_t
 template<class T>
 int adjoint_t(DiffImpl& h, const rk_diff_state* in, const rk_diff_state* ao, rk_diff_state* ai, hipStream_t st)
 {
-  const DBox<T> b = make_box<T>(h);
+  const DEnt<T>* b = table<T>(h);
   const T half = static_cast<T>(h.d.cfl / 2), full = static_cast<T>(h.d.cfl);
   const G3<T> none {{nullptr, nullptr, nullptr}};
   const G3<T> E = g3<T>(in->E), B = g3<T>(in->B);
@@ -1069,17 +1192,16 @@ int adjoint_t(DiffImpl& h, const rk_diff_state* in, const rk_diff_state* ao, rk_
   if(h.np > 0) {
     // bucket the particles by base cell: counts, offsets, then the records
     RK_HIP(hipMemsetAsync(h.ccount, 0, sizeof(uint32_t) * size_t(h.ncell), st));
-    hipLaunchKernelGGL(k_cell_count<T>, dim3(pblocks(h)), dim3(DWG), 0, st, b, g3<T>(in->x), h.ccount, h.pslot);
+    hipLaunchKernelGGL(k_cell_count<T>, pgrid(h), dim3(DWG), 0, st, b, g3<T>(in->x), h.ccount, h.pslot);
     RK_HIP(hipGetLastError());
     size_t bytes = h.scan_bytes;
     RK_HIP(hipcub::DeviceScan::ExclusiveSum(h.scan_tmp, bytes, h.ccount, h.cstart, int(h.ncell), st));
     T* rec = static_cast<T*>(h.rec);
-    hipLaunchKernelGGL(k_dpush_adj<T>, dim3(pblocks(h)), dim3(DWG), 0, st, b, NM8, g3<T>(in->x),
+    hipLaunchKernelGGL(k_dpush_adj<T>, pgrid(h), dim3(DWG), 0, st, b, NM8, g3<T>(in->x),
                        g3<T>(in->u), L4, T(-1), g3<T>(ao->x), g3<T>(ao->u), g3<T>(ai->x), g3<T>(ai->u), h.cstart,
                        h.pslot, rec);
     RK_HIP(hipGetLastError());
-    hipLaunchKernelGGL(k_cell_scatter<T>, dim3(cblocks(h)), dim3(DWG), 0, st, b, rec, h.cstart, h.ccount, M8,
-                       h.ncell);
+    hipLaunchKernelGGL(k_cell_scatter<T>, cgrid(h), dim3(DWG), 0, st, b, rec, h.cstart, h.ccount, M8);
     RK_HIP(hipGetLastError());
   }
   // node-mean adjoints: lambda_E += NM_E^T(ME), lambda_B1 = LB2 + NM_B^T(MB)
@@ -1114,28 +1236,49 @@ struct rk_diff : rk::DiffImpl {};
 
 extern "C" {
 
-int rk_diff_create(rk_diff** out, const rk_diff_desc* desc)
+int rk_diff_create_ensemble(rk_diff** out, const rk_diff_desc* descs, int32_t n_boxes)
 {
-  if(!out || !desc) return dfail(RK_EINVAL, "null argument");
+  if(!out || !descs) return dfail(RK_EINVAL, "null argument");
   *out = nullptr;
-  const rk_diff_desc& d = *desc;
-  for(int a = 0; a < 3; ++a)
-    if(d.n[a] < 2 || d.n[a] > 4096) return dfail(RK_EINVAL, "diff box: every side must be 2..4096 cells");
-  if(d.n_species < 0 || d.n_species > RK_MAX_SPECIES) return dfail(RK_EINVAL, "diff box: invalid species count");
-  if(d.precision != 4 && d.precision != 8) return dfail(RK_EINVAL, "diff box: precision must be 4 or 8");
-  if(!(d.cfl > 0.0) || d.filter_passes < 0) return dfail(RK_EINVAL, "diff box: invalid cfl or filter passes");
+  if(n_boxes < 1 || n_boxes > 65535) return dfail(RK_EINVAL, "diff ensemble: 1..65535 boxes");
+  const rk_diff_desc& d0 = descs[0];
+  if(d0.precision != 4 && d0.precision != 8) return dfail(RK_EINVAL, "diff box: precision must be 4 or 8");
+  if(!(d0.cfl > 0.0) || d0.filter_passes < 0) return dfail(RK_EINVAL, "diff box: invalid cfl or filter passes");
   auto* h = new rk_diff();
-  h->d = d;
-  h->vb = size_t(d.precision);
-  h->ncell = int64_t(d.n[0]) * d.n[1] * d.n[2];
-  for(int s = 0; s < d.n_species; ++s) {
-    if(d.count[s] < 0 || !(d.mass[s] > 0.0)) {
-      delete h;
-      return dfail(RK_EINVAL, "diff box: invalid particle count or mass");
+  h->d = d0;
+  h->vb = size_t(d0.precision);
+  for(int i = 0; i < n_boxes; ++i) {
+    const rk_diff_desc& d = descs[i];
+    const char* bad = nullptr;
+    for(int a = 0; a < 3; ++a)
+      if(d.n[a] < 2 || d.n[a] > 4096) bad = "diff box: every side must be 2..4096 cells";
+    if(d.n_species < 0 || d.n_species > RK_MAX_SPECIES) bad = "diff box: invalid species count";
+    if(d.precision != d0.precision || d.cfl != d0.cfl || d.filter_passes != d0.filter_passes)
+      bad = "diff ensemble: every box needs the same precision, cfl and filter passes";
+    int64_t np = 0;
+    for(int sp = 0; sp < d.n_species && !bad; ++sp) {
+      if(d.count[sp] < 0 || !(d.mass[sp] > 0.0)) bad = "diff box: invalid particle count or mass";
+      np += d.count[sp];
     }
-    h->np += d.count[s];
+    if(bad) {
+      delete h;
+      return dfail(RK_EINVAL, bad);
+    }
+    const int64_t nc = int64_t(d.n[0]) * d.n[1] * d.n[2];
+    h->boxes.push_back(d);
+    h->c0.push_back(h->ncell);
+    h->p0.push_back(h->np);
+    h->ncell += nc;
+    h->np += np;
+    h->ncmax = std::max(h->ncmax, nc);
+    h->npmax = std::max(h->npmax, np);
+    for(int a = 0; a < 3; ++a) h->nmax[a] = std::max(h->nmax[a], d.n[a]);
   }
-  hipError_t e = hipSuccess;
+  if(h->ncell >= (int64_t(1) << 31) || h->np >= (int64_t(1) << 31)) {
+    delete h;
+    return dfail(RK_EINVAL, "diff ensemble: cells and particles must each total below 2^31");
+  }
+  hipError_t e = h->vb == 4 ? upload_table<float>(*h) : upload_table<double>(*h);
   for(int i = 0; i < 5 && e == hipSuccess; ++i)
     for(int c = 0; c < 3 && e == hipSuccess; ++c) e = hipMalloc(&h->scratch[i][c], h->vb * size_t(h->ncell));
   if(e == hipSuccess) e = hipMalloc(&h->j4, 4 * h->vb * size_t(h->ncell));
@@ -1159,6 +1302,12 @@ int rk_diff_create(rk_diff** out, const rk_diff_desc* desc)
   return RK_OK;
 }
 
+int rk_diff_create(rk_diff** out, const rk_diff_desc* desc)
+{
+  if(!desc) return dfail(RK_EINVAL, "null argument");
+  return rk_diff_create_ensemble(out, desc, 1);
+}
+
 int rk_diff_destroy(rk_diff* h)
 {
   if(!h) return RK_OK;
@@ -1172,6 +1321,7 @@ int rk_diff_destroy(rk_diff* h)
                  h->scan_tmp})
     if(q) (void)hipFree(q);
   if(h->err) (void)hipFree(h->err);
+  if(h->tab) (void)hipFree(h->tab);
   delete h;
   return RK_OK;
 }

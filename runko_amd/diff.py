@@ -65,6 +65,7 @@ class DiffBox:
         h = ctypes.c_void_p()
         L.check(L.lib().rk_diff_create(ctypes.byref(h), ctypes.byref(d)))
         self.h = h
+        self._fshape = (3,) + self.n
 
     def __del__(self):
         h = getattr(self, "h", None)
@@ -77,14 +78,14 @@ class DiffBox:
 
     # ------------------------------------------------------------------ tensors
     def empty_state(self):
-        f = (3,) + self.n
+        f = self._fshape
         return (torch.empty(f, dtype=self.dtype, device=self.dev), torch.empty(f, dtype=self.dtype, device=self.dev),
                 torch.empty((3, self.np), dtype=self.dtype, device=self.dev),
                 torch.empty((3, self.np), dtype=self.dtype, device=self.dev))
 
     def _check(self, state):
         E, B, x, u = state
-        for t, shape in ((E, (3,) + self.n), (B, (3,) + self.n), (x, (3, self.np)), (u, (3, self.np))):
+        for t, shape in ((E, self._fshape), (B, self._fshape), (x, (3, self.np)), (u, (3, self.np))):
             if tuple(t.shape) != shape or t.dtype != self.dtype or t.device != self.dev or not t.is_contiguous():
                 raise ValueError(f"DiffBox: expected a contiguous {self.dtype} tensor of shape {shape} on "
                                  f"{self.dev}, got {tuple(t.shape)} {t.dtype} {t.device}")
@@ -124,6 +125,71 @@ class DiffBox:
         n = ctypes.c_int64(0)
         L.check(L.lib().rk_diff_error_count(self.h, ctypes.byref(n), _stream()))
         return n.value
+
+
+class DiffEnsemble(DiffBox):
+    """Independent periodic boxes stepped together: every kernel of a step
+    covers all boxes in one launch (rk_diff_create_ensemble).  boxes: a
+    sequence of dicts with n, charges, masses, counts; cfl, filter passes and
+    dtype are common.  States hold the boxes one after another: E, B are
+    [3, total cells] (box i's cells from c0[i], [n0][n1][n2] inside), x, u
+    are [3, total particles] (box i's from p0[i], its species consecutive);
+    field_view / particle_view give one box's part."""
+
+    def __init__(self, boxes, cfl=0.45, filter_passes=3, dtype=torch.float32, device="cuda"):
+        if dtype not in (torch.float32, torch.float64):
+            raise ValueError("DiffEnsemble: dtype must be float32 or float64")
+        boxes = list(boxes)
+        if not boxes:
+            raise ValueError("DiffEnsemble: at least one box")
+        self.ns, self.cells, self.nps, self.c0, self.p0 = [], [], [], [], []
+        descs = (L.DiffDesc * len(boxes))()
+        c0 = p0 = 0
+        for i, bx in enumerate(boxes):
+            n = tuple(int(v) for v in bx["n"])
+            charges, masses, counts = list(bx["charges"]), list(bx["masses"]), [int(c) for c in bx["counts"]]
+            if not (len(charges) == len(masses) == len(counts)) or len(charges) > L.RK_MAX_SPECIES:
+                raise ValueError("DiffEnsemble: charges, masses and counts must have one entry per species (at most 4)")
+            d = descs[i]
+            for a in range(3):
+                d.n[a] = n[a]
+            d.n_species = len(charges)
+            for sp in range(len(charges)):
+                d.count[sp] = counts[sp]
+                d.charge[sp] = float(charges[sp])
+                d.mass[sp] = float(masses[sp])
+            d.cfl = float(cfl)
+            d.filter_passes = int(filter_passes)
+            d.precision = 4 if dtype == torch.float32 else 8
+            self.ns.append(n)
+            self.cells.append(int(np.prod(n)))
+            self.nps.append(sum(counts))
+            self.c0.append(c0)
+            self.p0.append(p0)
+            c0 += self.cells[-1]
+            p0 += self.nps[-1]
+        self.ncell, self.np = c0, p0
+        self.cfl = float(cfl)
+        self.filter_passes = int(filter_passes)
+        self.dtype = dtype
+        dev = torch.device(device)
+        if dev.type == "cuda" and dev.index is None:
+            dev = torch.device("cuda", torch.cuda.current_device())
+        self.dev = dev
+        h = ctypes.c_void_p()
+        L.check(L.lib().rk_diff_create_ensemble(ctypes.byref(h), descs, len(boxes)))
+        self.h = h
+        self._fshape = (3, self.ncell)
+
+    @property
+    def n_boxes(self):
+        return len(self.ns)
+
+    def field_view(self, F, i):
+        return F[:, self.c0[i]:self.c0[i] + self.cells[i]].view((3,) + self.ns[i])
+
+    def particle_view(self, x, i):
+        return x[:, self.p0[i]:self.p0[i] + self.nps[i]]
 
 
 class _Rollout(torch.autograd.Function):
@@ -275,3 +341,76 @@ class PairPlasmaProblem:
         E, B, x, u = self.initial_state(amp, drift)
         E, B, x, u = rollout(self.box, E, B, x, u, steps, checkpoint_every)
         return band_magnetic_energy(B)
+
+
+class EnsembleProblem:
+    """The batched ensemble workload: n_boxes independent periodic pair
+    plasmas, each with a cubic side drawn from `sides`, ppc particles per
+    cell per species drawn from `ppcs`, theta log-uniform in theta_range,
+    sigma log-uniform in sigma_range (charges and guide field B0 along z of
+    projects/pic-turbulence/pic.py:44-68 with that box's ppc, theta, sigma),
+    and n_modes seeded dB modes (|k| <= kmax) with learnable amplitudes
+    [n_boxes, n_modes] in units of B0 / sqrt(n_modes).  The loss is the sum
+    over boxes of each box's final magnetic energy sum(B^2)/2: the boxes are
+    independent, so its gradient holds every box's own gradient."""
+
+    def __init__(self, n_boxes=64, sides=(32, 48, 64, 96), ppcs=(4, 8, 16), theta_range=(0.05, 3.0),
+                 sigma_range=(0.1, 10.0), n_modes=6, kmax=3, c_omp=10.0, cfl=0.45, seed=5, filter_passes=3,
+                 dtype=torch.float32, device="cuda"):
+        from .init import plasma_parameters, juttner_momenta
+        rng = np.random.default_rng(seed)
+        self.dtype = dtype
+        self.dev = torch.device(device)
+        self.n_modes = n_modes
+        self.params = []
+        boxes = []
+        for i in range(n_boxes):
+            side = int(rng.choice(sides))
+            ppc = int(rng.choice(ppcs))
+            theta = float(np.exp(rng.uniform(np.log(theta_range[0]), np.log(theta_range[1]))))
+            sigma = float(np.exp(rng.uniform(np.log(sigma_range[0]), np.log(sigma_range[1]))))
+            q0, q1, B0 = plasma_parameters(cfl, c_omp, ppc, theta, sigma)
+            npart = side ** 3 * ppc
+            self.params.append(dict(side=side, ppc=ppc, theta=theta, sigma=sigma, q0=q0, B0=B0,
+                                    table=mode_table(n_modes, kmax, seed=int(rng.integers(1 << 30)))))
+            boxes.append(dict(n=(side,) * 3, charges=[q0, q1], masses=[1.0, 1.0], counts=[npart, npart]))
+        self.box = DiffEnsemble(boxes, cfl=cfl, filter_passes=filter_passes, dtype=dtype, device=device)
+        gen = torch.Generator(device=self.dev)
+        gen.manual_seed(seed)
+        xs, us = [], []
+        for p in self.params:
+            side, ppc = p["side"], p["ppc"]
+            npart = side ** 3 * ppc
+            cells = torch.stack(torch.meshgrid(*(torch.arange(side, device=self.dev, dtype=torch.float64)
+                                                 for _ in range(3)), indexing="ij")).reshape(3, -1)
+            pos = cells.repeat_interleave(ppc, dim=1) + torch.rand((3, npart), generator=gen, device=self.dev,
+                                                                   dtype=torch.float64)
+            pos = pos.to(torch.float32)
+            xs.append(torch.cat([pos, pos], 1))
+            us.append(torch.cat([juttner_momenta(npart, p["theta"], gen, self.dev) for _ in range(2)], 1))
+            del cells, pos
+        self.x0 = torch.cat(xs, 1).to(dtype).contiguous()
+        self.u0 = torch.cat(us, 1).to(dtype).contiguous()
+        del xs, us
+
+    def initial_state(self, amp):
+        """(E, B, x, u) at t = 0 from the mode amplitudes [n_boxes, n_modes]."""
+        Bs = []
+        for i, p in enumerate(self.params):
+            n = (p["side"],) * 3
+            scale = p["B0"] / math.sqrt(self.n_modes)
+            Bs.append(initial_B(amp[i] * scale, p["table"], n, B0=(0.0, 0.0, p["B0"]),
+                                dtype=self.dtype).reshape(3, -1))
+        B = torch.cat(Bs, 1).contiguous()
+        E = torch.zeros_like(B)
+        return E, B, self.x0, self.u0
+
+    def box_magnetic_energies(self, B):
+        """[n_boxes] float64: sum(B^2)/2 of every box."""
+        return torch.stack([0.5 * (self.box.field_view(B, i).to(torch.float64) ** 2).sum()
+                            for i in range(self.box.n_boxes)])
+
+    def loss(self, amp, steps, checkpoint_every=10):
+        E, B, x, u = self.initial_state(amp)
+        E, B, x, u = rollout(self.box, E, B, x, u, steps, checkpoint_every)
+        return self.box_magnetic_energies(B).sum()
