@@ -283,6 +283,96 @@ def run_shock(args):
     return res
 
 
+def run_ensemble(args):
+    """Batched ensemble + gradients: n_boxes independent periodic pair plasmas
+    (sides from {32, 48, 64, 96}, ppc/species from {4, 8, 16}, theta
+    log-uniform in [0.05, 3], sigma log-uniform in [0.1, 10], 6 seeded dB
+    modes each with learnable amplitudes), stepped as one ensemble: every
+    kernel of a step covers all boxes in one launch.  Loss = sum of the boxes'
+    final magnetic energies; forward rollout + adjoint with checkpoints.  The
+    step kernels are also timed box by box (one launch set per box) for the
+    ensemble-vs-per-box ratio."""
+    import gc
+
+    import torch
+
+    from runko_amd.diff import DiffBox, EnsembleProblem, rollout
+
+    gc.collect()
+    torch.cuda.empty_cache()
+    steps, every = args.ensemble_steps, args.ensemble_every
+    t0 = time.perf_counter()
+    prob = EnsembleProblem(n_boxes=args.ensemble_boxes, seed=5)
+    box = prob.box
+    dev = prob.dev
+    nb = box.n_boxes
+    amp = torch.full((nb, prob.n_modes), 0.5, dtype=torch.float64, device=dev).requires_grad_(True)
+    prob.loss(amp, 2, 1).backward()  # warm-up
+    torch.cuda.synchronize()
+    log(f"[bench] ensemble: {nb} boxes, {box.np} particles, {box.ncell} cells, setup {time.perf_counter() - t0:.1f}s")
+    amp.grad = None
+    t0 = time.perf_counter()
+    E, B, x, u = prob.initial_state(amp)
+    out = rollout(box, E, B, x, u, steps, every)
+    loss = prob.box_magnetic_energies(out[1]).sum()
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    loss.backward()
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    fwd, bwd = t1 - t0, t2 - t1
+    log(f"[bench] ensemble: forward {fwd:.2f}s, backward {bwd:.2f}s for {steps} steps")
+    state = tuple(t.detach().contiguous() for t in out)
+    del out, loss, E, B, x, u
+    spare = box.empty_state()
+    lam = tuple(torch.ones_like(t) * 1e-3 for t in state)
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(3)]
+    k = 5
+    ev[0].record()
+    for _ in range(k):
+        box.step(state, spare)
+    ev[1].record()
+    for _ in range(k):
+        box.step_adjoint(state, lam, spare)
+    ev[2].record()
+    torch.cuda.synchronize()
+    f_ms, a_ms = ev[0].elapsed_time(ev[1]) / k, ev[1].elapsed_time(ev[2]) / k
+    del lam, spare
+    # the same boxes one at a time (each its own launches)
+    per_box_ms = 0.0
+    for i, p in enumerate(prob.params):
+        n = (p["side"],) * 3
+        cnt = box.nps[i] // 2
+        single = DiffBox(n, [p["q0"], -p["q0"]], [1.0, 1.0], [cnt, cnt], cfl=box.cfl,
+                         filter_passes=box.filter_passes, dtype=box.dtype)
+        st = (box.field_view(state[0], i).contiguous(), box.field_view(state[1], i).contiguous(),
+              box.particle_view(state[2], i).contiguous(), box.particle_view(state[3], i).contiguous())
+        sp = single.empty_state()
+        single.step(st, sp)
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        single.step(st, sp)
+        e1.record()
+        torch.cuda.synchronize()
+        per_box_ms += e0.elapsed_time(e1)
+        del single, st, sp
+    grad = amp.grad
+    res = dict(
+        workload=f"batched ensemble: {nb} periodic pair plasmas, sides from {{32,48,64,96}}, ppc/species from "
+                 f"{{4,8,16}}, theta log-uniform [0.05,3], sigma log-uniform [0.1,10], 6 seeded dB modes each; "
+                 f"loss = sum of final magnetic energies after {steps} steps, checkpoints every {every}, FP32",
+        boxes=nb, particles=box.np, cells=box.ncell, steps=steps, checkpoint_every=every,
+        forward_s=fwd, backward_s=bwd, adjoint_over_forward=bwd / fwd,
+        forward_ms_per_step=f_ms, adjoint_ms_per_step=a_ms, adjoint_step_over_forward_step=a_ms / f_ms,
+        forward_particle_steps_per_s=box.np * steps / fwd,
+        per_box_launches_forward_ms_per_step=per_box_ms, ensemble_vs_per_box=per_box_ms / f_ms,
+        grad_norm=float(grad.norm()), error_count=box.error_count())
+    del prob, box, state, amp
+    gc.collect()
+    torch.cuda.empty_cache()
+    return res
+
+
 def run_differentiation(args):
     """Differentiation workload: periodic pair plasma on a cells^3 box, ppc
     particles/cell/species, theta 0.3, initial dB from 12 seeded Fourier modes
@@ -389,6 +479,12 @@ def main():
     ap.add_argument("--shock-cells", type=int, nargs=3, default=[2048, 128, 128], metavar=("NX", "NY", "NZ"))
     ap.add_argument("--shock-steps", type=int, default=40)
     ap.add_argument("--shock-plain-steps", type=int, default=3)
+    ap.add_argument("--no-ensemble", action="store_true", help="skip the batched ensemble workload")
+    ap.add_argument("--only-ensemble", action="store_true",
+                    help="run only the ensemble workload and print its result object")
+    ap.add_argument("--ensemble-boxes", type=int, default=64)
+    ap.add_argument("--ensemble-steps", type=int, default=60)
+    ap.add_argument("--ensemble-every", type=int, default=8)
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="also run plain copies of MIB MiB (scale for rocprof byte counters)")
     args = ap.parse_args()
@@ -404,6 +500,9 @@ def main():
         return
     if args.only_shock:
         print(json.dumps(run_shock(args)), flush=True)
+        return
+    if args.only_ensemble:
+        print(json.dumps(run_ensemble(args)), flush=True)
         return
     if args.copy_probe:
         run_copy_probe(args.copy_probe)
@@ -437,6 +536,12 @@ def main():
         d = extra["differentiation"]
         log(f"[bench] adjoint/forward {d['adjoint_over_forward']:.2f} (rollout with recompute), "
             f"step kernels {d['forward_ms_per_step']:.2f} / {d['adjoint_ms_per_step']:.2f} ms")
+    if not args.no_ensemble:
+        extra["ensemble"] = run_ensemble(args)
+        d = extra["ensemble"]
+        log(f"[bench] ensemble adjoint/forward {d['adjoint_over_forward']:.2f}, step kernels "
+            f"{d['forward_ms_per_step']:.2f} / {d['adjoint_ms_per_step']:.2f} ms, "
+            f"{d['ensemble_vs_per_box']:.2f}x the per-box launches")
     cpu = None
     if not args.no_cpu_baseline:
         cpu = run_cpu_baseline(inputs, args.cpu_seconds)

@@ -64,6 +64,9 @@ template<class T>
 struct DEnt {
   DBox<T> b;
   int64_t c0, p0;
+  // particle and cell kernels run on flat grids: the box's blocks are
+  // [pb0, pb0 + pnb) and [cb0, cb0 + cnb), a per-block map gives the box
+  int64_t pb0, pnb, cb0, cnb;
 };
 
 template<class T>
@@ -542,11 +545,12 @@ __device__ inline int species_of(const DBox<T>& b, int64_t p)
 
 template<class T>
 __global__ void __launch_bounds__(DWG)
-  k_dpush(const DEnt<T>* __restrict__ tab, const T* __restrict__ NM8, G3<T> xin, G3<T> uin, G3<T> xout, G3<T> uout,
+  k_dpush(const DEnt<T>* __restrict__ tab, const uint16_t* __restrict__ bmap, const T* __restrict__ NM8, G3<T> xin, G3<T> uin, G3<T> xout, G3<T> uout,
           uint32_t* __restrict__ count, uint32_t* __restrict__ slot2, unsigned long long* err)
 {
-  const DEnt<T> en = tab[blockIdx.z];
+  const DEnt<T> en = tab[bmap[blockIdx.x]];
   const DBox<T>& b = en.b;
+  const int64_t blk = int64_t(blockIdx.x) - en.pb0, nblk = en.pnb;
   NM8 += 8 * en.c0;
   xin = sh(xin, en.p0);
   uin = sh(uin, en.p0);
@@ -555,7 +559,7 @@ __global__ void __launch_bounds__(DWG)
   count += en.c0;
   slot2 += 2 * en.p0;
   const int64_t np = b.off[b.nsp];
-  for(int64_t p = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; p < np; p += int64_t(gridDim.x) * blockDim.x) {
+  for(int64_t p = blk * blockDim.x + threadIdx.x; p < np; p += nblk * blockDim.x) {
     const int s = species_of(b, p);
     T x[3], u[3];
 #pragma unroll
@@ -620,17 +624,18 @@ __constant__ const int P12[12][2] = {{0, 0}, {0, 1}, {0, 2}, {1, 1}, {1, 2}, {2,
 // pushed state (the zigzag geometry is recomputed from x'', u')
 template<class T>
 __global__ void __launch_bounds__(DWG)
-  k_dep_records(const DEnt<T>* __restrict__ tab, G3<T> xout, G3<T> uout, const uint32_t* __restrict__ cstart,
+  k_dep_records(const DEnt<T>* __restrict__ tab, const uint16_t* __restrict__ bmap, G3<T> xout, G3<T> uout, const uint32_t* __restrict__ cstart,
                 const uint32_t* __restrict__ slot2, T* __restrict__ rec)
 {
-  const DEnt<T> en = tab[blockIdx.z];
+  const DEnt<T> en = tab[bmap[blockIdx.x]];
   const DBox<T>& b = en.b;
+  const int64_t blk = int64_t(blockIdx.x) - en.pb0, nblk = en.pnb;
   xout = sh(xout, en.p0);
   uout = sh(uout, en.p0);
   cstart += en.c0;
   slot2 += 2 * en.p0;
   const int64_t np = b.off[b.nsp];
-  for(int64_t p = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; p < np; p += int64_t(gridDim.x) * blockDim.x) {
+  for(int64_t p = blk * blockDim.x + threadIdx.x; p < np; p += nblk * blockDim.x) {
     const uint32_t s1 = slot2[2 * p], s2 = slot2[2 * p + 1];
     if(s1 == UINT32_MAX) continue;
     const int s = species_of(b, p);
@@ -657,16 +662,17 @@ __global__ void __launch_bounds__(DWG)
 // interleaved current at the cell's 7 zigzag nodes
 template<class T>
 __global__ void __launch_bounds__(DWG)
-  k_cell_deposit(const DEnt<T>* __restrict__ tab, const T* __restrict__ rec, const uint32_t* __restrict__ cstart,
+  k_cell_deposit(const DEnt<T>* __restrict__ tab, const uint16_t* __restrict__ bmap, const T* __restrict__ rec, const uint32_t* __restrict__ cstart,
                  const uint32_t* __restrict__ count, T* J4)
 {
-  const DEnt<T> en = tab[blockIdx.z];
+  const DEnt<T> en = tab[bmap[blockIdx.x]];
   const DBox<T>& b = en.b;
+  const int64_t blk = int64_t(blockIdx.x) - en.cb0, nblk = en.cnb;
   const int64_t ncell = int64_t(b.n[0]) * b.n[1] * b.n[2];
   cstart += en.c0;
   count += en.c0;
   J4 += 4 * en.c0;
-  for(int64_t c = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; c < ncell; c += int64_t(gridDim.x) * blockDim.x) {
+  for(int64_t c = blk * blockDim.x + threadIdx.x; c < ncell; c += nblk * blockDim.x) {
     const uint32_t cnt = count[c];
     if(cnt == 0) continue;
     const int64_t r0 = cstart[c];
@@ -734,12 +740,13 @@ __device__ inline void cross_adj(const T a[3], const T b[3], const T gc[3], T ga
 
 template<class T>
 __global__ void __launch_bounds__(DWG)
-  k_dpush_adj(const DEnt<T>* __restrict__ tab, const T* __restrict__ NM8, G3<T> xin, G3<T> uin, const T* L4, T jsign,
+  k_dpush_adj(const DEnt<T>* __restrict__ tab, const uint16_t* __restrict__ bmap, const T* __restrict__ NM8, G3<T> xin, G3<T> uin, const T* L4, T jsign,
               G3<T> lx_out, G3<T> lu_out, G3<T> lx_in, G3<T> lu_in, const uint32_t* __restrict__ cstart,
               const uint32_t* __restrict__ pslot, T* __restrict__ rec)
 {
-  const DEnt<T> en = tab[blockIdx.z];
+  const DEnt<T> en = tab[bmap[blockIdx.x]];
   const DBox<T>& b = en.b;
+  const int64_t blk = int64_t(blockIdx.x) - en.pb0, nblk = en.pnb;
   NM8 += 8 * en.c0;
   xin = sh(xin, en.p0);
   uin = sh(uin, en.p0);
@@ -755,7 +762,7 @@ __global__ void __launch_bounds__(DWG)
   // every lane of a wave runs every iteration (the scatter below reduces
   // across lanes); lanes past the end or outside the box compute on a dummy
   // particle and neither scatter nor store
-  for(int64_t base = int64_t(blockIdx.x) * blockDim.x; base < np; base += int64_t(gridDim.x) * blockDim.x) {
+  for(int64_t base = blk * blockDim.x; base < np; base += nblk * blockDim.x) {
     const int64_t p = base + int64_t(threadIdx.x);
     const bool live = p < np;
     const int64_t pl = live ? p : np - 1;
@@ -907,15 +914,16 @@ __global__ void __launch_bounds__(DWG)
 // bucket slots: particle p of base cell c gets slot pslot[p] in c's bucket
 template<class T>
 __global__ void __launch_bounds__(DWG)
-  k_cell_count(const DEnt<T>* __restrict__ tab, G3<T> xin, uint32_t* __restrict__ count, uint32_t* __restrict__ pslot)
+  k_cell_count(const DEnt<T>* __restrict__ tab, const uint16_t* __restrict__ bmap, G3<T> xin, uint32_t* __restrict__ count, uint32_t* __restrict__ pslot)
 {
-  const DEnt<T> en = tab[blockIdx.z];
+  const DEnt<T> en = tab[bmap[blockIdx.x]];
   const DBox<T>& b = en.b;
+  const int64_t blk = int64_t(blockIdx.x) - en.pb0, nblk = en.pnb;
   xin = sh(xin, en.p0);
   count += en.c0;
   pslot += en.p0;
   const int64_t np = b.off[b.nsp];
-  for(int64_t p = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; p < np; p += int64_t(gridDim.x) * blockDim.x) {
+  for(int64_t p = blk * blockDim.x + threadIdx.x; p < np; p += nblk * blockDim.x) {
     const T x[3] = {xin.c[0][p], xin.c[1][p], xin.c[2][p]};
     if(!in_box(b, x)) {
       pslot[p] = UINT32_MAX;
@@ -935,16 +943,17 @@ __global__ void __launch_bounds__(DWG)
 // node-mean adjoints (48 atomics per occupied cell instead of per particle)
 template<class T>
 __global__ void __launch_bounds__(DWG)
-  k_cell_scatter(const DEnt<T>* __restrict__ tab, const T* __restrict__ rec, const uint32_t* __restrict__ cstart,
+  k_cell_scatter(const DEnt<T>* __restrict__ tab, const uint16_t* __restrict__ bmap, const T* __restrict__ rec, const uint32_t* __restrict__ cstart,
                  const uint32_t* __restrict__ count, T* M8)
 {
-  const DEnt<T> en = tab[blockIdx.z];
+  const DEnt<T> en = tab[bmap[blockIdx.x]];
   const DBox<T>& b = en.b;
+  const int64_t blk = int64_t(blockIdx.x) - en.cb0, nblk = en.cnb;
   const int64_t ncell = int64_t(b.n[0]) * b.n[1] * b.n[2];
   cstart += en.c0;
   count += en.c0;
   M8 += 8 * en.c0;
-  for(int64_t c = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; c < ncell; c += int64_t(gridDim.x) * blockDim.x) {
+  for(int64_t c = blk * blockDim.x + threadIdx.x; c < ncell; c += nblk * blockDim.x) {
     const uint32_t cnt = count[c];
     if(cnt == 0) continue;
     const int64_t r0 = cstart[c];
@@ -996,6 +1005,10 @@ struct DiffImpl {
   int nmax[3] = {0, 0, 0};           // largest side per axis (the field kernels' grid)
   int64_t npmax = 0, ncmax = 0;      // largest box's particles and cells
   void* tab = nullptr;               // device DEnt<T>[boxes]
+  std::vector<int64_t> pb0, pnb, cb0, cnb;  // flat-grid block ranges per box (particles, cells)
+  int64_t pblocks = 0, cblocks_box = 0;     // flat-grid sizes
+  uint16_t* pmap = nullptr;                 // particle-kernel block -> box
+  uint16_t* cmap = nullptr;                 // cell-kernel block -> box
   int64_t ncell = 0, np = 0;         // totals
   size_t vb = 4;
   void* scratch[5][3] = {};  // B1, J, Jt, LB2, LE (planar)
@@ -1036,9 +1049,20 @@ template<class T>
 hipError_t upload_table(DiffImpl& h)
 {
   std::vector<DEnt<T>> t(h.boxes.size());
-  for(size_t i = 0; i < t.size(); ++i) t[i] = DEnt<T> {make_box<T>(h.boxes[i]), h.c0[i], h.p0[i]};
+  for(size_t i = 0; i < t.size(); ++i)
+    t[i] = DEnt<T> {make_box<T>(h.boxes[i]), h.c0[i], h.p0[i], h.pb0[i], h.pnb[i], h.cb0[i], h.cnb[i]};
   hipError_t e = hipMalloc(&h.tab, sizeof(DEnt<T>) * t.size());
   if(e == hipSuccess) e = hipMemcpy(h.tab, t.data(), sizeof(DEnt<T>) * t.size(), hipMemcpyHostToDevice);
+  // block -> box maps of the flat grids
+  std::vector<uint16_t> pm(size_t(std::max<int64_t>(h.pblocks, 1))), cm(size_t(std::max<int64_t>(h.cblocks_box, 1)));
+  for(size_t i = 0; i < t.size(); ++i) {
+    for(int64_t k = 0; k < h.pnb[i]; ++k) pm[size_t(h.pb0[i] + k)] = uint16_t(i);
+    for(int64_t k = 0; k < h.cnb[i]; ++k) cm[size_t(h.cb0[i] + k)] = uint16_t(i);
+  }
+  if(e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&h.pmap), sizeof(uint16_t) * pm.size());
+  if(e == hipSuccess) e = hipMemcpy(h.pmap, pm.data(), sizeof(uint16_t) * pm.size(), hipMemcpyHostToDevice);
+  if(e == hipSuccess) e = hipMalloc(reinterpret_cast<void**>(&h.cmap), sizeof(uint16_t) * cm.size());
+  if(e == hipSuccess) e = hipMemcpy(h.cmap, cm.data(), sizeof(uint16_t) * cm.size(), hipMemcpyHostToDevice);
   return e;
 }
 
@@ -1088,16 +1112,9 @@ inline dim3 grid_of(const DiffImpl& h)
   return dim3(unsigned(h.nmax[1]), unsigned(h.nmax[0]), unsigned(h.boxes.size()));
 }
 inline dim3 block_of(const DiffImpl& h) { return dim3(unsigned(std::min(h.nmax[2], DWG))); }
-inline dim3 pgrid(const DiffImpl& h)
-{
-  const int64_t per = std::max<int64_t>(1, std::min<int64_t>((h.npmax + DWG - 1) / DWG, int64_t(1) << 20));
-  return dim3(unsigned(per), 1, unsigned(h.boxes.size()));
-}
-inline dim3 cgrid(const DiffImpl& h)
-{
-  const int64_t per = std::max<int64_t>(1, std::min<int64_t>((h.ncmax + DWG - 1) / DWG, 4096));
-  return dim3(unsigned(per), 1, unsigned(h.boxes.size()));
-}
+// flat grids over every box's particle / cell blocks (block -> box by map)
+inline dim3 pgrid(const DiffImpl& h) { return dim3(unsigned(std::max<int64_t>(h.pblocks, 1))); }
+inline dim3 cgrid(const DiffImpl& h) { return dim3(unsigned(std::max<int64_t>(h.cblocks_box, 1))); }
 
 // Jf = F^P(J) in the J / Jt pair; returns the scratch slot holding the result
 template<class T>
@@ -1131,16 +1148,16 @@ int step_t(DiffImpl& h, const rk_diff_state* in, rk_diff_state* out, hipStream_t
   if(h.np > 0) {
     // push, then the current through cell buckets: slots, offsets, records, per-cell sums
     RK_HIP(hipMemsetAsync(h.ccount, 0, sizeof(uint32_t) * size_t(h.ncell), st));
-    hipLaunchKernelGGL(k_dpush<T>, pgrid(h), dim3(DWG), 0, st, b, NM8, g3<T>(in->x), g3<T>(in->u),
+    hipLaunchKernelGGL(k_dpush<T>, pgrid(h), dim3(DWG), 0, st, b, h.pmap, NM8, g3<T>(in->x), g3<T>(in->u),
                        g3<T>(out->x), g3<T>(out->u), h.ccount, h.pslot, h.err);
     RK_HIP(hipGetLastError());
     size_t bytes = h.scan_bytes;
     RK_HIP(hipcub::DeviceScan::ExclusiveSum(h.scan_tmp, bytes, h.ccount, h.cstart, int(h.ncell), st));
     T* rec = static_cast<T*>(h.rec);
-    hipLaunchKernelGGL(k_dep_records<T>, pgrid(h), dim3(DWG), 0, st, b, g3<T>(out->x), g3<T>(out->u),
+    hipLaunchKernelGGL(k_dep_records<T>, pgrid(h), dim3(DWG), 0, st, b, h.pmap, g3<T>(out->x), g3<T>(out->u),
                        h.cstart, h.pslot, rec);
     RK_HIP(hipGetLastError());
-    hipLaunchKernelGGL(k_cell_deposit<T>, cgrid(h), dim3(DWG), 0, st, b, rec, h.cstart, h.ccount, J4);
+    hipLaunchKernelGGL(k_cell_deposit<T>, cgrid(h), dim3(DWG), 0, st, b, h.cmap, rec, h.cstart, h.ccount, J4);
     RK_HIP(hipGetLastError());
   }
   hipLaunchKernelGGL(k_j4_split<T>, dim3(cblocks(h)), dim3(DWG), 0, st, J4, g3s<T>(h, S_J), h.ncell);
@@ -1192,16 +1209,16 @@ int adjoint_t(DiffImpl& h, const rk_diff_state* in, const rk_diff_state* ao, rk_
   if(h.np > 0) {
     // bucket the particles by base cell: counts, offsets, then the records
     RK_HIP(hipMemsetAsync(h.ccount, 0, sizeof(uint32_t) * size_t(h.ncell), st));
-    hipLaunchKernelGGL(k_cell_count<T>, pgrid(h), dim3(DWG), 0, st, b, g3<T>(in->x), h.ccount, h.pslot);
+    hipLaunchKernelGGL(k_cell_count<T>, pgrid(h), dim3(DWG), 0, st, b, h.pmap, g3<T>(in->x), h.ccount, h.pslot);
     RK_HIP(hipGetLastError());
     size_t bytes = h.scan_bytes;
     RK_HIP(hipcub::DeviceScan::ExclusiveSum(h.scan_tmp, bytes, h.ccount, h.cstart, int(h.ncell), st));
     T* rec = static_cast<T*>(h.rec);
-    hipLaunchKernelGGL(k_dpush_adj<T>, pgrid(h), dim3(DWG), 0, st, b, NM8, g3<T>(in->x),
+    hipLaunchKernelGGL(k_dpush_adj<T>, pgrid(h), dim3(DWG), 0, st, b, h.pmap, NM8, g3<T>(in->x),
                        g3<T>(in->u), L4, T(-1), g3<T>(ao->x), g3<T>(ao->u), g3<T>(ai->x), g3<T>(ai->u), h.cstart,
                        h.pslot, rec);
     RK_HIP(hipGetLastError());
-    hipLaunchKernelGGL(k_cell_scatter<T>, cgrid(h), dim3(DWG), 0, st, b, rec, h.cstart, h.ccount, M8);
+    hipLaunchKernelGGL(k_cell_scatter<T>, cgrid(h), dim3(DWG), 0, st, b, h.cmap, rec, h.cstart, h.ccount, M8);
     RK_HIP(hipGetLastError());
   }
   // node-mean adjoints: lambda_E += NM_E^T(ME), lambda_B1 = LB2 + NM_B^T(MB)
@@ -1278,6 +1295,16 @@ int rk_diff_create_ensemble(rk_diff** out, const rk_diff_desc* descs, int32_t n_
     delete h;
     return dfail(RK_EINVAL, "diff ensemble: cells and particles must each total below 2^31");
   }
+  for(size_t i = 0; i < h->boxes.size(); ++i) {
+    const int64_t np = (i + 1 < h->boxes.size() ? h->p0[i + 1] : h->np) - h->p0[i];
+    const int64_t nc = (i + 1 < h->boxes.size() ? h->c0[i + 1] : h->ncell) - h->c0[i];
+    h->pb0.push_back(h->pblocks);
+    h->pnb.push_back(std::min<int64_t>((np + DWG - 1) / DWG, int64_t(1) << 20));
+    h->pblocks += h->pnb.back();
+    h->cb0.push_back(h->cblocks_box);
+    h->cnb.push_back(std::max<int64_t>(1, std::min<int64_t>((nc + DWG - 1) / DWG, 4096)));
+    h->cblocks_box += h->cnb.back();
+  }
   hipError_t e = h->vb == 4 ? upload_table<float>(*h) : upload_table<double>(*h);
   for(int i = 0; i < 5 && e == hipSuccess; ++i)
     for(int c = 0; c < 3 && e == hipSuccess; ++c) e = hipMalloc(&h->scratch[i][c], h->vb * size_t(h->ncell));
@@ -1322,6 +1349,8 @@ int rk_diff_destroy(rk_diff* h)
     if(q) (void)hipFree(q);
   if(h->err) (void)hipFree(h->err);
   if(h->tab) (void)hipFree(h->tab);
+  if(h->pmap) (void)hipFree(h->pmap);
+  if(h->cmap) (void)hipFree(h->cmap);
   delete h;
   return RK_OK;
 }
