@@ -60,10 +60,21 @@ def run_turbulence(args):
     from runko_amd.stepper import BatchedLap
 
     N, tile, ppc = args.cells, args.tile, args.ppc
+    tuned = None
+    brick = None
+    if not args.no_autotune:
+        # brick shape from the autotuner (fused kernel timed per shape on a
+        # 2x2x2-tile world of the same tile, ppc and theta)
+        from runko_amd.autotune import tune_brick
+        t0 = time.time()
+        brick, table = tune_brick(tile, ppc=ppc, theta=0.3)
+        tuned = dict(tile=tile, ppc=ppc, theta=0.3, best=list(brick), seconds=time.time() - t0,
+                     ns_per_particle={",".join(map(str, b)): v for b, v in sorted(table.items(), key=lambda kv: kv[1])})
+        log(f"[bench] autotune: brick {brick} of {len(table)} shapes in {tuned['seconds']:.1f}s")
     t0 = time.time()
     w, info = turbulence_world(N=N, tile=tile, ppc=ppc, theta=0.3, sigma=10.0, cfl=0.45, c_omp=10.0, n_modes=8,
                                dB_over_B0=1.0, seed=1, mig_fraction=args.mig_fraction,
-                               slot_order=args.slot_order)
+                               slot_order=args.slot_order, brick=brick)
     npart = w.total_particles()
     ncell = N ** 3
     log(f"[bench] turbulence {N}^3, tiles {tile}^3, {npart} particles, brick {w.brick}, setup {time.time() - t0:.1f}s")
@@ -121,7 +132,7 @@ def run_turbulence(args):
         ms_per_step=1e3 * wall / args.steps, prof=prof, fused_ms=float(fused_ms), alg_bytes=alg_bytes,
         achieved_gbs=achieved, field_ms_per_step=field_ms / args.steps,
         cell_updates_per_s=ncell * args.steps / (field_ms * 1e-3) if field_ms > 0 else None,
-        error_counters=[int(v) for v in err], brick=list(w.brick), B0=info["B0"], plain=plain)
+        error_counters=[int(v) for v in err], brick=list(w.brick), B0=info["B0"], plain=plain, autotune=tuned)
 
 
 def config0_inputs(seed=42):
@@ -473,6 +484,7 @@ def main():
     ap.add_argument("--adjoint-ppc", type=int, default=16)
     ap.add_argument("--adjoint-steps", type=int, default=200)
     ap.add_argument("--adjoint-every", type=int, default=20)
+    ap.add_argument("--no-autotune", action="store_true", help="brick shape by the rule, not the autotuner")
     ap.add_argument("--no-shock", action="store_true", help="skip the reflecting-wall shock workload")
     ap.add_argument("--only-shock", action="store_true",
                     help="run only the shock workload and print its result object")
@@ -586,6 +598,7 @@ def main():
             "unit": "GB/s",
             "frac": (48.0 * res["npart"] + 228.0 * res["ncell"]) / (res["ms_per_step"] * 1e-3) / 1e9 / HBM_PEAK_GBS,
         },
+        "autotune": res["autotune"],
         "cpu_baseline": cpu,
         "fdtd_cell_updates_per_s": res["cell_updates_per_s"],
         "kernel_ms": res["prof"],

@@ -1,0 +1,96 @@
+"""Brick-shape autotuner for the fused particle step.
+
+A world's particles are binned by bricks (sub-blocks of a tile); one
+workgroup of the fused step owns one brick, staging its field means and
+accumulating its current in LDS.  The brick shape sets the LDS footprint,
+the occupancy, how many particles leave their brick each step and how much
+halo each window re-reads, so the best shape depends on the tile size, the
+particles per cell and the temperature.  `tune_brick` times the fused
+kernel (HIP events of the library's own profile) for every candidate shape
+on a small periodic world of the same tile size, ppc and theta (2 x 2 x 2
+tiles, seeded turbulence set-up), and returns the fastest per particle.
+Results are cached per (tile, ppc, theta) for the process.
+
+The shape is fixed when a world lays out its bins, so callers tune before
+creating the world (`TileGrid` does when the configuration sets
+`brick_shape = "auto"`).
+"""
+
+import itertools
+
+import numpy as np
+import torch
+
+from . import _lib as L
+
+# LDS bytes of one fused workgroup: E/B node means (float, 6 per node over
+# (b + 2)^3) plus the current window (double, 3 per node over (b + 3)^3);
+# at most 64 KiB, so two workgroups share a CU's 160 KiB
+LDS_LIMIT = 64 * 1024
+SIDES = (4, 8, 16)
+
+_CACHE = {}
+
+
+def fused_lds_bytes(brick):
+    m = 4 * 6 * np.prod([b + 2 for b in brick])
+    c = 8 * 3 * np.prod([b + 3 for b in brick])
+    return int(m + c)
+
+
+def candidate_bricks(n_cells, limit=LDS_LIMIT):
+    """Shapes with every side in {4, 8, 16} dividing the tile and a fused LDS
+    footprint within `limit`; the cubes first."""
+    out = []
+    for b in itertools.product(SIDES, repeat=3):
+        if all(n % v == 0 for n, v in zip(n_cells, b)) and fused_lds_bytes(b) <= limit:
+            out.append(tuple(int(v) for v in b))
+    out.sort(key=lambda b: (len(set(b)) != 1, -min(b), b))
+    return out
+
+
+def time_brick(tile, brick, ppc, theta, steps=3, warmup=2, seed=1):
+    """ms of the fused kernel per step and particles of the trial world."""
+    from .init import turbulence_world
+    from .stepper import BatchedLap
+
+    w, _ = turbulence_world(N=2 * tile, tile=tile, ppc=ppc, theta=theta, sigma=1.0, seed=seed, brick=brick)
+    try:
+        lap = BatchedLap(w)
+        for _ in range(warmup):
+            lap()
+            lap.settle()
+        torch.cuda.synchronize()
+        L.check(L.lib().rk_profile_enable(w.h, 1))
+        for _ in range(steps):
+            lap()
+            lap.settle()
+        torch.cuda.synchronize()
+        ms = np.zeros(9)
+        cnt = np.zeros(9, dtype=np.int64)
+        L.check(L.lib().rk_profile_read(w.h, ms.ctypes.data, cnt.ctypes.data))
+        L.check(L.lib().rk_profile_enable(w.h, 0))
+        return float(ms[0] / max(cnt[0], 1)), w.total_particles()
+    finally:
+        del w
+        torch.cuda.empty_cache()
+
+
+def tune_brick(tile, ppc=16, theta=0.3, candidates=None, steps=3):
+    """(best brick, {brick: ns per particle}) for cubic tiles of side `tile`."""
+    key = (int(tile), int(ppc), float(theta))
+    if key in _CACHE and candidates is None:
+        return _CACHE[key]
+    table = {}
+    for b in candidates or candidate_bricks((tile,) * 3):
+        try:
+            ms, npart = time_brick(tile, b, ppc, theta, steps=steps)
+        except (L.RkError, RuntimeError):
+            continue  # a shape the device or the library refuses
+        table[b] = 1e6 * ms / npart
+    if not table:
+        raise RuntimeError(f"autotune: no brick shape ran for tile {tile}")
+    best = min(table, key=table.get)
+    if candidates is None:
+        _CACHE[key] = (best, table)
+    return best, table

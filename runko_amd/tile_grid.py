@@ -62,6 +62,8 @@ class TileGrid:
             raise RuntimeError(f"tile grid is missing tiles, e.g. {missing[0]}")
         tiles = self.tiles()
         t0 = tiles[0]
+        if brick == "auto":
+            brick = self._tuned_brick(tiles)
         world = World(t0.n, self.n_tiles, (0, 0, 0), self.n_tiles, periodic=True, cfl=t0.cfl,
                       charges=t0._charges, masses=t0._masses, current_filter=t0.current_filter or "binomial2",
                       brick=brick)
@@ -100,6 +102,19 @@ class TileGrid:
         world.ensure_migration(max(1 << 16, int(0.12 * total)))
         self.world = world
         return world
+
+    @staticmethod
+    def _tuned_brick(tiles):
+        """The autotuner's brick for cubic tiles at the grid's mean ppc (the
+        rule of choose_brick otherwise)."""
+        n = tuple(tiles[0].n)
+        if len(set(n)) != 1:
+            return None
+        from .autotune import tune_brick
+        nsp = max(1, len(tiles[0]._charges))
+        total = sum(tile._world.tile_count(0, s) for tile in tiles for s in range(len(tiles[0]._charges)))
+        ppc = max(1, int(round(total / (np.prod(n) * len(tiles) * nsp))))
+        return tune_brick(n[0], ppc=ppc)[0]
 
     @property
     def _corgi_grid(self):

@@ -52,8 +52,10 @@ def random_inputs(nt, n, nsp, seed, ppc=4, field_amp=0.05, umax=1.5):
 
 
 def build_gpu(nt, n, charges, masses, inputs, filt="binomial2", execution="plain", comm_prelude=True, pusher="boris",
-              stencil=None):
+              stencil=None, brick_shape=None):
     conf = make_config(nt, n, charges, masses, filt=filt, execution=execution, pusher=pusher, stencil=stencil)
+    if brick_shape is not None:
+        conf.brick_shape = brick_shape
     grid = runko.TileGrid(conf)
     for idx in grid.local_tile_indices():
         F, parts = inputs[idx]
