@@ -179,9 +179,10 @@ int rk_deposit_current(rk_world* w, int32_t tile, void* stream); /* zigzag, firs
 int rk_pack_outgoing_particles(rk_world* w, int32_t tile, void* stream);
 int rk_sort_particles(rk_world* w, int32_t tile, void* stream);
 
-/* Fused batched particle phase: interpolation + Boris + tile/brick
- * classification + zigzag deposit of every particle of the world in one
- * launch, followed by the migration append.  Equivalent to push_particles,
+/* Fused batched particle phase: interpolation + push (the world's pusher:
+ * Boris, Higuera-Cary or Faraday) + tile/brick classification + zigzag
+ * deposit of every particle of the world in one launch, followed by the
+ * migration append.  Equivalent to push_particles,
  * pack_outgoing_particles, comm_local(pic_particle), deposit_current on every
  * tile.  No counterpart in the reference (its driver issues those four). */
 int rk_particle_step_fused(rk_world* w, void* stream);

@@ -515,7 +515,6 @@ int rk_particle_step_fused_walls(rk_world* w, const rk_reflector_wall* walls, in
   if(!w->mbox.meta || !w->mig[0].meta || !w->mig[1].meta || !w->jstage)
     return fail(RK_EINVAL, "fused particle step: mailbox, migration or current staging buffers not set");
   if(w->filter_open) return fail(RK_ESTATE, "particle step during an open filter pass");
-  if(w->g.pusher != 0) return fail(RK_ESTATE, "the fused particle step implements the boris pusher only");
   WallSet ws {};
   for(int i = 0; i < n_walls; ++i) ws.w[ws.n++] = walls[i];
   if(ws.n > 0)

@@ -1368,7 +1368,9 @@ struct NbrTab {
 // brick's tile reflect or park each pushed particle, reflections add their
 // correction current to the world buffer Jc, parked particles leave the bin
 // without a record and deposit nothing.
-template<int BR, int FAST, int WALL = 0>
+// PUSHER: 0 Boris, 1 Higuera-Cary, 2 Faraday (the same device functions as
+// the per-tile push, so the fused step moves particles bit-identically).
+template<int BR, int FAST, int WALL = 0, int PUSHER = 0>
 __global__ void __launch_bounds__(FWG) __attribute__((amdgpu_waves_per_eu(4)))
   k_fused(Geo g, SpParams sp, FieldPtrs f, MigPtrs m, const MigPtrs* mtab, int nbins, int span, uint32_t* counters,
           float* jstage, int64_t jstage_stride, WallSet walls, float* Jc0, float* Jc1, float* Jc2)
@@ -1477,7 +1479,9 @@ __global__ void __launch_bounds__(FWG) __attribute__((amdgpu_waves_per_eu(4)))
         if(!gather6(S, ldsM, ci[0] + moff[0], ci[1] + moff[1], ci[2] + moff[2], dd, E, Bf))
           gather(g, f, MeanWin {{0, 0, 0}, {0, 0, 0}}, nullptr, Bk.toffh[0] + ci[0], Bk.toffh[1] + ci[1],
                  Bk.toffh[2] + ci[2], dd, E, Bf, &counters[C_STENCIL]);
-        boris<FAST>(g, qm, E, Bf, pos, vel);
+        if constexpr(PUSHER == 1) higuera_cary(g, qm, E, Bf, pos, vel);
+        else if constexpr(PUSHER == 2) faraday(g, qm, E, Bf, pos, vel);
+        else boris<FAST>(g, qm, E, Bf, pos, vel);
         if(WALL && wmask) {
           float* const Jc[3] = {Jc0, Jc1, Jc2};
           dead = !reflect_particle(g, walls, wmask, g.cfl, Bk.origo, Bk.toffh, Jc, charge, pos, vel,
@@ -1934,11 +1938,19 @@ int launch_fused(rk_world* w, hipStream_t s, const WallSet* walls)
   const bool cube8 = g.br[0] == 8 && g.br[1] == 8 && g.br[2] == 8;
   const bool cube4 = g.br[0] == 4 && g.br[1] == 4 && g.br[2] == 4;
   const int lvl = g.fast_div ? (g.fast_sqrt ? 2 : 1) : -1;
-  if(ws.n > 0) {
-    if(!w->corrJ[0]) {
-      set_error("wall correction buffer not set");
-      return RK_EINVAL;
-    }
+  if(ws.n > 0 && !w->corrJ[0]) {
+    set_error("wall correction buffer not set");
+    return RK_EINVAL;
+  }
+  if(g.pusher != 0) {
+    // alternative pushers: IEEE arithmetic throughout, 8^3 bricks or the generic shape
+    const bool hc = g.pusher == 1;
+    if(ws.n > 0) {
+      if(cube8) hc ? go(k_fused<8, -1, 1, 1>) : go(k_fused<8, -1, 1, 2>);
+      else hc ? go(k_fused<0, -1, 1, 1>) : go(k_fused<0, -1, 1, 2>);
+    } else if(cube8) hc ? go(k_fused<8, -1, 0, 1>) : go(k_fused<8, -1, 0, 2>);
+    else hc ? go(k_fused<0, -1, 0, 1>) : go(k_fused<0, -1, 0, 2>);
+  } else if(ws.n > 0) {
     if(cube8) lvl == 2 ? go(k_fused<8, 2, 1>) : (lvl == 1 ? go(k_fused<8, 1, 1>) : go(k_fused<8, -1, 1>));
     else if(cube4) lvl == 2 ? go(k_fused<4, 2, 1>) : (lvl == 1 ? go(k_fused<4, 1, 1>) : go(k_fused<4, -1, 1>));
     else go(k_fused<0, 0, 1>);

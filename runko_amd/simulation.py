@@ -184,7 +184,7 @@ class Simulation:
                     w.settle_overflow()
             return
         self._flush()
-        if op == "push" and self._world.pusher == "boris":  # the fused step implements boris
+        if op == "push":
             self._pending = ["push"]
         else:
             self._run_unfused(op)

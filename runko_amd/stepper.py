@@ -156,9 +156,9 @@ class GraphLap:
 class ShockLap:
     """walls: list of rk_reflector_wall; bcs: list of rk_edge_bc (all tiles).
 
-    fused (batched form, Boris pusher): push, reflect, pack, comm
-    pic_particle and deposit run as the fused particle step with the walls
-    inside it; otherwise as separate launches."""
+    fused (batched form): push, reflect, pack, comm pic_particle and deposit
+    run as the fused particle step with the walls inside it; otherwise as
+    separate launches."""
 
     def __init__(self, world, walls, bcs, filter_passes=3, tiles=None, fused=True):
         self.w = world
@@ -166,7 +166,7 @@ class ShockLap:
         self.bcs = list(bcs)
         self.filter_passes = filter_passes
         self.tiles = tiles  # None: one launch per method over every tile
-        self.fused = fused and tiles is None and world.pusher == "boris"
+        self.fused = fused and tiles is None
         self.laps_run = 0
 
     def _targets(self):

@@ -37,15 +37,16 @@ def world_particles(w, s):
     return sts[o], ids[o]
 
 
-@pytest.mark.parametrize("beta", [0.0, 0.25])
-def test_fused_reflect_parity_with_oracle(orc, beta):
+@pytest.mark.parametrize("beta,pusher", [(0.0, "boris"), (0.25, "boris"), (0.25, "higuera_cary"),
+                                         (0.0, "faraday")])
+def test_fused_reflect_parity_with_oracle(orc, beta, pusher):
     # particles start on both sides of the wall: in the fused step some
     # reflect, some are parked and removed, the rest are untouched
     nt, n = (2, 2, 1), (8, 8, 8)
     walloc = 5.25
     inputs = wall_inputs(nt, n, seed=21)
-    conf, grid, sim = build_gpu(nt, n, Q, M, inputs, execution="batched")
-    w = build_oracle(orc, nt, n, Q, M, inputs)
+    conf, grid, sim = build_gpu(nt, n, Q, M, inputs, execution="batched", pusher=pusher)
+    w = build_oracle(orc, nt, n, Q, M, inputs, pusher=pusher)
     gam = 1.0 / np.sqrt(1.0 - beta**2)
     wall = runko.pic.threeD.reflector_wall(walloc=walloc, betawall=beta, gammawall=gam)
     for tile in grid.tiles():
