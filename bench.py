@@ -271,7 +271,7 @@ def run_shock(args):
         lap()
         lap.settle()
         torch.cuda.synchronize()
-        r0 = w.repacks
+        r0, s0 = w.repacks, w.settle_seconds
         t1 = time.perf_counter()
         for i in range(steps):
             lap()
@@ -280,9 +280,13 @@ def run_shock(args):
         dt = time.perf_counter() - t1
         assert w.total_particles() == npart, "particle count changed (shock)"
         res[name] = dict(steps=steps, ms_per_step=1e3 * dt / steps, particle_steps_per_s=npart * steps / dt,
-                         repacks=w.repacks - r0)
+                         repacks=w.repacks - r0, repack_ms_per_step=1e3 * (w.settle_seconds - s0) / steps)
         log(f"[bench] shock {name}: {1e3 * dt / steps:.2f} ms/step, {npart * steps / dt:.3e} p-s/s, "
-            f"{w.repacks - r0} repacks")
+            f"{w.repacks - r0} repacks ({1e3 * (w.settle_seconds - s0) / steps:.2f} ms/step)")
+        if hasattr(lap, "recaptures"):
+            res[name].update(recaptures=lap.recaptures, recapture_ms_per_step=1e3 * lap.recapture_seconds / steps)
+            log(f"[bench] shock {name}: {lap.recaptures} graph recaptures "
+                f"({1e3 * lap.recapture_seconds / steps:.2f} ms/step)")
         del lap
     if "plain_per_tile" in res:
         for name in ("graph", "batched_fused", "batched_unfused"):

@@ -15,6 +15,7 @@ without the lap-function dispatch of Simulation:
 """
 
 import ctypes
+import time
 
 from . import _lib as L
 
@@ -118,6 +119,7 @@ class GraphLap:
         self.lap = lap
         if world.mbox is None or world.jstage is None:
             world.ensure_mailbox()
+        self.recaptures, self.recapture_seconds = 0, 0.0
         self._capture()
 
     def _capture(self):
@@ -149,8 +151,11 @@ class GraphLap:
         if self.w.pending_overflow():
             self.w.settle_overflow()
         if self.w.layout_version != self.version:
+            t0 = time.perf_counter()
             del self.graphs
             self._capture()
+            self.recaptures += 1
+            self.recapture_seconds += time.perf_counter() - t0
 
 
 class ShockLap:

@@ -13,6 +13,7 @@ that species' pool with larger bins and re-appends.
 
 import ctypes
 import math
+import time
 import os
 
 import numpy as np
@@ -50,6 +51,10 @@ def _require_gpu():
         raise RuntimeError("runko_amd needs a ROCm GPU (no CPU fallback)")
 
 
+ARENA = 0.10          # spare pool capacity, fraction of the laid-out bins
+ARENA_MAX = 1 << 26   # at most 64 M particle slots
+
+
 class _Pool:
     def __init__(self, nb, cap_per_bin, dev):
         caps = np.asarray(cap_per_bin, dtype=np.int64)
@@ -58,7 +63,12 @@ class _Pool:
     def set_layout(self, caps, dev):
         starts = np.zeros_like(caps)
         starts[1:] = np.cumsum(caps)[:-1]
-        self.capacity = int(caps.sum())
+        # bins fill [0, used); the arena [used, capacity) takes bins that
+        # outgrow their place (World.settle_overflow moves them there without
+        # a new pool, so buffer addresses stay the same)
+        self.used = int(caps.sum())
+        self.capacity = self.used + min(int(ARENA * self.used), ARENA_MAX) if self.used else 0
+        self.start_h, self.cap_h = starts.copy(), caps.copy()
         n = max(self.capacity, 1)
         self.pos = torch.empty((3, n), dtype=torch.float32, device=dev)
         self.vel = torch.empty((3, n), dtype=torch.float32, device=dev)
@@ -110,6 +120,8 @@ class World:
         # (captured HIP graphs hold the old one)
         self.layout_version = 0
         self.repacks = 0
+        self.bin_moves = 0  # repacks done inside the arena (no new pool)
+        self.settle_seconds = 0.0  # wall time spent repacking and re-appending
         self.count = [torch.zeros((self.nbricks,), dtype=torch.int32, device=self.dev) for _ in range(self.nsp)]
         self.pools = [_Pool(self.nbricks, np.zeros(self.nbricks, dtype=np.int64), self.dev) for _ in range(self.nsp)]
         for s in range(self.nsp):
@@ -260,24 +272,57 @@ class World:
         pending records.  Synchronous.  A full bin grows to its demand plus
         slack and at least 1.5x, so a bin that keeps filling (a compressed
         layer) is repacked a logarithmic number of times."""
+        t0 = time.perf_counter()
         for _ in range(4):
             if self.pending_overflow() == 0:
                 break
             for s in range(self.nsp):
                 demand = self.count[s].cpu().numpy().view(np.uint32).astype(np.int64)
-                caps = self.pools[s].cap.cpu().numpy().view(np.uint32).astype(np.int64)
+                pool = self.pools[s]
+                caps = pool.cap_h
                 full = demand > caps
                 if not np.any(full):
                     continue
                 grown = np.maximum(np.ceil(demand * SLACK).astype(np.int64) + PAD, (3 * caps + 1) // 2)
-                newcaps = np.where(full, grown, caps)
-                newcaps = (newcaps + 31) // 32 * 32
-                self._repack(s, newcaps)
+                grown = (grown + 31) // 32 * 32
+                need = int(grown[full].sum())
+                if pool.used + need <= pool.capacity:
+                    self._move_bins(s, full, grown)
+                else:
+                    self._repack(s, np.where(full, grown, caps))
             L.check(L.lib().rk_retry_append(self.h, _stream()))
         if self.pending_overflow():
             raise RuntimeError("runko_amd: particle bins still overflowing after repack")
+        self.settle_seconds += time.perf_counter() - t0
         # the mailboxes keep their size (leavers past it spill into the shared
         # record sets); replacing them here would orphan pending records
+
+    def _move_bins(self, s, full, grown):
+        """Full bins of species s to new places in the pool's arena, with
+        capacity `grown`; the other bins stay.  Same buffers: captured graphs
+        stay valid."""
+        pool = self.pools[s]
+        starts, caps = pool.start_h.copy(), pool.cap_h.copy()
+        idx = np.nonzero(full)[0]
+        g = grown[idx]
+        starts[idx] = pool.used + np.concatenate([[0], np.cumsum(g)[:-1]])
+        caps[idx] = g
+        new_start = torch.from_numpy(starts).to(self.dev)
+        new_cap = torch.from_numpy(caps.astype(np.int32)).to(self.dev)
+        # copies every bin to its (possibly new) place: unmoved bins onto
+        # themselves, moved ones into the arena (disjoint from every bin)
+        L.check(L.lib().rk_repack(self.h, s, pool.capacity, L.p3(pool.pos), L.p3(pool.vel), pool.id.data_ptr(),
+                                  new_start.data_ptr(), new_cap.data_ptr(), _stream()))
+        pool.start.copy_(new_start)
+        pool.cap.copy_(new_cap)
+        torch.cuda.synchronize()
+        pool.used += int(g.sum())
+        pool.start_h, pool.cap_h = starts, caps
+        L.check(L.lib().rk_world_set_species(self.h, s, pool.capacity, L.p3(pool.pos), L.p3(pool.vel),
+                                             pool.id.data_ptr(), pool.start.data_ptr(), pool.cap.data_ptr(),
+                                             self.count[s].data_ptr()))
+        self.repacks += 1
+        self.bin_moves += 1
 
     def _repack(self, s, newcaps):
         newp = _Pool(self.nbricks, newcaps, self.dev)
