@@ -301,10 +301,15 @@ int rk_apply_edge_bc(rk_world* w, int32_t tile, const rk_edge_bc* bc, int32_t mo
   if(!F[0]) return fail_w(RK_EINVAL, "fields not set");
   hipStream_t st = reinterpret_cast<hipStream_t>(stream);
   const int d = bc->direction;
-  const int t0 = tile < 0 ? 0 : tile, t1 = tile < 0 ? g.ntiles() : tile + 1;
+  // tile -1: one launch per slab of tiles along d (tile index tc[d]); every
+  // tile of a slab has the same width, and the union of their regions and
+  // world-halo cells is the slab's region over the whole world across d
+  const bool all = tile < 0;
+  const int t0 = all ? 0 : tile, t1 = all ? g.nt[d] : tile + 1;
   for(int t = t0; t < t1; ++t) {
-    int tc[3];
-    g.tile_coords(t, tc);
+    int tc[3] = {0, 0, 0};
+    if(all) tc[d] = t;
+    else g.tile_coords(t, tc);
     // Tile::edge_bc_width, emf/tile.c++:810-828
     const float tmin = g.tile_min(tc[d], d), tmax = float((g.toff[d] + tc[d] + 1) * g.n[d]);
     const int Nd = g.n[d];
@@ -325,10 +330,11 @@ int rk_apply_edge_bc(rk_world* w, int32_t tile, const rk_edge_bc* bc, int32_t mo
     // set those cells themselves; halo cells outside the world are set here.
     Box b;
     for(int a = 0; a < 3; ++a) {
-      b.r0[a] = HALO + tc[a] * g.n[a];
-      b.r1[a] = b.r0[a] + g.n[a];
+      const bool whole = all && a != d;
+      b.r0[a] = HALO + (whole ? 0 : tc[a] * g.n[a]);
+      b.r1[a] = b.r0[a] + (whole ? g.N[a] : g.n[a]);
       b.b0[a] = b.r0[a] - HALO;
-      b.bn[a] = g.n[a] + 2 * HALO;
+      b.bn[a] = (whole ? g.N[a] : g.n[a]) + 2 * HALO;
     }
     if(bc->side == 0) {
       b.r1[d] = b.r0[d] + wdt;

@@ -7,6 +7,7 @@ against the Simulation's shock lap, whose tile-by-tile form the wall tests
 compare with the oracle; and the on-GPU shock set-up (cold flow along -x,
 density ramp at the wall)."""
 
+import ctypes
 import math
 
 import numpy as np
@@ -179,3 +180,29 @@ def test_shock_world_batched_matches_plain_and_keeps_particles(form):
         assert np.array_equal(ib, ip)
         assert np.abs(sb - sp).max() < 1e-4
     assert rel_err(fb, fp) < 1e-5
+
+
+@pytest.mark.parametrize("direction,side,position", [(0, 0, 11.5), (0, 1, 11.5), (1, 0, 3.0), (2, 1, 12.0),
+                                                     (1, 1, 0.5), (2, 0, 24.0)])
+def test_world_edge_bc_matches_per_tile(direction, side, position):
+    # one launch per slab of tiles (tile -1) sets exactly the cells that the
+    # per-tile calls set, world halo included
+    from gpu_common import random_inputs
+    nt, n = (3, 2, 2), (8, 8, 8)
+    inputs = random_inputs(nt, n, 1, seed=41, ppc=1)
+    conf, grid, sim = build_gpu(nt, n, Q[:1], M[:1], inputs)
+    w = grid.world
+    bc = L.EdgeBC(direction, side, position, (1.0, 2.0, 3.0), (4.0, 5.0, 6.0), (7.0, 8.0, 9.0), 0b101, 0b011, 0b111)
+    for mode in (0, 1, 2):  # emf_J, emf_E, emf_B
+        w.F[2].normal_()
+        w.F[3].copy_(w.F[2])
+        start = w.F.clone()
+        L.check(L.lib().rk_apply_edge_bc(w.h, -1, ctypes.byref(bc), mode, stepper._stream()))
+        torch.cuda.synchronize()
+        got = w.F.clone()
+        w.F.copy_(start)
+        for t in range(w.ntiles):
+            L.check(L.lib().rk_apply_edge_bc(w.h, t, ctypes.byref(bc), mode, stepper._stream()))
+        torch.cuda.synchronize()
+        assert torch.equal(got, w.F), mode
+        assert not torch.equal(got, start), mode
