@@ -83,6 +83,13 @@ def run_turbulence(args):
         lap()
         lap.settle()
         log(f"[bench] warmup step {i} done")
+    launch = None
+    if not args.no_autotune:
+        # launch form of the fused step, timed on this world (one lap each)
+        from runko_amd.autotune import tune_fused_launch
+        best, times = tune_fused_launch(w, lap)
+        launch = dict(chosen=best, k_fused_ms=times)
+        log(f"[bench] fused launch: {best} ({', '.join(f'{k} {v:.2f} ms' for k, v in times.items())})")
     torch.cuda.synchronize()
     _lib.check(_lib.lib().rk_profile_enable(w.h, 1))
     ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
@@ -132,7 +139,8 @@ def run_turbulence(args):
         ms_per_step=1e3 * wall / args.steps, prof=prof, fused_ms=float(fused_ms), alg_bytes=alg_bytes,
         achieved_gbs=achieved, field_ms_per_step=field_ms / args.steps,
         cell_updates_per_s=ncell * args.steps / (field_ms * 1e-3) if field_ms > 0 else None,
-        error_counters=[int(v) for v in err], brick=list(w.brick), B0=info["B0"], plain=plain, autotune=tuned)
+        error_counters=[int(v) for v in err], brick=list(w.brick), B0=info["B0"], plain=plain,
+        autotune=dict(brick=tuned, fused_launch=launch))
 
 
 def config0_inputs(seed=42):
@@ -261,6 +269,11 @@ def run_shock(args):
     fused = ShockLap(w, walls, bcs)
     fused()
     fused.settle()
+    if not args.no_autotune:
+        from runko_amd.autotune import tune_fused_launch
+        best, times = tune_fused_launch(w, fused)
+        res["fused_launch"] = dict(chosen=best, k_fused_ms=times)
+        log(f"[bench] shock fused launch: {best} ({', '.join(f'{k} {v:.2f} ms' for k, v in times.items())})")
     runs = [("graph", lambda: GraphLap(w, lap=fused), args.shock_steps),
             ("batched_fused", lambda: fused, args.shock_steps),
             ("batched_unfused", lambda: ShockLap(w, walls, bcs, fused=False), max(1, args.shock_steps // 4))]

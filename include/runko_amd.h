@@ -187,6 +187,14 @@ int rk_sort_particles(rk_world* w, int32_t tile, void* stream);
  * tile.  No counterpart in the reference (its driver issues those four). */
 int rk_particle_step_fused(rk_world* w, void* stream);
 
+/* How the fused particle step is launched: mode 0, one workgroup per brick
+ * (bricks dealt to XCDs by whole tile-column groups); mode 1, persistent
+ * workgroups (as many as stay resident) that take bricks from per-XCD
+ * queues in the same order, for Boris with 8^3 bricks (other shapes and
+ * pushers launch as mode 0).  Results do not depend on the mode.  No
+ * counterpart in the reference. */
+int rk_world_set_fused_launch(rk_world* w, int32_t mode);
+
 /* The fused particle phase with reflecting walls: the tile methods
  * push_particles, reflect_particles (every wall in registration order on the
  * tiles it lies in, src/runko/pic/reflector_wall.c++:253-289),

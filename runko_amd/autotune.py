@@ -11,6 +11,10 @@ on a small periodic world of the same tile size, ppc and theta (2 x 2 x 2
 tiles, seeded turbulence set-up), and returns the fastest per particle.
 Results are cached per (tile, ppc, theta) for the process.
 
+`tune_fused_launch` picks, on the live world, between the two launch forms
+of the fused step (one workgroup per brick, or persistent workgroups taking
+bricks from per-XCD queues) by timing a lap of each.
+
 The shape is fixed when a world lays out its bins, so callers tune before
 creating the world (`TileGrid` does when the configuration sets
 `brick_shape = "auto"`).
@@ -94,3 +98,27 @@ def tune_brick(tile, ppc=16, theta=0.3, candidates=None, steps=3):
     if candidates is None:
         _CACHE[key] = (best, table)
     return best, table
+
+
+def tune_fused_launch(world, lap, laps_per_mode=1):
+    """Times the fused particle step of `lap` (a lap object of `world` that
+    calls the fused step: BatchedLap or a fused ShockLap) in both launch
+    forms, `laps_per_mode` laps each (the laps advance the simulation), and
+    leaves the world on the faster.  Returns (best, {mode: ms per launch})."""
+    times = {}
+    for mode in ("grid", "queue"):
+        world.set_fused_launch(mode)
+        torch.cuda.synchronize()
+        L.check(L.lib().rk_profile_enable(world.h, 1))
+        for _ in range(laps_per_mode):
+            lap()
+            lap.settle()
+        torch.cuda.synchronize()
+        ms = np.zeros(9)
+        cnt = np.zeros(9, dtype=np.int64)
+        L.check(L.lib().rk_profile_read(world.h, ms.ctypes.data, cnt.ctypes.data))
+        L.check(L.lib().rk_profile_enable(world.h, 0))
+        times[mode] = float(ms[0] / max(cnt[0], 1))
+    best = min(times, key=times.get)
+    world.set_fused_launch(best)
+    return best, times

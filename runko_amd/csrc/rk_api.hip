@@ -503,6 +503,14 @@ int rk_sort_particles(rk_world* w, int32_t tile, void* stream)
   return launch_sort(w, tile, S(stream));
 }
 
+int rk_world_set_fused_launch(rk_world* w, int32_t mode)
+{
+  if(!w) return fail(RK_EINVAL, "null world");
+  if(mode != 0 && mode != 1) return fail(RK_EINVAL, "fused launch: 0 (one workgroup per brick) or 1 (persistent queue)");
+  w->fused_queue = mode;
+  return RK_OK;
+}
+
 int rk_particle_step_fused(rk_world* w, void* stream) { return rk_particle_step_fused_walls(w, nullptr, 0, stream); }
 
 int rk_particle_step_fused_walls(rk_world* w, const rk_reflector_wall* walls, int32_t n_walls, void* stream)

@@ -130,7 +130,8 @@ enum Counter {
   C_MIGLOST = 3,     // records that did not fit the record buffer (particle kept in place)
   C_STENCIL = 4,     // particles whose stencil left the haloed lattice
   C_SLOWCOMPACT = 5, // bins compacted on the slow path
-  C_NUM = 8
+  C_QUEUE = 8,       // 8 per-XCD brick queues of the persistent fused step
+  C_NUM = 16
 };
 
 }  // namespace rk
@@ -161,6 +162,7 @@ struct rk_world {
   // B propagator: extended stencil (emf/yee_lattice_stencil.c++) when set,
   // [axis][row][column] coefficient matrices with M[0][0] normalised
   bool stencil_on = false;
+  int fused_queue = 0;  // fused step launch: 0 one workgroup per brick, 1 persistent queue
   float stencil[3][3][5] = {};
   std::vector<char> corr_pending;
   int corr_any = 0;

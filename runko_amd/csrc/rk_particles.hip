@@ -1370,16 +1370,12 @@ struct NbrTab {
 // without a record and deposit nothing.
 // PUSHER: 0 Boris, 1 Higuera-Cary, 2 Faraday (the same device functions as
 // the per-tile push, so the fused step moves particles bit-identically).
-template<int BR, int FAST, int WALL = 0, int PUSHER = 0>
-__global__ void __launch_bounds__(FWG) __attribute__((amdgpu_waves_per_eu(4)))
-  k_fused(Geo g, SpParams sp, FieldPtrs f, MigPtrs m, const MigPtrs* mtab, int nbins, int span, uint32_t* counters,
-          float* jstage, int64_t jstage_stride, WallSet walls, float* Jc0, float* Jc1, float* Jc2)
+template<int BR, int FAST, int WALL, int PUSHER>
+__device__ __forceinline__ void fused_brick(const Geo& g, const SpParams& sp, const FieldPtrs& f, const MigPtrs& m,
+                                            const MigPtrs* mtab, int b, uint32_t* counters, float* jstage,
+                                            int64_t jstage_stride, const WallSet& walls, float* Jc0, float* Jc1,
+                                            float* Jc2, double* lds_dyn, int* wcnt, NbrTab& NT)
 {
-  extern __shared__ double lds_dyn[];
-  __shared__ int wcnt[2 * (FWG / 64)];
-  __shared__ NbrTab NT;
-  const int b = xcd_brick(int(blockIdx.x), span);
-  if(b >= nbins) return;
   const Shape<BR> S(g);
   const Brick Bk = brick_of(g, b);
   uint64_t* ldsJ = reinterpret_cast<uint64_t*>(lds_dyn);
@@ -1580,6 +1576,45 @@ __global__ void __launch_bounds__(FWG) __attribute__((amdgpu_waves_per_eu(4)))
   const int V3 = 3 * S.DV();
   float* stage = jstage + int64_t(b) * jstage_stride;
   for(int q = int(threadIdx.x); q < V3; q += FWG) stage[q] = float(double(int64_t(ldsJ[q])) * g.jinv);
+}
+
+// QUEUE 0: one workgroup per brick, the XCD-aware map of blockIdx.  QUEUE 1:
+// persistent workgroups (as many as stay resident); the workgroups of XCD x
+// take bricks from that XCD's queue, an atomic counter over the same brick
+// order the map gives XCD x, until it runs past the last brick.  The queue
+// counters are cleared before the launch.
+template<int BR, int FAST, int WALL = 0, int PUSHER = 0, int QUEUE = 0>
+__global__ void __launch_bounds__(FWG) __attribute__((amdgpu_waves_per_eu(4)))
+  k_fused(Geo g, SpParams sp, FieldPtrs f, MigPtrs m, const MigPtrs* mtab, int nbins, int span, uint32_t* counters,
+          float* jstage, int64_t jstage_stride, WallSet walls, float* Jc0, float* Jc1, float* Jc2)
+{
+  extern __shared__ double lds_dyn[];
+  __shared__ int wcnt[2 * (FWG / 64)];
+  __shared__ NbrTab NT;
+  if constexpr(QUEUE == 0) {
+    const int b = xcd_brick(int(blockIdx.x), span);
+    if(b >= nbins) return;
+    fused_brick<BR, FAST, WALL, PUSHER>(g, sp, f, m, mtab, b, counters, jstage, jstage_stride, walls, Jc0, Jc1, Jc2,
+                                        lds_dyn, wcnt, NT);
+  } else {
+    __shared__ int next;
+    const int x = int(blockIdx.x) % 8;
+    for(;;) {
+      if(threadIdx.x == 0) next = xcd_brick(x + 8 * int(atomicAdd(&counters[C_QUEUE + x], 1u)), span);
+      __syncthreads();
+      const int b = next;
+      __syncthreads();
+      if(b >= nbins) break;  // uniform: every later pop of this queue is past the end too
+      fused_brick<BR, FAST, WALL, PUSHER>(g, sp, f, m, mtab, b, counters, jstage, jstage_stride, walls, Jc0, Jc1,
+                                          Jc2, lds_dyn, wcnt, NT);
+      __syncthreads();  // the brick's LDS is read out before the next brick fills it
+    }
+  }
+}
+
+__global__ void k_reset_queues(uint32_t* counters)
+{
+  if(threadIdx.x < 8) counters[C_QUEUE + threadIdx.x] = 0;
 }
 
 // J on every lattice node += sum of the staged windows of the bricks that
@@ -1950,6 +1985,26 @@ int launch_fused(rk_world* w, hipStream_t s, const WallSet* walls)
       else hc ? go(k_fused<0, -1, 1, 1>) : go(k_fused<0, -1, 1, 2>);
     } else if(cube8) hc ? go(k_fused<8, -1, 0, 1>) : go(k_fused<8, -1, 0, 2>);
     else hc ? go(k_fused<0, -1, 0, 1>) : go(k_fused<0, -1, 0, 2>);
+  } else if(w->fused_queue && cube8) {
+    // persistent queue: as many workgroups as stay resident, a multiple of 8
+    auto goq = [&](auto kern) -> int {
+      int per_cu = 0, cus = 0, dev = 0;
+      RK_HIP(hipGetDevice(&dev));
+      RK_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+      RK_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kern, FWG, dyn));
+      const int blocks = std::max(8, (std::max(per_cu, 1) * cus + 7) / 8 * 8);
+      hipLaunchKernelGGL(k_reset_queues, dim3(1), dim3(64), 0, s, w->counters);
+      hipLaunchKernelGGL(kern, dim3(unsigned(blocks)), dim3(FWG), dyn, s, g, make_sp(w), make_fields(w), make_mig(w),
+                         static_cast<const MigPtrs*>(w->mig_table), g.nbricks, span, w->counters, w->jstage,
+                         w->jstage_stride, ws, w->corrJ[0], w->corrJ[1], w->corrJ[2]);
+      return RK_OK;
+    };
+    int e = RK_OK;
+    if(ws.n > 0)
+      e = lvl == 2 ? goq(k_fused<8, 2, 1, 0, 1>) : (lvl == 1 ? goq(k_fused<8, 1, 1, 0, 1>) : goq(k_fused<8, -1, 1, 0, 1>));
+    else
+      e = lvl == 2 ? goq(k_fused<8, 2, 0, 0, 1>) : (lvl == 1 ? goq(k_fused<8, 1, 0, 0, 1>) : goq(k_fused<8, -1, 0, 0, 1>));
+    if(e) return e;
   } else if(ws.n > 0) {
     if(cube8) lvl == 2 ? go(k_fused<8, 2, 1>) : (lvl == 1 ? go(k_fused<8, 1, 1>) : go(k_fused<8, -1, 1>));
     else if(cube4) lvl == 2 ? go(k_fused<4, 2, 1>) : (lvl == 1 ? go(k_fused<4, 1, 1>) : go(k_fused<4, -1, 1>));

@@ -120,6 +120,7 @@ class World:
         # (captured HIP graphs hold the old one)
         self.layout_version = 0
         self.repacks = 0
+        self.fused_launch = "grid"
         self.bin_moves = 0  # repacks done inside the arena (no new pool)
         self.settle_seconds = 0.0  # wall time spent repacking and re-appending
         self.count = [torch.zeros((self.nbricks,), dtype=torch.int32, device=self.dev) for _ in range(self.nsp)]
@@ -250,6 +251,15 @@ class World:
             self.settle_overflow()
             return
         L.check(rc)
+
+    def set_fused_launch(self, mode):
+        """"grid": one workgroup per brick; "queue": persistent workgroups
+        taking bricks from per-XCD queues."""
+        modes = {"grid": 0, "queue": 1}
+        if mode not in modes:
+            raise ValueError(f"fused launch must be one of {sorted(modes)}")
+        L.check(L.lib().rk_world_set_fused_launch(self.h, modes[mode]))
+        self.fused_launch = mode
 
     def particle_step_fused(self, walls=None):
         """walls: optional sequence of rk_reflector_wall structs, applied to

@@ -153,16 +153,18 @@ def test_shock_world_initial_state():
     assert torch.all(E[1][inner] == np.float32(Ey)) and torch.all(B[2][inner] == np.float32(B0))
 
 
-@pytest.mark.parametrize("form", ["fused", "graph"])
+@pytest.mark.parametrize("form", ["fused", "graph", "graph_queue"])
 def test_shock_world_batched_matches_plain_and_keeps_particles(form):
     runs = []
     for kind in (form, "plain"):
         w, walls, bcs, par = small_shock(seed=7)
         n0 = w.total_particles()
         lap = stepper.PlainShockLap(w, walls, bcs) if kind == "plain" else stepper.ShockLap(w, walls, bcs)
+        if kind == "graph_queue":
+            w.set_fused_launch("queue")
         lap()
         lap.settle()
-        if kind == "graph":
+        if kind.startswith("graph"):
             lap = stepper.GraphLap(w, lap=lap)
         for _ in range(4):
             lap()
