@@ -65,16 +65,19 @@ def run_turbulence(args):
     if not args.no_autotune:
         # brick shape from the autotuner (fused kernel timed per shape on a
         # 2x2x2-tile world of the same tile, ppc and theta)
-        from runko_amd.autotune import tune_brick
+        from runko_amd.autotune import tune_shape
         t0 = time.time()
-        brick, table = tune_brick(tile, ppc=ppc, theta=0.3)
-        tuned = dict(tile=tile, ppc=ppc, theta=0.3, best=list(brick), seconds=time.time() - t0,
-                     ns_per_particle={",".join(map(str, b)): v for b, v in sorted(table.items(), key=lambda kv: kv[1])})
-        log(f"[bench] autotune: brick {brick} of {len(table)} shapes in {tuned['seconds']:.1f}s")
+        brick, wg, table = tune_shape(tile, ppc=ppc, theta=0.3)
+        tuned = dict(tile=tile, ppc=ppc, theta=0.3, best=list(brick), workgroup=wg, seconds=time.time() - t0,
+                     ns_per_particle={f"{','.join(map(str, b))}/{t}": v
+                                      for (b, t), v in sorted(table.items(), key=lambda kv: kv[1])})
+        log(f"[bench] autotune: brick {brick}, {wg} threads, of {len(table)} shapes in {tuned['seconds']:.1f}s")
     t0 = time.time()
     w, info = turbulence_world(N=N, tile=tile, ppc=ppc, theta=0.3, sigma=10.0, cfl=0.45, c_omp=10.0, n_modes=8,
                                dB_over_B0=1.0, seed=1, mig_fraction=args.mig_fraction,
                                slot_order=args.slot_order, brick=brick)
+    if tuned:
+        w.set_fused_workgroup(tuned["workgroup"])
     npart = w.total_particles()
     ncell = N ** 3
     log(f"[bench] turbulence {N}^3, tiles {tile}^3, {npart} particles, brick {w.brick}, setup {time.time() - t0:.1f}s")

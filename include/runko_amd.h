@@ -195,6 +195,12 @@ int rk_particle_step_fused(rk_world* w, void* stream);
  * counterpart in the reference. */
 int rk_world_set_fused_launch(rk_world* w, int32_t mode);
 
+/* Threads per workgroup of the fused particle step: 512 (default) or 256; 256
+ * applies to Boris with 8^3 bricks launched one workgroup per brick (other
+ * cases use 512).  Results do not depend on it.  No counterpart in the
+ * reference. */
+int rk_world_set_fused_workgroup(rk_world* w, int32_t threads);
+
 /* The fused particle phase with reflecting walls: the tile methods
  * push_particles, reflect_particles (every wall in registration order on the
  * tiles it lies in, src/runko/pic/reflector_wall.c++:253-289),

@@ -5,10 +5,11 @@ workgroup of the fused step owns one brick, staging its field means and
 accumulating its current in LDS.  The brick shape sets the LDS footprint,
 the occupancy, how many particles leave their brick each step and how much
 halo each window re-reads, so the best shape depends on the tile size, the
-particles per cell and the temperature.  `tune_brick` times the fused
-kernel (HIP events of the library's own profile) for every candidate shape
-on a small periodic world of the same tile size, ppc and theta (2 x 2 x 2
-tiles, seeded turbulence set-up), and returns the fastest per particle.
+particles per cell and the temperature.  `tune_shape` times the fused
+kernel (HIP events of the library's own profile) for every candidate brick
+shape, and for 8^3 bricks both workgroup sizes (512 and 256 threads), on a
+small periodic world of the same tile size, ppc and theta (2 x 2 x 2 tiles,
+seeded turbulence set-up), and returns the fastest per particle.
 Results are cached per (tile, ppc, theta) for the process.
 
 `tune_fused_launch` picks, on the live world, between the two launch forms
@@ -53,13 +54,23 @@ def candidate_bricks(n_cells, limit=LDS_LIMIT):
     return out
 
 
-def time_brick(tile, brick, ppc, theta, steps=3, warmup=2, seed=1):
+def candidate_shapes(n_cells, limit=LDS_LIMIT):
+    """(brick, threads per workgroup) pairs: every candidate brick with 512
+    threads, and 8^3 bricks with 256 threads as well."""
+    out = [(b, 512) for b in candidate_bricks(n_cells, limit)]
+    if out and out[0][0] == (8, 8, 8):
+        out.insert(1, ((8, 8, 8), 256))
+    return out
+
+
+def time_brick(tile, brick, ppc, theta, steps=3, warmup=2, seed=1, workgroup=512):
     """ms of the fused kernel per step and particles of the trial world."""
     from .init import turbulence_world
     from .stepper import BatchedLap
 
     w, _ = turbulence_world(N=2 * tile, tile=tile, ppc=ppc, theta=theta, sigma=1.0, seed=seed, brick=brick)
     try:
+        w.set_fused_workgroup(workgroup)
         lap = BatchedLap(w)
         for _ in range(warmup):
             lap()
@@ -80,24 +91,38 @@ def time_brick(tile, brick, ppc, theta, steps=3, warmup=2, seed=1):
         torch.cuda.empty_cache()
 
 
-def tune_brick(tile, ppc=16, theta=0.3, candidates=None, steps=3):
-    """(best brick, {brick: ns per particle}) for cubic tiles of side `tile`."""
+def tune_shape(tile, ppc=16, theta=0.3, candidates=None, steps=3):
+    """(best brick, best threads per workgroup, {(brick, threads): ns per
+    particle}) for cubic tiles of side `tile`; candidates: (brick, threads)
+    pairs (default candidate_shapes)."""
     key = (int(tile), int(ppc), float(theta))
     if key in _CACHE and candidates is None:
         return _CACHE[key]
     table = {}
-    for b in candidates or candidate_bricks((tile,) * 3):
+    for b, wg in candidates or candidate_shapes((tile,) * 3):
         try:
-            ms, npart = time_brick(tile, b, ppc, theta, steps=steps)
+            ms, npart = time_brick(tile, b, ppc, theta, steps=steps, workgroup=wg)
         except (L.RkError, RuntimeError):
             continue  # a shape the device or the library refuses
-        table[b] = 1e6 * ms / npart
+        table[(b, wg)] = 1e6 * ms / npart
     if not table:
         raise RuntimeError(f"autotune: no brick shape ran for tile {tile}")
     best = min(table, key=table.get)
+    out = (best[0], best[1], table)
     if candidates is None:
-        _CACHE[key] = (best, table)
-    return best, table
+        _CACHE[key] = out
+    return out
+
+
+def tune_brick(tile, ppc=16, theta=0.3, candidates=None, steps=3):
+    """(best brick, {brick: ns per particle}) for cubic tiles of side `tile`
+    (each brick at its best workgroup size); candidates: bricks."""
+    pairs = None if candidates is None else [(b, 512) for b in candidates]
+    _, _, table = tune_shape(tile, ppc, theta, pairs, steps)
+    per = {}
+    for (b, _), v in table.items():
+        per[b] = min(v, per.get(b, v))
+    return min(per, key=per.get), per
 
 
 def tune_fused_launch(world, lap, laps_per_mode=1):

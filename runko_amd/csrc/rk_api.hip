@@ -511,6 +511,14 @@ int rk_world_set_fused_launch(rk_world* w, int32_t mode)
   return RK_OK;
 }
 
+int rk_world_set_fused_workgroup(rk_world* w, int32_t threads)
+{
+  if(!w) return fail(RK_EINVAL, "null world");
+  if(threads != 256 && threads != 512) return fail(RK_EINVAL, "fused workgroup: 256 or 512 threads");
+  w->fused_wg = threads;
+  return RK_OK;
+}
+
 int rk_particle_step_fused(rk_world* w, void* stream) { return rk_particle_step_fused_walls(w, nullptr, 0, stream); }
 
 int rk_particle_step_fused_walls(rk_world* w, const rk_reflector_wall* walls, int32_t n_walls, void* stream)

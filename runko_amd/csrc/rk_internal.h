@@ -163,6 +163,7 @@ struct rk_world {
   // [axis][row][column] coefficient matrices with M[0][0] normalised
   bool stencil_on = false;
   int fused_queue = 0;  // fused step launch: 0 one workgroup per brick, 1 persistent queue
+  int fused_wg = 512;   // fused step threads per workgroup (256 for Boris 8^3 bricks, else 512)
   float stencil[3][3][5] = {};
   std::vector<char> corr_pending;
   int corr_any = 0;

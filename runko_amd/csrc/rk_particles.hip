@@ -1370,7 +1370,7 @@ struct NbrTab {
 // without a record and deposit nothing.
 // PUSHER: 0 Boris, 1 Higuera-Cary, 2 Faraday (the same device functions as
 // the per-tile push, so the fused step moves particles bit-identically).
-template<int BR, int FAST, int WALL, int PUSHER>
+template<int BR, int FAST, int WALL, int PUSHER, int WGS>
 __device__ __forceinline__ void fused_brick(const Geo& g, const SpParams& sp, const FieldPtrs& f, const MigPtrs& m,
                                             const MigPtrs* mtab, int b, uint32_t* counters, float* jstage,
                                             int64_t jstage_stride, const WallSet& walls, float* Jc0, float* Jc1,
@@ -1380,9 +1380,9 @@ __device__ __forceinline__ void fused_brick(const Geo& g, const SpParams& sp, co
   const Brick Bk = brick_of(g, b);
   uint64_t* ldsJ = reinterpret_cast<uint64_t*>(lds_dyn);
   float* ldsM = reinterpret_cast<float*>(lds_dyn + 3 * S.DV());
-  for(int q = int(threadIdx.x); q < 3 * S.DV(); q += FWG) ldsJ[q] = 0;
+  for(int q = int(threadIdx.x); q < 3 * S.DV(); q += WGS) ldsJ[q] = 0;
   const bool any = bin_total(sp, b) != 0;
-  if(any) stage_means6<BR, FWG>(g, f, S, Bk.c0, ldsM);
+  if(any) stage_means6<BR, WGS>(g, f, S, Bk.c0, ldsM);
   __syncthreads();
   // window index of a particle's cell: tile-haloed cell - 3 - lb*br (means),
   // tile-haloed node - 2 - lb*br (current)
@@ -1446,14 +1446,14 @@ __device__ __forceinline__ void fused_brick(const Geo& g, const SpParams& sp, co
       nvel[2] = U2[q0];
       nid = ID[q0];
     }
-    for(int base = 0; base < count; base += FWG) {
+    for(int base = 0; base < count; base += WGS) {
       const int t = base + int(threadIdx.x);
       bool lv0 = false, lv1 = false, dead = false;
       float pos[3] = {npos[0], npos[1], npos[2]}, vel[3] = {nvel[0], nvel[1], nvel[2]};
       const uint64_t id = nid;
       int dir = 13;
-      if(t + FWG < count) {
-        const uint32_t qn = uint32_t(t + FWG);
+      if(t + WGS < count) {
+        const uint32_t qn = uint32_t(t + WGS);
         npos[0] = P0[qn];
         npos[1] = P1[qn];
         npos[2] = P2[qn];
@@ -1522,7 +1522,7 @@ __device__ __forceinline__ void fused_brick(const Geo& g, const SpParams& sp, co
       // the shared record sets
       const bool hole = lv0 || lv1;
       int tot;
-      const int r = block_rank1<FWG>(hole, wcnt, par, tot);
+      const int r = block_rank1<WGS>(hole, wcnt, par, tot);
       const int64_t local = int64_t(nrec) + r;
       bool spill = false, kept = false;
       uint32_t slot0 = 0, slot1 = 0;
@@ -1543,10 +1543,10 @@ __device__ __forceinline__ void fused_brick(const Geo& g, const SpParams& sp, co
       // it that departed (r counts all leavers, rk the kept ones)
       int kept_tot = 0;
       int rk = 0;
-      if(int64_t(nrec) + tot > m.per_brick) rk = block_rank1<FWG>(hole && kept, wcnt, par, kept_tot);
+      if(int64_t(nrec) + tot > m.per_brick) rk = block_rank1<WGS>(hole && kept, wcnt, par, kept_tot);
       // parked particles before this lane (they are not leavers)
       int dead_tot = 0, rd = 0;
-      if(WALL && wmask) rd = block_rank1<FWG>(dead, wcnt, par, dead_tot);
+      if(WALL && wmask) rd = block_rank1<WGS>(dead, wcnt, par, dead_tot);
       const int dst = wr + int(threadIdx.x) - (r - rk) - rd;
       if(t < count && hole && !kept) {
         // records keep the unwrapped position; the append applies the wrap
@@ -1564,7 +1564,7 @@ __device__ __forceinline__ void fused_brick(const Geo& g, const SpParams& sp, co
         U2[d] = vel[2];
         if(d != uint32_t(t)) ID[d] = id;
       }
-      wr += min(FWG, count - base) - (tot - kept_tot) - dead_tot;
+      wr += min(WGS, count - base) - (tot - kept_tot) - dead_tot;
       nrec += (int64_t(nrec) + tot <= m.per_brick) ? tot : int(m.per_brick - nrec);
     }
     __syncthreads();
@@ -1575,26 +1575,27 @@ __device__ __forceinline__ void fused_brick(const Geo& g, const SpParams& sp, co
   __syncthreads();
   const int V3 = 3 * S.DV();
   float* stage = jstage + int64_t(b) * jstage_stride;
-  for(int q = int(threadIdx.x); q < V3; q += FWG) stage[q] = float(double(int64_t(ldsJ[q])) * g.jinv);
+  for(int q = int(threadIdx.x); q < V3; q += WGS) stage[q] = float(double(int64_t(ldsJ[q])) * g.jinv);
 }
 
+// WGS: threads per workgroup (a multiple of 64).
 // QUEUE 0: one workgroup per brick, the XCD-aware map of blockIdx.  QUEUE 1:
 // persistent workgroups (as many as stay resident); the workgroups of XCD x
 // take bricks from that XCD's queue, an atomic counter over the same brick
 // order the map gives XCD x, until it runs past the last brick.  The queue
 // counters are cleared before the launch.
-template<int BR, int FAST, int WALL = 0, int PUSHER = 0, int QUEUE = 0>
-__global__ void __launch_bounds__(FWG) __attribute__((amdgpu_waves_per_eu(4)))
+template<int BR, int FAST, int WALL = 0, int PUSHER = 0, int QUEUE = 0, int WGS = FWG>
+__global__ void __launch_bounds__(WGS) __attribute__((amdgpu_waves_per_eu(4)))
   k_fused(Geo g, SpParams sp, FieldPtrs f, MigPtrs m, const MigPtrs* mtab, int nbins, int span, uint32_t* counters,
           float* jstage, int64_t jstage_stride, WallSet walls, float* Jc0, float* Jc1, float* Jc2)
 {
   extern __shared__ double lds_dyn[];
-  __shared__ int wcnt[2 * (FWG / 64)];
+  __shared__ int wcnt[2 * (WGS / 64)];
   __shared__ NbrTab NT;
   if constexpr(QUEUE == 0) {
     const int b = xcd_brick(int(blockIdx.x), span);
     if(b >= nbins) return;
-    fused_brick<BR, FAST, WALL, PUSHER>(g, sp, f, m, mtab, b, counters, jstage, jstage_stride, walls, Jc0, Jc1, Jc2,
+    fused_brick<BR, FAST, WALL, PUSHER, WGS>(g, sp, f, m, mtab, b, counters, jstage, jstage_stride, walls, Jc0, Jc1, Jc2,
                                         lds_dyn, wcnt, NT);
   } else {
     __shared__ int next;
@@ -1605,7 +1606,7 @@ __global__ void __launch_bounds__(FWG) __attribute__((amdgpu_waves_per_eu(4)))
       const int b = next;
       __syncthreads();
       if(b >= nbins) break;  // uniform: every later pop of this queue is past the end too
-      fused_brick<BR, FAST, WALL, PUSHER>(g, sp, f, m, mtab, b, counters, jstage, jstage_stride, walls, Jc0, Jc1,
+      fused_brick<BR, FAST, WALL, PUSHER, WGS>(g, sp, f, m, mtab, b, counters, jstage, jstage_stride, walls, Jc0, Jc1,
                                           Jc2, lds_dyn, wcnt, NT);
       __syncthreads();  // the brick's LDS is read out before the next brick fills it
     }
@@ -1985,6 +1986,18 @@ int launch_fused(rk_world* w, hipStream_t s, const WallSet* walls)
       else hc ? go(k_fused<0, -1, 1, 1>) : go(k_fused<0, -1, 1, 2>);
     } else if(cube8) hc ? go(k_fused<8, -1, 0, 1>) : go(k_fused<8, -1, 0, 2>);
     else hc ? go(k_fused<0, -1, 0, 1>) : go(k_fused<0, -1, 0, 2>);
+  } else if(w->fused_wg == 256 && cube8 && !w->fused_queue) {
+    auto go256 = [&](auto kern) {
+      hipLaunchKernelGGL(kern, grid, dim3(256), dyn, s, g, make_sp(w), make_fields(w), make_mig(w),
+                         static_cast<const MigPtrs*>(w->mig_table), g.nbricks, span, w->counters, w->jstage,
+                         w->jstage_stride, ws, w->corrJ[0], w->corrJ[1], w->corrJ[2]);
+    };
+    if(ws.n > 0)
+      lvl == 2 ? go256(k_fused<8, 2, 1, 0, 0, 256>)
+               : (lvl == 1 ? go256(k_fused<8, 1, 1, 0, 0, 256>) : go256(k_fused<8, -1, 1, 0, 0, 256>));
+    else
+      lvl == 2 ? go256(k_fused<8, 2, 0, 0, 0, 256>)
+               : (lvl == 1 ? go256(k_fused<8, 1, 0, 0, 0, 256>) : go256(k_fused<8, -1, 0, 0, 0, 256>));
   } else if(w->fused_queue && cube8) {
     // persistent queue: as many workgroups as stay resident, a multiple of 8
     auto goq = [&](auto kern) -> int {

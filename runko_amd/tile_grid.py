@@ -62,12 +62,15 @@ class TileGrid:
             raise RuntimeError(f"tile grid is missing tiles, e.g. {missing[0]}")
         tiles = self.tiles()
         t0 = tiles[0]
+        workgroup = None
         if brick == "auto":
-            brick = self._tuned_brick(tiles)
+            brick, workgroup = self._tuned_shape(tiles)
         world = World(t0.n, self.n_tiles, (0, 0, 0), self.n_tiles, periodic=True, cfl=t0.cfl,
                       charges=t0._charges, masses=t0._masses, current_filter=t0.current_filter or "binomial2",
                       brick=brick)
         world.set_schemes(getattr(t0, "particle_pusher", "boris"), t0._stencil)
+        if workgroup:
+            world.set_fused_workgroup(workgroup)
         torch.cuda.synchronize()
         # fields
         for t, tile in enumerate(tiles):
@@ -104,17 +107,18 @@ class TileGrid:
         return world
 
     @staticmethod
-    def _tuned_brick(tiles):
-        """The autotuner's brick for cubic tiles at the grid's mean ppc (the
-        rule of choose_brick otherwise)."""
+    def _tuned_shape(tiles):
+        """The autotuner's brick and workgroup size for cubic tiles at the
+        grid's mean ppc (the rule of choose_brick otherwise)."""
         n = tuple(tiles[0].n)
         if len(set(n)) != 1:
-            return None
-        from .autotune import tune_brick
+            return None, None
+        from .autotune import tune_shape
         nsp = max(1, len(tiles[0]._charges))
         total = sum(tile._world.tile_count(0, s) for tile in tiles for s in range(len(tiles[0]._charges)))
         ppc = max(1, int(round(total / (np.prod(n) * len(tiles) * nsp))))
-        return tune_brick(n[0], ppc=ppc)[0]
+        brick, wg, _ = tune_shape(n[0], ppc=ppc)
+        return brick, wg
 
     @property
     def _corgi_grid(self):

@@ -121,6 +121,7 @@ class World:
         self.layout_version = 0
         self.repacks = 0
         self.fused_launch = "grid"
+        self.fused_workgroup = 512
         self.bin_moves = 0  # repacks done inside the arena (no new pool)
         self.settle_seconds = 0.0  # wall time spent repacking and re-appending
         self.count = [torch.zeros((self.nbricks,), dtype=torch.int32, device=self.dev) for _ in range(self.nsp)]
@@ -260,6 +261,11 @@ class World:
             raise ValueError(f"fused launch must be one of {sorted(modes)}")
         L.check(L.lib().rk_world_set_fused_launch(self.h, modes[mode]))
         self.fused_launch = mode
+
+    def set_fused_workgroup(self, threads):
+        """Threads per workgroup of the fused step (512 or 256)."""
+        L.check(L.lib().rk_world_set_fused_workgroup(self.h, int(threads)))
+        self.fused_workgroup = int(threads)
 
     def particle_step_fused(self, walls=None):
         """walls: optional sequence of rk_reflector_wall structs, applied to

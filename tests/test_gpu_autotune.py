@@ -21,6 +21,9 @@ def test_tuner_times_every_candidate_and_picks_the_fastest():
     assert set(table) == set(cands)
     assert all(v > 0 for v in table.values())
     assert best == min(table, key=table.get)
+    from runko_amd.autotune import tune_shape
+    b, wg, t2 = tune_shape(8, ppc=2, candidates=[((8, 8, 8), 512), ((8, 8, 8), 256)], steps=2)
+    assert set(t2) == {((8, 8, 8), 512), ((8, 8, 8), 256)} and (b, wg) == min(t2, key=t2.get)
 
 
 def test_auto_brick_grid_matches_rule_brick_grid():
@@ -41,25 +44,29 @@ def test_auto_brick_grid_matches_rule_brick_grid():
     assert rel_err(fb, fa) < 1e-5
 
 
-@pytest.mark.parametrize("walls", [False, True])
-def test_queue_launch_matches_grid_launch(walls):
-    # the persistent queue visits the same bricks in another order; every
-    # brick's work is independent of the order (current windows summed in
-    # fixed point, gathered in a fixed order), so the laps agree bit for bit
+@pytest.mark.parametrize("walls,variant", [(False, "queue"), (True, "queue"), (False, "wg256"), (True, "wg256")])
+def test_queue_launch_matches_grid_launch(walls, variant):
+    # the persistent queue visits the same bricks in another order, and 256
+    # threads take a brick's particles in chunks of 256 instead of 512; every
+    # particle's work is independent of both (current windows summed in fixed
+    # point, gathered in a fixed order), so the laps agree bit for bit
     import torch
     from runko_amd.init import shock_world, turbulence_world
     from runko_amd.stepper import BatchedLap, ShockLap
     from test_gpu_shock import world_particles
 
     res = []
-    for mode in ("grid", "queue"):
+    for mode in ("grid", variant):
         if walls:
             w, wl, bcs, _ = shock_world(nx=128, ny=32, nz=32, tile=16, ppc=2, ramp_len=40.0, seed=9)
             lap = ShockLap(w, wl, bcs)
         else:
             w, _ = turbulence_world(N=64, tile=16, ppc=4, seed=3)
             lap = BatchedLap(w)
-        w.set_fused_launch(mode)
+        if mode == "queue":
+            w.set_fused_launch("queue")
+        elif mode == "wg256":
+            w.set_fused_workgroup(256)
         for _ in range(3):
             lap()
             lap.settle()
