@@ -1944,7 +1944,7 @@ int launch_append(rk_world* w, int set, hipStream_t s)
 
 int launch_append_mailbox(rk_world* w, hipStream_t s)
 {
-  const int grid = std::min(w->g.nbricks, 8192);
+  const int grid = w->g.nbricks;  // one block per brick (measured faster than a capped grid)
   hipLaunchKernelGGL(k_append_mailbox, dim3(unsigned(grid)), dim3(WG), 0, s, w->g, make_sp(w), make_mig(w),
                      w->g.nbricks, w->counters, w->ovf_list, w->ovf_cap);
   RK_HIP(hipGetLastError());
