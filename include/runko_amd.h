@@ -211,6 +211,19 @@ int rk_world_set_fused_workgroup(rk_world* w, int32_t threads);
  * counterpart in the reference (its driver issues those five). */
 int rk_particle_step_fused_walls(rk_world* w, const rk_reflector_wall* walls, int32_t n_walls, void* stream);
 
+/* Walls held in device memory, so that a moving wall can advance inside a
+ * captured HIP graph: rk_world_set_walls copies the walls to the device
+ * (synchronous; not under capture), rk_particle_step_fused_device_walls is
+ * rk_particle_step_fused_walls with those walls (its correction currents are
+ * added over the whole lattice, as the host does not track which tiles the
+ * walls are in), rk_advance_walls is Tile::advance_reflector_walls
+ * (src/runko/pic/reflector_wall.c++:293-300) for them as a kernel, and
+ * rk_read_walls copies them back (synchronous).  At most 8 walls. */
+int rk_world_set_walls(rk_world* w, const rk_reflector_wall* walls, int32_t n_walls);
+int rk_particle_step_fused_device_walls(rk_world* w, void* stream);
+int rk_advance_walls(rk_world* w, void* stream);
+int rk_read_walls(rk_world* w, rk_reflector_wall* out, int32_t n_walls, void* stream);
+
 /* E += cfl curl B followed by E -= J in one pass (same two roundings per
  * node as rk_push_e then rk_add_current).  Batched form of that pair. */
 int rk_push_e_add_current(rk_world* w, int32_t tile, void* stream);

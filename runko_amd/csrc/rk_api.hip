@@ -255,6 +255,7 @@ int rk_world_destroy(rk_world* w)
   (void)hipFree(w->mbox_count);
   (void)hipFree(w->mig_table);
   for(int c = 0; c < 3; ++c) (void)hipFree(w->corrJ[c]);
+  (void)hipFree(w->walls_dev);
   delete w;
   return RK_OK;
 }
@@ -521,19 +522,40 @@ int rk_world_set_fused_workgroup(rk_world* w, int32_t threads)
 
 int rk_particle_step_fused(rk_world* w, void* stream) { return rk_particle_step_fused_walls(w, nullptr, 0, stream); }
 
+namespace {
+int fused_step(rk_world* w, const rk_reflector_wall* walls, int32_t n_walls, bool device_walls, void* stream);
+}
+
 int rk_particle_step_fused_walls(rk_world* w, const rk_reflector_wall* walls, int32_t n_walls, void* stream)
 {
-  if(int rc = check_tile(w, -1)) return rc;
+  if(!w) return fail(RK_EINVAL, "null world");
   if(n_walls < 0 || n_walls > MAX_WALLS) return fail(RK_EINVAL, "at most 8 reflector walls");
   if(n_walls > 0 && !walls) return fail(RK_EINVAL, "null wall array");
+  return fused_step(w, walls, n_walls, false, stream);
+}
+
+int rk_particle_step_fused_device_walls(rk_world* w, void* stream)
+{
+  if(!w) return fail(RK_EINVAL, "null world");
+  return fused_step(w, nullptr, w->walls_dev_n, true, stream);
+}
+
+}  // extern "C"
+
+namespace {
+int fused_step(rk_world* w, const rk_reflector_wall* walls, int32_t n_walls, bool device_walls, void* stream)
+{
+  if(int rc = check_tile(w, -1)) return rc;
   if(int rc = check_particles(w)) return rc;
   if(!w->g.periodic) return fail(RK_EINVAL, "the fused particle step needs a periodic tile grid");
   if(!w->mbox.meta || !w->mig[0].meta || !w->mig[1].meta || !w->jstage)
     return fail(RK_EINVAL, "fused particle step: mailbox, migration or current staging buffers not set");
   if(w->filter_open) return fail(RK_ESTATE, "particle step during an open filter pass");
   WallSet ws {};
-  for(int i = 0; i < n_walls; ++i) ws.w[ws.n++] = walls[i];
-  if(ws.n > 0)
+  if(device_walls) ws.n = n_walls;  // the kernel reads the walls from device memory
+  else
+    for(int i = 0; i < n_walls; ++i) ws.w[ws.n++] = walls[i];
+  if(ws.n > 0 && !device_walls)
     if(int rc = mark_wall_tiles(w, ws)) return rc;
   hipStream_t s = S(stream);
   {
@@ -543,12 +565,14 @@ int rk_particle_step_fused_walls(rk_world* w, const rk_reflector_wall* walls, in
   w->j_accumulating = true;
   {
     ProfScope ps(w, 0, s);
-    if(int rc = launch_fused(w, s, &ws)) return rc;
+    if(int rc = launch_fused(w, s, &ws, device_walls ? w->walls_dev : nullptr)) return rc;
   }
   {
     ProfScope ps(w, 2, s);
     if(int rc = launch_current_gather(w, s)) return rc;
     if(int rc = apply_corrections(w, -1, s)) return rc;
+    if(device_walls && ws.n > 0)
+      if(int rc = add_corrections_world(w, s)) return rc;
   }
   ProfScope ps(w, 4, s);
   if(int rc = launch_append_mailbox(w, s)) return rc;
@@ -557,6 +581,9 @@ int rk_particle_step_fused_walls(rk_world* w, const rk_reflector_wall* walls, in
   if(int rc = launch_reset_counter(w, C_MIG0, s)) return rc;
   return launch_reset_counter(w, C_MIG1, s);
 }
+}  // namespace
+
+extern "C" {
 
 int rk_comm_local(rk_world* w, int32_t mode, void* stream)
 {

@@ -166,6 +166,8 @@ struct rk_world {
   int fused_wg = 512;   // fused step threads per workgroup (256 for Boris 8^3 bricks, else 512)
   float stencil[3][3][5] = {};
   std::vector<char> corr_pending;
+  rk_reflector_wall* walls_dev = nullptr;  // walls held on the device (rk_world_set_walls)
+  int walls_dev_n = 0;
   int corr_any = 0;
   // library scratch
   uint32_t* counters = nullptr;   // rk::C_NUM
@@ -249,7 +251,9 @@ int apply_corrections(rk_world* w, int tile, hipStream_t s);  // rk_wall.hip
 struct WallSet;                                               // rk_reflect.h
 int mark_wall_tiles(rk_world* w, const WallSet& ws);          // rk_wall.hip
 int launch_sort(rk_world* w, int tile, hipStream_t s);
-int launch_fused(rk_world* w, hipStream_t s, const WallSet* walls = nullptr);
+int launch_fused(rk_world* w, hipStream_t s, const WallSet* walls = nullptr,
+                 const rk_reflector_wall* wdev = nullptr);
+int add_corrections_world(rk_world* w, hipStream_t s);      // rk_wall.hip
 int kinetic_energies(rk_world* w, int tile, double* out, hipStream_t s);
 int launch_inject(rk_world* w, int tile, int s, int64_t n, const float* const pos[3],
                   const float* const vel[3], const uint64_t* id, hipStream_t st);

@@ -1587,15 +1587,25 @@ __device__ __forceinline__ void fused_brick(const Geo& g, const SpParams& sp, co
 template<int BR, int FAST, int WALL = 0, int PUSHER = 0, int QUEUE = 0, int WGS = FWG>
 __global__ void __launch_bounds__(WGS) __attribute__((amdgpu_waves_per_eu(4)))
   k_fused(Geo g, SpParams sp, FieldPtrs f, MigPtrs m, const MigPtrs* mtab, int nbins, int span, uint32_t* counters,
-          float* jstage, int64_t jstage_stride, WallSet walls, float* Jc0, float* Jc1, float* Jc2)
+          float* jstage, int64_t jstage_stride, WallSet walls, const rk_reflector_wall* wdev, float* Jc0, float* Jc1,
+          float* Jc2)
 {
   extern __shared__ double lds_dyn[];
   __shared__ int wcnt[2 * (WGS / 64)];
   __shared__ NbrTab NT;
+  // the walls (from the launch arguments, or from device memory when wdev
+  // is given: walls that move on the device), staged in LDS
+  __shared__ WallSet sw;
+  if constexpr(WALL) {
+    if(threadIdx.x == 0) sw.n = walls.n;
+    for(int i = 0; i < walls.n; ++i)
+      if(int(threadIdx.x) == i) sw.w[i] = wdev ? wdev[i] : walls.w[i];
+    __syncthreads();
+  }
   if constexpr(QUEUE == 0) {
     const int b = xcd_brick(int(blockIdx.x), span);
     if(b >= nbins) return;
-    fused_brick<BR, FAST, WALL, PUSHER, WGS>(g, sp, f, m, mtab, b, counters, jstage, jstage_stride, walls, Jc0, Jc1, Jc2,
+    fused_brick<BR, FAST, WALL, PUSHER, WGS>(g, sp, f, m, mtab, b, counters, jstage, jstage_stride, sw, Jc0, Jc1, Jc2,
                                         lds_dyn, wcnt, NT);
   } else {
     __shared__ int next;
@@ -1606,7 +1616,7 @@ __global__ void __launch_bounds__(WGS) __attribute__((amdgpu_waves_per_eu(4)))
       const int b = next;
       __syncthreads();
       if(b >= nbins) break;  // uniform: every later pop of this queue is past the end too
-      fused_brick<BR, FAST, WALL, PUSHER, WGS>(g, sp, f, m, mtab, b, counters, jstage, jstage_stride, walls, Jc0, Jc1,
+      fused_brick<BR, FAST, WALL, PUSHER, WGS>(g, sp, f, m, mtab, b, counters, jstage, jstage_stride, sw, Jc0, Jc1,
                                           Jc2, lds_dyn, wcnt, NT);
       __syncthreads();  // the brick's LDS is read out before the next brick fills it
     }
@@ -1958,7 +1968,7 @@ int launch_reset_counter(rk_world* w, int which, hipStream_t s)
   return RK_OK;
 }
 
-int launch_fused(rk_world* w, hipStream_t s, const WallSet* walls)
+int launch_fused(rk_world* w, hipStream_t s, const WallSet* walls, const rk_reflector_wall* wdev)
 {
   const Geo& g = w->g;
   const size_t dyn = means_lds(g) + cur_lds(g);
@@ -1969,7 +1979,7 @@ int launch_fused(rk_world* w, hipStream_t s, const WallSet* walls)
   auto go = [&](auto kern) {
     hipLaunchKernelGGL(kern, grid, dim3(FWG), dyn, s, g, make_sp(w), make_fields(w), make_mig(w),
                        static_cast<const MigPtrs*>(w->mig_table), g.nbricks, span, w->counters, w->jstage,
-                       w->jstage_stride, ws, w->corrJ[0], w->corrJ[1], w->corrJ[2]);
+                       w->jstage_stride, ws, wdev, w->corrJ[0], w->corrJ[1], w->corrJ[2]);
   };
   const bool cube8 = g.br[0] == 8 && g.br[1] == 8 && g.br[2] == 8;
   const bool cube4 = g.br[0] == 4 && g.br[1] == 4 && g.br[2] == 4;
@@ -1990,7 +2000,7 @@ int launch_fused(rk_world* w, hipStream_t s, const WallSet* walls)
     auto go256 = [&](auto kern) {
       hipLaunchKernelGGL(kern, grid, dim3(256), dyn, s, g, make_sp(w), make_fields(w), make_mig(w),
                          static_cast<const MigPtrs*>(w->mig_table), g.nbricks, span, w->counters, w->jstage,
-                         w->jstage_stride, ws, w->corrJ[0], w->corrJ[1], w->corrJ[2]);
+                         w->jstage_stride, ws, wdev, w->corrJ[0], w->corrJ[1], w->corrJ[2]);
     };
     if(ws.n > 0)
       lvl == 2 ? go256(k_fused<8, 2, 1, 0, 0, 256>)
@@ -2009,7 +2019,7 @@ int launch_fused(rk_world* w, hipStream_t s, const WallSet* walls)
       hipLaunchKernelGGL(k_reset_queues, dim3(1), dim3(64), 0, s, w->counters);
       hipLaunchKernelGGL(kern, dim3(unsigned(blocks)), dim3(FWG), dyn, s, g, make_sp(w), make_fields(w), make_mig(w),
                          static_cast<const MigPtrs*>(w->mig_table), g.nbricks, span, w->counters, w->jstage,
-                         w->jstage_stride, ws, w->corrJ[0], w->corrJ[1], w->corrJ[2]);
+                         w->jstage_stride, ws, wdev, w->corrJ[0], w->corrJ[1], w->corrJ[2]);
       return RK_OK;
     };
     int e = RK_OK;

@@ -216,6 +216,38 @@ int mark_wall_tiles(rk_world* w, const WallSet& ws)
   return RK_OK;
 }
 
+// J += corrections and clear them on every node of the world lattice (walls
+// whose tiles the host does not track: they move on the device)
+__global__ void k_add_corr_world(int64_t n, float* J0, float* J1, float* J2, float* C0, float* C1, float* C2)
+{
+  for(int64_t q = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; q < n; q += int64_t(gridDim.x) * blockDim.x) {
+    J0[q] = J0[q] + C0[q];
+    J1[q] = J1[q] + C1[q];
+    J2[q] = J2[q] + C2[q];
+    C0[q] = 0.0f;
+    C1[q] = 0.0f;
+    C2[q] = 0.0f;
+  }
+}
+
+// advance_reflector_walls on the device: walloc += betawall * cfl in float32
+// (reflector_wall.c++:293-300)
+__global__ void k_advance_walls(rk_reflector_wall* wl, int n, float cfl)
+{
+  const int i = int(threadIdx.x);
+  if(i < n) wl[i].walloc = wl[i].walloc + wl[i].betawall * cfl;
+}
+
+int add_corrections_world(rk_world* w, hipStream_t s)
+{
+  const int64_t n = int64_t(w->g.nodes());
+  const unsigned blocks = unsigned(std::min<int64_t>((n + 255) / 256, 8192));
+  hipLaunchKernelGGL(k_add_corr_world, dim3(blocks), dim3(256), 0, s, n, w->J[0], w->J[1], w->J[2], w->corrJ[0],
+                     w->corrJ[1], w->corrJ[2]);
+  RK_HIP(hipGetLastError());
+  return RK_OK;
+}
+
 int apply_corrections(rk_world* w, int tile, hipStream_t s)
 {
   if(!w->corr_any) return RK_OK;
@@ -279,6 +311,42 @@ int rk_reflect_particles(rk_world* w, int32_t tile, const rk_reflector_wall* wal
     hipLaunchKernelGGL(k_compact_dead, dim3(unsigned(g.bpt)), dim3(WWG), 0, st, sp, b0);
     RK_HIP(hipGetLastError());
   }
+  return RK_OK;
+}
+
+int rk_world_set_walls(rk_world* w, const rk_reflector_wall* walls, int32_t n_walls)
+{
+  if(!w) return fail_w(RK_EINVAL, "null world");
+  if(n_walls < 0 || n_walls > MAX_WALLS) return fail_w(RK_EINVAL, "at most 8 reflector walls");
+  if(n_walls > 0 && !walls) return fail_w(RK_EINVAL, "null wall array");
+  if(!w->walls_dev)
+    RK_HIP(hipMalloc(reinterpret_cast<void**>(&w->walls_dev), sizeof(rk_reflector_wall) * MAX_WALLS));
+  if(n_walls > 0) {
+    if(int rc = ensure_corrections(w)) return rc;
+    RK_HIP(hipMemcpy(w->walls_dev, walls, sizeof(rk_reflector_wall) * size_t(n_walls), hipMemcpyHostToDevice));
+  }
+  w->walls_dev_n = n_walls;
+  return RK_OK;
+}
+
+int rk_advance_walls(rk_world* w, void* stream)
+{
+  if(!w) return fail_w(RK_EINVAL, "null world");
+  if(w->walls_dev_n == 0) return RK_OK;
+  hipLaunchKernelGGL(k_advance_walls, dim3(1), dim3(64), 0, reinterpret_cast<hipStream_t>(stream), w->walls_dev,
+                     w->walls_dev_n, w->g.cfl);
+  RK_HIP(hipGetLastError());
+  return RK_OK;
+}
+
+int rk_read_walls(rk_world* w, rk_reflector_wall* out, int32_t n_walls, void* stream)
+{
+  if(!w || !out) return fail_w(RK_EINVAL, "null argument");
+  if(n_walls != w->walls_dev_n) return fail_w(RK_EINVAL, "read_walls: n_walls differs from the walls set");
+  if(n_walls == 0) return RK_OK;
+  hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+  RK_HIP(hipMemcpyAsync(out, w->walls_dev, sizeof(rk_reflector_wall) * size_t(n_walls), hipMemcpyDeviceToHost, st));
+  RK_HIP(hipStreamSynchronize(st));
   return RK_OK;
 }
 

@@ -96,9 +96,10 @@ class GraphLap:
     Every filter pass swaps the roles of J and the filter target, so with an
     odd number of passes consecutive laps run on swapped buffers: two graphs
     are captured (even and odd laps) and replayed alternately, starting with
-    the even one.  `lap` may instead be a fused ShockLap whose walls stand
-    still (the walls travel as launch arguments) and which has run one eager
-    lap (that lap allocates the wall correction buffer).  Capture records
+    the even one.  `lap` may instead be a fused ShockLap that has run one
+    eager lap (that lap allocates the wall correction buffer), with walls that
+    stand still (they travel as launch arguments) or walls held in device
+    memory (device_walls=True; they may move).  Capture records
     kernels only (the library allocates nothing and never synchronises inside
     these calls; kernel timing must be off).
     After every replay settle() checks the overflow counter (synchronous), as
@@ -112,8 +113,9 @@ class GraphLap:
         elif not isinstance(lap, ShockLap) or not lap.fused:
             raise ValueError("only BatchedLap and the fused ShockLap are captured (the separate particle "
                              "methods check their counters synchronously)")
-        elif any(float(wall.betawall) != 0.0 for wall in lap.walls):
-            raise ValueError("a moving wall changes the reflect launch every lap; replay it eagerly")
+        elif not lap.device_walls and any(float(wall.betawall) != 0.0 for wall in lap._walls):
+            raise ValueError("a moving wall given as launch arguments changes every lap: use "
+                             "ShockLap(..., device_walls=True) to replay it from a graph")
         elif lap.laps_run == 0:
             raise ValueError("run one eager lap of the ShockLap before capturing it")
         self.lap = lap
@@ -163,16 +165,25 @@ class ShockLap:
 
     fused (batched form): push, reflect, pack, comm pic_particle and deposit
     run as the fused particle step with the walls inside it; otherwise as
-    separate launches."""
+    separate launches.  device_walls (fused form): the walls live in device
+    memory and advance with a kernel, so a moving wall replays from a graph;
+    `walls` then reads them back."""
 
-    def __init__(self, world, walls, bcs, filter_passes=3, tiles=None, fused=True):
+    def __init__(self, world, walls, bcs, filter_passes=3, tiles=None, fused=True, device_walls=False):
         self.w = world
-        self.walls = list(walls)
+        self._walls = list(walls)
         self.bcs = list(bcs)
         self.filter_passes = filter_passes
         self.tiles = tiles  # None: one launch per method over every tile
         self.fused = fused and tiles is None
+        self.device_walls = bool(device_walls) and self.fused
+        if self.device_walls:
+            world.set_walls(self._walls)
         self.laps_run = 0
+
+    @property
+    def walls(self):
+        return self.w.read_walls() if self.device_walls else self._walls
 
     def _targets(self):
         return [-1] if self.tiles is None else range(self.w.ntiles)
@@ -191,9 +202,9 @@ class ShockLap:
             torch.cuda.synchronize()
 
     def _reflect(self):
-        arr = (L.ReflectorWall * len(self.walls))(*self.walls)
+        arr = (L.ReflectorWall * len(self._walls))(*self._walls)
         for t in self._targets():
-            L.check(L.lib().rk_reflect_particles(self.w.h, t, arr, len(self.walls), _stream()))
+            L.check(L.lib().rk_reflect_particles(self.w.h, t, arr, len(self._walls), _stream()))
         if self.tiles is not None:
             torch.cuda.synchronize()
 
@@ -202,8 +213,10 @@ class ShockLap:
         self._each("rk_push_half_b")
         self._bcs(comm_mode.emf_B)
         w.comm_local(int(comm_mode.emf_B))
-        if self.fused:
-            w.particle_step_fused(self.walls)
+        if self.device_walls:
+            w.particle_step_fused_device()
+        elif self.fused:
+            w.particle_step_fused(self._walls)
         else:
             self._each("rk_push_particles")
             self._reflect()
@@ -226,9 +239,12 @@ class ShockLap:
         self._each("rk_add_current")
         self._bcs(comm_mode.emf_E)
         w.comm_local(int(comm_mode.emf_E))
-        cfl = np.float32(w.cfl)
-        for wall in self.walls:
-            wall.walloc = float(np.float32(wall.walloc) + np.float32(wall.betawall) * cfl)
+        if self.device_walls:
+            w.advance_walls()
+        else:
+            cfl = np.float32(w.cfl)
+            for wall in self._walls:
+                wall.walloc = float(np.float32(wall.walloc) + np.float32(wall.betawall) * cfl)
         self.laps_run += 1
         if self.tiles is not None:
             torch.cuda.synchronize()

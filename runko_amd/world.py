@@ -262,6 +262,29 @@ class World:
         L.check(L.lib().rk_world_set_fused_launch(self.h, modes[mode]))
         self.fused_launch = mode
 
+    def set_walls(self, walls):
+        """Hold the reflecting walls (rk_reflector_wall structs) in device
+        memory for particle_step_fused_device / advance_walls."""
+        arr = (L.ReflectorWall * max(len(walls), 1))(*walls)
+        L.check(L.lib().rk_world_set_walls(self.h, arr, len(walls)))
+        self.n_device_walls = len(walls)
+
+    def particle_step_fused_device(self):
+        """The fused particle step with the walls held in device memory."""
+        if self._mbox_per_brick == 0:
+            self.ensure_mailbox()
+        L.check(L.lib().rk_particle_step_fused_device_walls(self.h, _stream()))
+
+    def advance_walls(self):
+        L.check(L.lib().rk_advance_walls(self.h, _stream()))
+
+    def read_walls(self):
+        """The device walls (synchronous)."""
+        n = getattr(self, "n_device_walls", 0)
+        arr = (L.ReflectorWall * max(n, 1))()
+        L.check(L.lib().rk_read_walls(self.h, arr, n, _stream()))
+        return [arr[i] for i in range(n)]
+
     def set_fused_workgroup(self, threads):
         """Threads per workgroup of the fused step (512 or 256)."""
         L.check(L.lib().rk_world_set_fused_workgroup(self.h, int(threads)))
