@@ -233,6 +233,21 @@ def run_cpu_baseline(inputs, seconds):
                        f"{laps} full laps ({dt:.1f} s), same step order as the GPU run")
 
 
+# the stencil coefficients of the reference's 3D shock configuration
+# (projects/pic-shock/3d_shock.ini:48-70, "StencilCylindricalDivFree_2x2")
+SHOCK_INI_STENCIL = {
+    "stencil_x_delta": -0.13512912586798165, "stencil_x_gamma": 0.016100341594971104,
+    "stencil_x_beta_p1": 0.022350417588794424, "stencil_x_beta_p2": 0.022350417588794424,
+    "stencil_x_zeta_p1": 0.012206124727660462, "stencil_x_zeta_p2": 0.012206124727660462,
+    "stencil_y_delta": 0.010144292861133962, "stencil_y_gamma": 0.012206124727660462,
+    "stencil_y_beta_p1": 0.022350417588794424, "stencil_y_beta_p2": -0.11902878427301054,
+    "stencil_y_zeta_p1": 0.012206124727660462, "stencil_y_zeta_p2": 0.016100341594971104,
+    "stencil_z_delta": 0.010144292861133962, "stencil_z_gamma": 0.012206124727660462,
+    "stencil_z_beta_p1": -0.11902878427301054, "stencil_z_beta_p2": 0.022350417588794424,
+    "stencil_z_zeta_p1": 0.016100341594971104, "stencil_z_zeta_p2": 0.012206124727660462,
+}
+
+
 def run_shock(args):
     """Reflecting-wall shock: nx x ny x nz cells in tile^3 tiles, gamma 10 cold
     flow along -x, 8 ppc/species upstream, density ramped 4x toward the wall
@@ -311,6 +326,38 @@ def run_shock(args):
     del runs, fused, w
     gc.collect()
     torch.cuda.empty_cache()
+    if not args.no_shock_schemes:
+        # the same shock with the schemes of the reference's 3D shock
+        # configuration: Faraday pusher, extended-stencil B push,
+        # binomial2_unrolled filter (projects/pic-shock/3d_shock.ini:40-70)
+        from runko_amd.stepper import GraphLap
+        from runko_amd.tiles import stencil_from_config
+        w, walls, bcs, par = shock_world(nx=nx, ny=ny, nz=nz, tile=args.tile, ppc=8, theta=1e-4, gamma_flow=10.0,
+                                         sigma=0.1, seed=3, pusher="faraday",
+                                         stencil=stencil_from_config(SHOCK_INI_STENCIL),
+                                         current_filter="binomial2_unrolled")
+        lap = ShockLap(w, walls, bcs)
+        lap()
+        lap.settle()
+        g = GraphLap(w, lap=lap)
+        g()
+        g.settle()
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        for _ in range(args.shock_steps):
+            g()
+            g.settle()
+        torch.cuda.synchronize()
+        dt = time.perf_counter() - t1
+        res["graph_faraday_stencil"] = dict(steps=args.shock_steps, ms_per_step=1e3 * dt / args.shock_steps,
+                                            particle_steps_per_s=w.total_particles() * args.shock_steps / dt,
+                                            schemes="faraday pusher, stencil B push (3d_shock.ini coefficients), "
+                                                    "binomial2_unrolled filter")
+        log(f"[bench] shock graph (faraday, stencil, binomial2_unrolled): {1e3 * dt / args.shock_steps:.2f} ms/step, "
+            f"{w.total_particles() * args.shock_steps / dt:.3e} p-s/s")
+        del g, lap, w
+        gc.collect()
+        torch.cuda.empty_cache()
     return res
 
 
@@ -508,6 +555,8 @@ def main():
     ap.add_argument("--no-shock", action="store_true", help="skip the reflecting-wall shock workload")
     ap.add_argument("--only-shock", action="store_true",
                     help="run only the shock workload and print its result object")
+    ap.add_argument("--no-shock-schemes", action="store_true",
+                    help="skip the shock run with the reference configuration's schemes")
     ap.add_argument("--shock-cells", type=int, nargs=3, default=[2048, 128, 128], metavar=("NX", "NY", "NZ"))
     ap.add_argument("--shock-steps", type=int, default=40)
     ap.add_argument("--shock-plain-steps", type=int, default=3)

@@ -280,17 +280,22 @@ def fill_shock_plasma(world, ppc, theta, gamma_flow, walloc, ramp_len, ramp_fact
 
 
 def shock_world(nx=2048, ny=128, nz=128, tile=32, ppc=8, theta=1e-4, gamma_flow=10.0, sigma=0.1, cfl=0.45,
-                c_omp=10.0, walloc=15.0, ramp_len=512.0, ramp_factor=4.0, seed=1, brick=None, mig_fraction=0.12):
+                c_omp=10.0, walloc=15.0, ramp_len=512.0, ramp_factor=4.0, seed=1, brick=None, mig_fraction=0.12,
+                pusher="boris", stencil=None, current_filter="binomial2"):
     """Reflecting-wall shock set-up: nx x ny x nz cells in tile^3 tiles, a cold
     gamma_flow flow along -x with a perpendicular field (B0 along z, motional
     E_y), a wall at walloc with the conducting and upstream edge conditions
-    of projects/pic-shock/pic.py:124-133.  Returns the world, the walls and
-    the edge conditions (as rk structs) and the parameters."""
+    of projects/pic-shock/pic.py:124-133.  pusher, stencil ([3][3][5] float32
+    matrices of the stencil B push, or None for fdtd2) and current_filter
+    select the schemes.  Returns the world, the walls and the edge conditions
+    (as rk structs) and the parameters."""
     from . import _lib as L
     nt = (nx // tile, ny // tile, nz // tile)
     q0, q1, B0, Ey = shock_parameters(cfl, c_omp, ppc, gamma_flow, sigma)
     w = World((tile,) * 3, nt, (0, 0, 0), nt, periodic=True, cfl=cfl, charges=[q0, q1], masses=[1.0, 1.0],
-              current_filter="binomial2", brick=brick)
+              current_filter=current_filter, brick=brick)
+    if pusher != "boris" or stencil is not None:
+        w.set_schemes(pusher, stencil)
     inner = (slice(HALO, HALO + w.N[0]), slice(HALO, HALO + w.N[1]), slice(HALO, HALO + w.N[2]))
     w.F.zero_()
     w.F[0][1][inner] = float(Ey)
