@@ -69,11 +69,12 @@ struct StencilM {
   float M[3][3][5];
 };
 
-__device__ inline float stencil_d(const float* __restrict__ F, int64_t c, const int64_t st[3], int a,
-                                  const float (&M)[3][5])
+// sum over n, col of M[n][col] * diff(n, col) in the reference's order;
+// at(ax, o1, o2) reads the field at offset ax along the derivative's axis a
+// and o1, o2 along the cyclic perpendicular axes (a+1)%3, (a+2)%3
+template<class At>
+__device__ inline float stencil_sum(At&& at, const float (&M)[3][5])
 {
-  const int64_t sa = st[a], s1 = st[(a + 1) % 3], s2 = st[(a + 2) % 3];
-  auto at = [&](int ax, int o1, int o2) { return F[c + ax * sa + o1 * s1 + o2 * s2]; };
   float sum = 0.0f;
 #pragma unroll
   for(int n = 0; n < 3; ++n) {
@@ -97,26 +98,107 @@ __device__ inline float stencil_d(const float* __restrict__ F, int64_t c, const 
   return sum;
 }
 
+// The stencil reads offsets -2..3 along the derivative's axis and -2..2
+// across it (324 reads of E per cell).  A block (BX cells along z, BY along
+// y) marches through SCH x planes, keeping the last six planes of E over
+// y, z in [start-2, end+3) in an LDS ring: each plane is read from memory
+// once per block, in rows along z, and every cell takes its six derivatives
+// from the ring.
+constexpr int SSY = BY + 5, SSZ = BX + 5, SPL = SSY * SSZ, SCH = 32;
+
+// One plane of the ring is 3 * SSY rows of SSZ floats; wave w of the block
+// moves rows w, w + 4, ... (lanes along z), through registers so the loads
+// of the next plane overlap the current plane's arithmetic.
+constexpr int SWAVES = BX * BY / 64, SROWS = (3 * SSY + SWAVES - 1) / SWAVES, SLANES = (SSZ + 63) / 64;
+static_assert(BX % 64 == 0, "a wave covers whole rows along z");
+
+__device__ inline void stencil_load(const Geo& g, float (&v)[SROWS][SLANES], int gi, int j0, int k0,
+                                    const float* __restrict__ Ex, const float* __restrict__ Ey,
+                                    const float* __restrict__ Ez)
+{
+  const int wave = (int(threadIdx.y) * BX + int(threadIdx.x)) / 64, lane = int(threadIdx.x) % 64;
+  const bool xin = gi >= 0 && gi < g.L[0];
+#pragma unroll
+  for(int q = 0; q < SROWS; ++q) {
+    const int row = wave + SWAVES * q, c = row / SSY, y = row % SSY, gj = j0 - 2 + y;
+    const float* __restrict__ F = c == 0 ? Ex : (c == 1 ? Ey : Ez);
+    const bool rin = row < 3 * SSY && xin && gj >= 0 && gj < g.L[1];
+#pragma unroll
+    for(int h = 0; h < SLANES; ++h) {
+      const int gk = k0 - 2 + lane + 64 * h;
+      v[q][h] = (rin && lane + 64 * h < SSZ && gk >= 0 && gk < g.L[2]) ? F[g.at(gi, gj, gk)] : 0.0f;
+    }
+  }
+}
+
+__device__ inline void stencil_store(float (&E)[3][6 * SPL], int slot, const float (&v)[SROWS][SLANES])
+{
+  const int wave = (int(threadIdx.y) * BX + int(threadIdx.x)) / 64, lane = int(threadIdx.x) % 64;
+#pragma unroll
+  for(int q = 0; q < SROWS; ++q) {
+    const int row = wave + SWAVES * q, c = row / SSY, y = row % SSY;
+#pragma unroll
+    for(int h = 0; h < SLANES; ++h)
+      if(row < 3 * SSY && lane + 64 * h < SSZ) E[c][slot * SPL + y * SSZ + lane + 64 * h] = v[q][h];
+  }
+}
+
+// derivative along axis A of the staged component Ec at the ring cell `cell`;
+// sb[d] is the ring offset of plane x + d - 2
+template<int A>
+__device__ inline float stencil_lds(const float* Ec, const int (&sb)[6], int cell, const float (&M)[3][5])
+{
+  return stencil_sum(
+    [&](int ax, int o1, int o2) {
+      int d[3];
+      d[A] = ax;
+      d[(A + 1) % 3] = o1;
+      d[(A + 2) % 3] = o2;
+      return Ec[sb[d[0] + 2] + cell + d[1] * SSZ + d[2]];
+    },
+    M);
+}
+
 __global__ void __launch_bounds__(BX* BY)
   k_curl_b_stencil(Geo g, Region r, float dt, StencilM S, const float* __restrict__ Ex,
                    const float* __restrict__ Ey, const float* __restrict__ Ez, float* __restrict__ Bx,
                    float* __restrict__ By, float* __restrict__ Bz)
 {
-  const int k = r.r0[2] + int(blockIdx.x) * BX + int(threadIdx.x);
-  const int j = r.r0[1] + int(blockIdx.y) * BY + int(threadIdx.y);
-  const int i = r.r0[0] + int(blockIdx.z);
-  if(k >= r.r1[2] || j >= r.r1[1]) return;
-  const int64_t c = g.at(i, j, k);
-  const int64_t st[3] = {int64_t(g.L[1]) * g.L[2], int64_t(g.L[2]), 1};
-  const float DzEy = stencil_d(Ey, c, st, 2, S.M[2]);
-  const float DyEz = stencil_d(Ez, c, st, 1, S.M[1]);
-  Bx[c] = Bx[c] + dt * (DzEy - DyEz);
-  const float DxEz = stencil_d(Ez, c, st, 0, S.M[0]);
-  const float DzEx = stencil_d(Ex, c, st, 2, S.M[2]);
-  By[c] = By[c] + dt * (DxEz - DzEx);
-  const float DyEx = stencil_d(Ex, c, st, 1, S.M[1]);
-  const float DxEy = stencil_d(Ey, c, st, 0, S.M[0]);
-  Bz[c] = Bz[c] + dt * (DyEx - DxEy);
+  __shared__ float E[3][6 * SPL];
+  const int k0 = r.r0[2] + int(blockIdx.x) * BX, j0 = r.r0[1] + int(blockIdx.y) * BY;
+  const int i0 = r.r0[0] + int(blockIdx.z) * SCH, i1 = min(r.r1[0], i0 + SCH);
+  const int k = k0 + int(threadIdx.x), j = j0 + int(threadIdx.y);
+  const bool active = k < r.r1[2] && j < r.r1[1];
+  const int cell = (2 + int(threadIdx.y)) * SSZ + 2 + int(threadIdx.x);
+  float v[SROWS][SLANES];
+  for(int p = 0; p < 5; ++p) {
+    stencil_load(g, v, i0 - 2 + p, j0, k0, Ex, Ey, Ez);
+    stencil_store(E, p, v);
+  }
+  stencil_load(g, v, i0 + 3, j0, k0, Ex, Ey, Ez);
+  int base = 0;  // ring slot of plane i - 2
+  for(int i = i0; i < i1; ++i) {
+    stencil_store(E, base == 0 ? 5 : base - 1, v);
+    __syncthreads();
+    if(i + 1 < i1) stencil_load(g, v, i + 4, j0, k0, Ex, Ey, Ez);
+    if(active) {
+      int sb[6];
+#pragma unroll
+      for(int d = 0; d < 6; ++d) sb[d] = (base + d < 6 ? base + d : base + d - 6) * SPL;
+      const int64_t c = g.at(i, j, k);
+      const float DzEy = stencil_lds<2>(E[1], sb, cell, S.M[2]);
+      const float DyEz = stencil_lds<1>(E[2], sb, cell, S.M[1]);
+      Bx[c] = Bx[c] + dt * (DzEy - DyEz);
+      const float DxEz = stencil_lds<0>(E[2], sb, cell, S.M[0]);
+      const float DzEx = stencil_lds<2>(E[0], sb, cell, S.M[2]);
+      By[c] = By[c] + dt * (DxEz - DzEx);
+      const float DyEx = stencil_lds<1>(E[0], sb, cell, S.M[1]);
+      const float DxEy = stencil_lds<0>(E[1], sb, cell, S.M[0]);
+      Bz[c] = Bz[c] + dt * (DyEx - DxEy);
+    }
+    __syncthreads();
+    base = base == 5 ? 0 : base + 1;
+  }
 }
 
 template<bool ADDJ>
@@ -404,7 +486,9 @@ int launch_curl_b_stencil(rk_world* w, const int r0[3], const int r1[3], float d
   if(empty(r)) return RK_OK;
   StencilM S;
   std::memcpy(S.M, w->stencil, sizeof(S.M));
-  hipLaunchKernelGGL(k_curl_b_stencil, region_grid(r), dim3(BX, BY), 0, s, w->g, r, dt, S, w->E[0], w->E[1],
+  dim3 grid = region_grid(r);
+  grid.z = unsigned((r.r1[0] - r.r0[0] + SCH - 1) / SCH);
+  hipLaunchKernelGGL(k_curl_b_stencil, grid, dim3(BX, BY), 0, s, w->g, r, dt, S, w->E[0], w->E[1],
                      w->E[2], w->B[0], w->B[1], w->B[2]);
   RK_HIP(hipGetLastError());
   return RK_OK;
