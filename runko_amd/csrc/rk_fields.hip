@@ -306,10 +306,34 @@ __global__ void __launch_bounds__(BX* BY)
 }
 
 // separable z, y, x passes (0.25, 0.5, 0.25), each output evaluated with the
-// reference's intermediate values recomputed in the same order
+// reference's intermediate values computed in the same order.  A lane sweeps
+// a column of FCH outputs along x; the z-then-y partial of a plane,
+//   T(p) = 0.25 Z(p, j-1) + 0.5 Z(p, j) + 0.25 Z(p, j+1),
+// depends on the plane alone, so each is computed once and reused by the
+// three outputs that take it.
 __device__ inline float sep_z(const float* __restrict__ S, uint32_t q)
 {
   return 0.25f * S[q - 1] + 0.5f * S[q] + 0.25f * S[q + 1];
+}
+
+__device__ inline float sep_zy(const Geo& g, const float* __restrict__ S, int i, int j, int k)
+{
+  const float y0 = sep_z(S, g.at(i, j - 1, k));
+  const float y1 = sep_z(S, g.at(i, j, k));
+  const float y2 = sep_z(S, g.at(i, j + 1, k));
+  return 0.25f * y0 + 0.5f * y1 + 0.25f * y2;
+}
+
+__device__ inline void sep_column(const Geo& g, const float* __restrict__ S, float* __restrict__ D, int i0, int i1,
+                                  int j, int k)
+{
+  float t0 = sep_zy(g, S, i0 - 1, j, k), t1 = sep_zy(g, S, i0, j, k);
+  for(int i = i0; i < i1; ++i) {
+    const float t2 = sep_zy(g, S, i + 1, j, k);
+    D[g.at(i, j, k)] = 0.25f * t0 + 0.5f * t1 + 0.25f * t2;
+    t0 = t1;
+    t1 = t2;
+  }
 }
 
 __global__ void __launch_bounds__(BX* BY)
@@ -319,22 +343,12 @@ __global__ void __launch_bounds__(BX* BY)
 {
   const int k = r.r0[2] + int(blockIdx.x) * BX + int(threadIdx.x);
   const int j = r.r0[1] + int(blockIdx.y) * BY + int(threadIdx.y);
-  const int i = r.r0[0] + int(blockIdx.z);
+  const int i0 = r.r0[0] + int(blockIdx.z) * FCH;
+  const int i1 = min(i0 + FCH, r.r1[0]);
   if(k >= r.r1[2] || j >= r.r1[1]) return;
-  const float* S[3] = {S0, S1, S2};
-  float* D[3] = {D0, D1, D2};
-#pragma unroll
-  for(int c = 0; c < 3; ++c) {
-    float t2[3];
-#pragma unroll
-    for(int a = 0; a < 3; ++a) {
-      const float y0 = sep_z(S[c], g.at(i + a - 1, j - 1, k));
-      const float y1 = sep_z(S[c], g.at(i + a - 1, j, k));
-      const float y2 = sep_z(S[c], g.at(i + a - 1, j + 1, k));
-      t2[a] = 0.25f * y0 + 0.5f * y1 + 0.25f * y2;
-    }
-    D[c][g.at(i, j, k)] = 0.25f * t2[0] + 0.5f * t2[1] + 0.25f * t2[2];
-  }
+  sep_column(g, S0, D0, i0, i1, j, k);
+  sep_column(g, S1, D1, i0, i1, j, k);
+  sep_column(g, S2, D2, i0, i1, j, k);
 }
 
 __device__ inline int wrap_idx(int c, int N) { return c < HALO ? c + N : (c >= N + HALO ? c - N : c); }
@@ -528,9 +542,12 @@ int launch_filter(rk_world* w, const int r0[3], const int r1[3], hipStream_t s)
     hipLaunchKernelGGL(k_filter27c, grid, dim3(BX, BY), 0, s, w->g, r, w->J[0], w->J[1], w->J[2], w->Jf[0],
                        w->Jf[1], w->Jf[2]);
   }
-  else
-    hipLaunchKernelGGL(k_filter_sep, region_grid(r), dim3(BX, BY), 0, s, w->g, r, w->J[0], w->J[1], w->J[2],
-                       w->Jf[0], w->Jf[1], w->Jf[2]);
+  else {
+    dim3 grid = region_grid(r);
+    grid.z = unsigned((r.r1[0] - r.r0[0] + FCH - 1) / FCH);
+    hipLaunchKernelGGL(k_filter_sep, grid, dim3(BX, BY), 0, s, w->g, r, w->J[0], w->J[1], w->J[2], w->Jf[0],
+                       w->Jf[1], w->Jf[2]);
+  }
   RK_HIP(hipGetLastError());
   return RK_OK;
 }
