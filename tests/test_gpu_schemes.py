@@ -65,6 +65,58 @@ def test_stencil_half_b_bitexact(orc):
     assert np.array_equal(gpu_fields(grid), w.global_fields())
 
 
+@pytest.mark.parametrize("execution", ["plain", "batched"])
+def test_stencil_half_b_bitexact_long_regions(orc, execution):
+    # tiles of 37 x planes (per tile: a 32-plane march and a partial one) and
+    # a 74-plane world (batched: two full marches and a partial one); 10 y
+    # rows end in a partial 4-row block
+    nt, n = (2, 1, 1), (37, 10, 7)
+    inputs = random_inputs(nt, n, 0, seed=86)
+    conf, grid, sim = build_gpu(nt, n, [], [], inputs, execution=execution, stencil=STENCIL)
+    w = build_oracle(orc, nt, n, [], [], inputs, stencil=STENCIL)
+    cm = runko.tools.comm_mode
+
+    def lap(x):
+        x.grid_push_half_b()
+        x.comm_local(cm.emf_B)
+        x.grid_push_e()
+        x.comm_local(cm.emf_E)
+        x.grid_push_half_b()
+
+    for _ in range(2):
+        sim.for_one_lap(lap)
+        w.push_half_b(); w.comm_local("emf_B"); w.push_e(); w.comm_local("emf_E"); w.push_half_b()
+    assert np.array_equal(gpu_fields(grid), w.global_fields())
+
+
+@pytest.mark.parametrize("execution", ["plain", "batched"])
+def test_unrolled_filter_bitexact_long_regions(orc, execution):
+    # the x-column sweep of the unrolled filter over 37-plane tiles / a
+    # 74-plane world (columns of 8 outputs, the last one partial)
+    nt, n = (2, 1, 1), (37, 10, 7)
+    inputs = random_inputs(nt, n, 0, seed=87)
+    rng = np.random.default_rng(88)
+    for F, _ in inputs.values():
+        F[6:9] = rng.standard_normal((3,) + n)
+    conf, grid, sim = build_gpu(nt, n, [], [], inputs, execution=execution, filt="binomial2_unrolled")
+    w = build_oracle(orc, nt, n, [], [], inputs, filt="binomial2_unrolled")
+    cm = runko.tools.comm_mode
+
+    def lap(x):
+        x.comm_local(cm.emf_J)
+        x.grid_filter_current()
+        x.comm_local(cm.emf_J)
+        x.grid_filter_current()
+
+    sim.for_one_lap(lap)
+    for _ in range(2):
+        w.comm_local("emf_J")
+        w.filter_current()
+    out = gpu_fields(grid)[6:9]
+    assert np.array_equal(out, w.global_fields()[6:9])
+    assert 0.0 < out.std() < 0.8  # unit-variance noise, smoothed by two passes
+
+
 def test_zero_stencil_is_fdtd2():
     nt, n = (1, 2, 1), (8, 8, 8)
     inputs = random_inputs(nt, n, 0, seed=84)
