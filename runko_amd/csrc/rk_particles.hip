@@ -1989,12 +1989,16 @@ int launch_fused(rk_world* w, hipStream_t s, const WallSet* walls, const rk_refl
     return RK_EINVAL;
   }
   if(g.pusher != 0) {
-    // alternative pushers: IEEE arithmetic throughout, 8^3 bricks or the generic shape
+    // alternative pushers (their own arithmetic in IEEE forms), 8^3 bricks or
+    // the generic shape; on 8^3 bricks the deposit and classification take
+    // the verified sqrt/rcp forms, which give the IEEE results
     const bool hc = g.pusher == 1;
     if(ws.n > 0) {
-      if(cube8) hc ? go(k_fused<8, -1, 1, 1>) : go(k_fused<8, -1, 1, 2>);
+      if(cube8 && lvl == 2) hc ? go(k_fused<8, 2, 1, 1>) : go(k_fused<8, 2, 1, 2>);
+      else if(cube8) hc ? go(k_fused<8, -1, 1, 1>) : go(k_fused<8, -1, 1, 2>);
       else hc ? go(k_fused<0, -1, 1, 1>) : go(k_fused<0, -1, 1, 2>);
-    } else if(cube8) hc ? go(k_fused<8, -1, 0, 1>) : go(k_fused<8, -1, 0, 2>);
+    } else if(cube8 && lvl == 2) hc ? go(k_fused<8, 2, 0, 1>) : go(k_fused<8, 2, 0, 2>);
+    else if(cube8) hc ? go(k_fused<8, -1, 0, 1>) : go(k_fused<8, -1, 0, 2>);
     else hc ? go(k_fused<0, -1, 0, 1>) : go(k_fused<0, -1, 0, 2>);
   } else if(w->fused_wg == 256 && cube8 && !w->fused_queue) {
     auto go256 = [&](auto kern) {
