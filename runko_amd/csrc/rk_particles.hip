@@ -6,24 +6,26 @@
 // Reference semantics followed (hel-astro-lab/runko v5.0.0):
 //   interpolation  src/runko/emf/yee_lattice_interpolate_linear_1st.h:29-138
 //   Boris          src/runko/pic/particle_boris.h:15-62
-//   Higuera-Cary   src/runko/pic/particle_higuera_cary.h:15-77 (unfused push only)
-//   Faraday        src/runko/pic/particle_faraday.h:37-109 (unfused push only)
+//   Higuera-Cary   src/runko/pic/particle_higuera_cary.h:15-77
+//   Faraday        src/runko/pic/particle_faraday.h:37-109
 //   zigzag         src/runko/pic/particle_current_zigzag_1st.c++:225-339
 //   divide         src/runko/pic/particle.c++:198-348
 //   append/wrap    src/runko/pic/particle.h:449-572
 //   sort key       src/runko/pic/tile.c++:417-438
 //   kinetic energy src/runko/pic/particle.c++:351-377
 //
-// Execution model: one workgroup (256 lanes, 4 waves) per particle bin
-// (brick).  The brick's node-centred E/B means (brick + 1 node) are staged in
-// LDS once; every particle of the brick gathers its 8 corners from LDS.  The
-// brick's current is accumulated with LDS float atomics over brick + (1,2)
-// nodes and flushed once to HBM with global float atomics.  A particle whose
-// stencil leaves the staged window (possible only on the plain per-tile path,
-// e.g. two pushes without a pack) takes the same arithmetic through global
-// memory.  Leavers are written to SoA migration records reserved with one
-// atomic per wave (ballot + popcount) and their slots are refilled from the
-// bin tail (swap-remove) so bins stay compact.
+// Execution model of the per-tile methods: one workgroup (256 lanes, 4
+// waves) per particle bin (brick).  The brick's node-centred E/B means (brick
+// + 1 node) are staged in LDS once; every particle of the brick gathers its 8
+// corners from LDS.  The deposit accumulates the brick's current in an FP64
+// LDS window over brick + (1,2) nodes and adds it to J once.  A particle
+// whose stencil leaves the staged window (possible only on the plain per-tile
+// path, e.g. two pushes without a pack) takes the same arithmetic through
+// global memory.  Leavers are written to SoA migration records reserved with
+// one atomic per wave (ballot + popcount) and their slots are refilled from
+// the bin tail (swap-remove) so bins stay compact.  The fused step (k_fused)
+// does push, walls, classification, deposit and in-order compaction of a
+// brick in one workgroup.
 #include "rk_internal.h"
 #include "rk_reflect.h"
 
@@ -343,6 +345,14 @@ __device__ inline float two_over(const Geo& g, float t)
 {
   if((FAST > 1 || (FAST == 0 && g.fast_sqrt)) && t >= 1.0f && t <= 0x1p60f) return div_near(2.0f, t);
   return 2.0f / t;
+}
+
+// 1 / t for t = 1 + b.b (particle_faraday.h)
+template<int FAST = 0>
+__device__ inline float one_over(const Geo& g, float t)
+{
+  if((FAST > 1 || (FAST == 0 && g.fast_sqrt)) && t >= 1.0f && t <= 0x1p60f) return div_near(1.0f, t);
+  return 1.0f / t;
 }
 
 template<int FAST = 0>
@@ -762,6 +772,7 @@ __device__ inline void cross3p(const float a[3], const float b[3], float r[3])
 
 // push_particles_higuera_cary, src/runko/pic/particle_higuera_cary.h:34-73,
 // expression order as written there (scalar * vector is v(i) * s)
+template<int FAST = 0>
 __device__ inline void higuera_cary(const Geo& g, float qm, const float E[3], const float B[3], float pos[3],
                                     float vel[3])
 {
@@ -782,12 +793,12 @@ __device__ inline void higuera_cary(const Geo& g, float qm, const float E[3], co
   const float bdotu = dot3p(Bt, u0);
   const float gmb = 1.0f + u0sq * cinv2 - b2 * cinv2;
   const float disc = gmb * gmb + 4.0f * (b2 * cinv2 + bdotu * bdotu * cinv2);
-  const float ginv = 1.0f / sqrtf(0.5f * (gmb + sqrtf(disc)));
+  const float ginv = inv_sqrt<FAST>(g, 0.5f * (gmb + sqrtf(disc)));
   const float gc = ginv * cinv;
   float B0[3];
 #pragma unroll
   for(int c = 0; c < 3; ++c) B0[c] = Bt[c] * gc;
-  const float f = 2.0f / (1.0f + gc * gc * b2);
+  const float f = two_over<FAST>(g, 1.0f + gc * gc * b2);
   float cr[3], u1[3], u2[3];
   cross3p(u0, B0, cr);
 #pragma unroll
@@ -795,7 +806,7 @@ __device__ inline void higuera_cary(const Geo& g, float qm, const float E[3], co
   cross3p(u1, B0, cr);
 #pragma unroll
   for(int c = 0; c < 3; ++c) u2[c] = (u0[c] + cr[c]) + E0[c];
-  const float ginv2 = cfl / sqrtf(cfl2 + dot3p(u2, u2));
+  const float ginv2 = cfl_over_sqrt<FAST>(g, cfl2 + dot3p(u2, u2));
 #pragma unroll
   for(int c = 0; c < 3; ++c) {
     vel[c] = u2[c] * cinv;
@@ -805,6 +816,7 @@ __device__ inline void higuera_cary(const Geo& g, float qm, const float E[3], co
 
 // push_particles_faraday, src/runko/pic/particle_faraday.h:54-104 (Cayley
 // transform of the field bivector, Sherman-Morrison reduction)
+template<int FAST = 0>
 __device__ inline void faraday(const Geo& g, float qm, const float E[3], const float B[3], float pos[3],
                                float vel[3])
 {
@@ -831,7 +843,7 @@ __device__ inline void faraday(const Geo& g, float qm, const float E[3], const f
 #pragma unroll
   for(int c = 0; c < 3; ++c) W[c] = ((v0[c] + eps[c] * gcfl) + cr[c]) + eps[c] * w0;
   const float b2 = dot3p(beta, beta);
-  const float f = 1.0f / (1.0f + b2);
+  const float f = one_over<FAST>(g, 1.0f + b2);
   cross3p(beta, W, cr);
   const float bW = dot3p(beta, W);
   float Wr[3];
@@ -848,8 +860,8 @@ __device__ inline void faraday(const Geo& g, float qm, const float E[3], const f
 #pragma unroll
   for(int c = 0; c < 3; ++c) u2[c] = Wr[c] + er[c] * sc;
 #pragma unroll
-  for(int c = 0; c < 3; ++c) vel[c] = u2[c] / cfl;
-  const float ginv2 = cfl / sqrtf(cfl * cfl + dot3p(u2, u2));
+  for(int c = 0; c < 3; ++c) vel[c] = div_cfl<FAST>(g, u2[c]);
+  const float ginv2 = cfl_over_sqrt<FAST>(g, cfl * cfl + dot3p(u2, u2));
 #pragma unroll
   for(int c = 0; c < 3; ++c) pos[c] = pos[c] + vel[c] * ginv2 * cfl;
 }
@@ -1475,8 +1487,8 @@ __device__ __forceinline__ void fused_brick(const Geo& g, const SpParams& sp, co
         if(!gather6(S, ldsM, ci[0] + moff[0], ci[1] + moff[1], ci[2] + moff[2], dd, E, Bf))
           gather(g, f, MeanWin {{0, 0, 0}, {0, 0, 0}}, nullptr, Bk.toffh[0] + ci[0], Bk.toffh[1] + ci[1],
                  Bk.toffh[2] + ci[2], dd, E, Bf, &counters[C_STENCIL]);
-        if constexpr(PUSHER == 1) higuera_cary(g, qm, E, Bf, pos, vel);
-        else if constexpr(PUSHER == 2) faraday(g, qm, E, Bf, pos, vel);
+        if constexpr(PUSHER == 1) higuera_cary<FAST>(g, qm, E, Bf, pos, vel);
+        else if constexpr(PUSHER == 2) faraday<FAST>(g, qm, E, Bf, pos, vel);
         else boris<FAST>(g, qm, E, Bf, pos, vel);
         if(WALL && wmask) {
           float* const Jc[3] = {Jc0, Jc1, Jc2};
@@ -1858,7 +1870,7 @@ unsigned xcd_grid(int n, int span)
   return unsigned((groups + 7) / 8 * 8 * span);
 }
 
-// Compare the fast forms of x/cfl, cfl/sqrt(X), 2/t and 1/sqrt(t) with the
+// Compare the fast forms of x/cfl, cfl/sqrt(X), 2/t, 1/t and 1/sqrt(t) with the
 // IEEE operations over every float of their fast ranges; mismatches summed.
 __global__ void __launch_bounds__(256) k_verify_fast(float cfl, float rcp, uint32_t* mismatches)
 {
@@ -1875,6 +1887,7 @@ __global__ void __launch_bounds__(256) k_verify_fast(float cfl, float rcp, uint3
       bad2 += __float_as_uint(div_near(cfl, sqrt_near(x))) != __float_as_uint(cfl / sqrtf(x)) ? 1u : 0u;
     if(x >= 1.0f && x <= 0x1p60f) {
       bad2 += __float_as_uint(div_near(2.0f, x)) != __float_as_uint(2.0f / x) ? 1u : 0u;
+      bad2 += __float_as_uint(div_near(1.0f, x)) != __float_as_uint(1.0f / x) ? 1u : 0u;
       bad2 += __float_as_uint(div_near(1.0f, sqrt_near(x))) != __float_as_uint(1.0f / sqrtf(x)) ? 1u : 0u;
     }
   }
