@@ -116,7 +116,8 @@ __device__ inline void stencil_load(const Geo& g, float (&v)[SROWS][SLANES], int
                                     const float* __restrict__ Ex, const float* __restrict__ Ey,
                                     const float* __restrict__ Ez)
 {
-  const int wave = (int(threadIdx.y) * BX + int(threadIdx.x)) / 64, lane = int(threadIdx.x) % 64;
+  const int wave = __builtin_amdgcn_readfirstlane((int(threadIdx.y) * BX + int(threadIdx.x)) / 64);
+  const int lane = int(threadIdx.x) % 64;
   const bool xin = gi >= 0 && gi < g.L[0];
 #pragma unroll
   for(int q = 0; q < SROWS; ++q) {
@@ -133,7 +134,8 @@ __device__ inline void stencil_load(const Geo& g, float (&v)[SROWS][SLANES], int
 
 __device__ inline void stencil_store(float (&E)[3][6 * SPL], int slot, const float (&v)[SROWS][SLANES])
 {
-  const int wave = (int(threadIdx.y) * BX + int(threadIdx.x)) / 64, lane = int(threadIdx.x) % 64;
+  const int wave = __builtin_amdgcn_readfirstlane((int(threadIdx.y) * BX + int(threadIdx.x)) / 64);
+  const int lane = int(threadIdx.x) % 64;
 #pragma unroll
   for(int q = 0; q < SROWS; ++q) {
     const int row = wave + SWAVES * q, c = row / SSY, y = row % SSY;
