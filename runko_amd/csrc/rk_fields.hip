@@ -253,43 +253,52 @@ __global__ void __launch_bounds__(BX* BY)
 // 27-tap (1,2,1)^3/64, taps in layout_right order, accumulator starting at 0.
 // Same 27-tap filter, each lane sweeping a column of FCH outputs along x: the
 // 3x3 (y, z) neighbourhood of three consecutive x planes stays in registers,
-// so every input is loaded once per column instead of three times.  Each
-// output still accumulates its 27 taps in layout_right order from 0.
+// so every input is loaded once per column instead of three times.  The
+// three components share the sweep, so a plane's 27 loads are in flight
+// together.  Each output still accumulates its 27 taps in layout_right order
+// from 0.
 constexpr int FCH = 8;
 
-__device__ inline void filter_column(const Geo& g, const float* __restrict__ S, float* __restrict__ D, int i0, int i1,
-                                     int j, int k)
+__device__ inline void filter_columns(const Geo& g, const float* const (&S)[3], float* const (&D)[3], int i0, int i1,
+                                      int j, int k)
 {
-  const uint32_t si = uint32_t(g.L[1]) * uint32_t(g.L[2]), sj = uint32_t(g.L[2]);
-  float P[3][3][3];  // [plane i-1, i, i+1][b][c]
-  auto load = [&](float(&Q)[3][3], int i) {
+  const uint32_t sj = uint32_t(g.L[2]);
+  float P[3][3][3][3];  // [component][plane i-1, i, i+1][b][c]
+  auto load = [&](int p, int i) {
     const uint32_t c0 = g.at(i, j, k);
 #pragma unroll
-    for(int b = 0; b < 3; ++b)
-#pragma unroll
-      for(int cc = 0; cc < 3; ++cc) Q[b][cc] = S[c0 + uint32_t(b - 1) * sj + uint32_t(cc - 1)];
-  };
-  load(P[0], i0 - 1);
-  load(P[1], i0);
-  (void)si;
-  for(int i = i0; i < i1; ++i) {
-    load(P[2], i + 1);
-    const float w[3] = {1.0f, 2.0f, 1.0f};
-    float acc = 0.0f;
-#pragma unroll
-    for(int a = 0; a < 3; ++a)
+    for(int q = 0; q < 3; ++q)
 #pragma unroll
       for(int b = 0; b < 3; ++b)
 #pragma unroll
-        for(int cc = 0; cc < 3; ++cc) acc = acc + ((w[a] * w[b] * w[cc]) * (1.0f / 64.0f)) * P[a][b][cc];
-    D[g.at(i, j, k)] = acc;
+        for(int cc = 0; cc < 3; ++cc) P[q][p][b][cc] = S[q][c0 + uint32_t(b - 1) * sj + uint32_t(cc - 1)];
+  };
+  load(0, i0 - 1);
+  load(1, i0);
+  for(int i = i0; i < i1; ++i) {
+    load(2, i + 1);
+    const float w[3] = {1.0f, 2.0f, 1.0f};
+    const uint32_t c = g.at(i, j, k);
 #pragma unroll
-    for(int b = 0; b < 3; ++b)
+    for(int q = 0; q < 3; ++q) {
+      float acc = 0.0f;
 #pragma unroll
-      for(int cc = 0; cc < 3; ++cc) {
-        P[0][b][cc] = P[1][b][cc];
-        P[1][b][cc] = P[2][b][cc];
-      }
+      for(int a = 0; a < 3; ++a)
+#pragma unroll
+        for(int b = 0; b < 3; ++b)
+#pragma unroll
+          for(int cc = 0; cc < 3; ++cc) acc = acc + ((w[a] * w[b] * w[cc]) * (1.0f / 64.0f)) * P[q][a][b][cc];
+      D[q][c] = acc;
+    }
+#pragma unroll
+    for(int q = 0; q < 3; ++q)
+#pragma unroll
+      for(int b = 0; b < 3; ++b)
+#pragma unroll
+        for(int cc = 0; cc < 3; ++cc) {
+          P[q][0][b][cc] = P[q][1][b][cc];
+          P[q][1][b][cc] = P[q][2][b][cc];
+        }
   }
 }
 
@@ -302,9 +311,9 @@ __global__ void __launch_bounds__(BX* BY)
   const int i0 = r.r0[0] + int(blockIdx.z) * FCH;
   const int i1 = min(i0 + FCH, r.r1[0]);
   if(k >= r.r1[2] || j >= r.r1[1]) return;
-  filter_column(g, S0, D0, i0, i1, j, k);
-  filter_column(g, S1, D1, i0, i1, j, k);
-  filter_column(g, S2, D2, i0, i1, j, k);
+  const float* const S[3] = {S0, S1, S2};
+  float* const D[3] = {D0, D1, D2};
+  filter_columns(g, S, D, i0, i1, j, k);
 }
 
 // separable z, y, x passes (0.25, 0.5, 0.25), each output evaluated with the
@@ -326,15 +335,26 @@ __device__ inline float sep_zy(const Geo& g, const float* __restrict__ S, int i,
   return 0.25f * y0 + 0.5f * y1 + 0.25f * y2;
 }
 
-__device__ inline void sep_column(const Geo& g, const float* __restrict__ S, float* __restrict__ D, int i0, int i1,
-                                  int j, int k)
+__device__ inline void sep_columns(const Geo& g, const float* const (&S)[3], float* const (&D)[3], int i0, int i1,
+                                   int j, int k)
 {
-  float t0 = sep_zy(g, S, i0 - 1, j, k), t1 = sep_zy(g, S, i0, j, k);
+  float t0[3], t1[3];
+#pragma unroll
+  for(int q = 0; q < 3; ++q) {
+    t0[q] = sep_zy(g, S[q], i0 - 1, j, k);
+    t1[q] = sep_zy(g, S[q], i0, j, k);
+  }
   for(int i = i0; i < i1; ++i) {
-    const float t2 = sep_zy(g, S, i + 1, j, k);
-    D[g.at(i, j, k)] = 0.25f * t0 + 0.5f * t1 + 0.25f * t2;
-    t0 = t1;
-    t1 = t2;
+    float t2[3];
+#pragma unroll
+    for(int q = 0; q < 3; ++q) t2[q] = sep_zy(g, S[q], i + 1, j, k);
+    const uint32_t c = g.at(i, j, k);
+#pragma unroll
+    for(int q = 0; q < 3; ++q) {
+      D[q][c] = 0.25f * t0[q] + 0.5f * t1[q] + 0.25f * t2[q];
+      t0[q] = t1[q];
+      t1[q] = t2[q];
+    }
   }
 }
 
@@ -348,9 +368,9 @@ __global__ void __launch_bounds__(BX* BY)
   const int i0 = r.r0[0] + int(blockIdx.z) * FCH;
   const int i1 = min(i0 + FCH, r.r1[0]);
   if(k >= r.r1[2] || j >= r.r1[1]) return;
-  sep_column(g, S0, D0, i0, i1, j, k);
-  sep_column(g, S1, D1, i0, i1, j, k);
-  sep_column(g, S2, D2, i0, i1, j, k);
+  const float* const S[3] = {S0, S1, S2};
+  float* const D[3] = {D0, D1, D2};
+  sep_columns(g, S, D, i0, i1, j, k);
 }
 
 __device__ inline int wrap_idx(int c, int N) { return c < HALO ? c + N : (c >= N + HALO ? c - N : c); }
