@@ -376,31 +376,46 @@ __global__ void __launch_bounds__(BX* BY)
 __device__ inline int wrap_idx(int c, int N) { return c < HALO ? c + N : (c >= N + HALO ? c - N : c); }
 
 // periodic ghost copy: every ghost node takes the value of its interior image
-__global__ void __launch_bounds__(256) k_ghost_copy(Geo g, float* F0, float* F1, float* F2)
+// (sources are interior nodes, destinations ghost nodes, so the copies are
+// independent).  k_ghost_rows copies the whole z rows whose i or j lies in
+// the ghost shell: 2*HALO*L1 rows with a ghost i, then N0*2*HALO rows with an
+// interior i and a ghost j, one workgroup per row.  k_ghost_z copies the z
+// ghosts of the interior rows, one lane per ghost node.
+__global__ void __launch_bounds__(64) k_ghost_rows(Geo g, float* F0, float* F1, float* F2)
 {
-  const int j = int(blockIdx.x);
-  const int i = int(blockIdx.y);
-  const bool gi = (i < HALO || i >= g.N[0] + HALO);
-  const bool gj = (j < HALO || j >= g.N[1] + HALO);
-  const int Lz = g.L[2];
-  const int si = wrap_idx(i, g.N[0]), sj = wrap_idx(j, g.N[1]);
-  if(gi || gj) {
-    for(int k = int(threadIdx.x); k < Lz; k += int(blockDim.x)) {
-      const uint32_t d = g.at(i, j, k), s = g.at(si, sj, wrap_idx(k, g.N[2]));
-      F0[d] = F0[s];
-      F1[d] = F1[s];
-      F2[d] = F2[s];
-    }
+  int b = int(blockIdx.x), i, j;
+  const int nA = 2 * HALO * g.L[1];
+  if(b < nA) {
+    const int ii = b / g.L[1];
+    j = b - ii * g.L[1];
+    i = ii < HALO ? ii : g.N[0] + ii;
   } else {
-    const int t = int(threadIdx.x);
-    if(t < 2 * HALO) {
-      const int k = t < HALO ? t : g.N[2] + t;
-      const uint32_t d = g.at(i, j, k), s = g.at(i, j, wrap_idx(k, g.N[2]));
-      F0[d] = F0[s];
-      F1[d] = F1[s];
-      F2[d] = F2[s];
-    }
+    b -= nA;
+    const int jj = b % (2 * HALO);
+    i = HALO + b / (2 * HALO);
+    j = jj < HALO ? jj : g.N[1] + jj;
   }
+  const int si = wrap_idx(i, g.N[0]), sj = wrap_idx(j, g.N[1]);
+  for(int k = int(threadIdx.x); k < g.L[2]; k += int(blockDim.x)) {
+    const uint32_t d = g.at(i, j, k), s = g.at(si, sj, wrap_idx(k, g.N[2]));
+    F0[d] = F0[s];
+    F1[d] = F1[s];
+    F2[d] = F2[s];
+  }
+}
+
+__global__ void __launch_bounds__(256) k_ghost_z(Geo g, float* F0, float* F1, float* F2)
+{
+  const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if(t >= int64_t(g.N[0]) * g.N[1] * 2 * HALO) return;
+  const int h = int(t % (2 * HALO));
+  const int64_t row = t / (2 * HALO);
+  const int i = HALO + int(row / g.N[1]), j = HALO + int(row % g.N[1]);
+  const int k = h < HALO ? h : g.N[2] + h;
+  const uint32_t d = g.at(i, j, k), s = g.at(i, j, wrap_idx(k, g.N[2]));
+  F0[d] = F0[s];
+  F1[d] = F1[s];
+  F2[d] = F2[s];
 }
 
 // images of an interior coordinate c in the ghost shell: c-N (c >= N) and c+N (c < 2*HALO)
@@ -576,7 +591,11 @@ int launch_filter(rk_world* w, const int r0[3], const int r1[3], hipStream_t s)
 
 int launch_ghost_copy(rk_world* w, float* const F[3], hipStream_t s)
 {
-  hipLaunchKernelGGL(k_ghost_copy, dim3(w->g.L[1], w->g.L[0]), dim3(64), 0, s, w->g, F[0], F[1], F[2]);
+  const Geo& g = w->g;
+  hipLaunchKernelGGL(k_ghost_rows, dim3(unsigned(2 * HALO * g.L[1] + g.N[0] * 2 * HALO)), dim3(64), 0, s, g, F[0],
+                     F[1], F[2]);
+  const int64_t nz = int64_t(g.N[0]) * g.N[1] * 2 * HALO;
+  hipLaunchKernelGGL(k_ghost_z, dim3(unsigned((nz + 255) / 256)), dim3(256), 0, s, g, F[0], F[1], F[2]);
   RK_HIP(hipGetLastError());
   return RK_OK;
 }
