@@ -81,11 +81,15 @@ def run_turbulence(args):
     npart = w.total_particles()
     ncell = N ** 3
     log(f"[bench] turbulence {N}^3, tiles {tile}^3, {npart} particles, brick {w.brick}, setup {time.time() - t0:.1f}s")
+    mem0 = w.memory_report()
+    log("[bench] pools: " + ", ".join(f"species {s}: {p['particles']} particles, {p['bins_used']} slots in bins, "
+                                      f"{p['capacity']} capacity ({p['gib']:.1f} GiB)"
+                                      for s, p in enumerate(mem0["species"])))
     lap = BatchedLap(w)
     for i in range(args.warmup):
         lap()
         lap.settle()
-        log(f"[bench] warmup step {i} done")
+        log(f"[bench] warmup step {i} done (moves {w.bin_moves}, relayouts {w.relayouts})")
     launch = None
     if not args.no_autotune:
         # launch form of the fused step, timed on this world (one lap each)
@@ -94,6 +98,7 @@ def run_turbulence(args):
         launch = dict(chosen=best, k_fused_ms=times)
         log(f"[bench] fused launch: {best} ({', '.join(f'{k} {v:.2f} ms' for k, v in times.items())})")
     torch.cuda.synchronize()
+    mem_t0 = w.memory_report()
     _lib.check(_lib.lib().rk_profile_enable(w.h, 1))
     ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     torch.cuda.synchronize()
@@ -106,6 +111,12 @@ def run_turbulence(args):
     torch.cuda.synchronize()
     wall = time.perf_counter() - t_start
     _lib.check(_lib.lib().rk_profile_enable(w.h, 0))
+    mem = w.memory_report()
+    mem["timed"] = dict(repacks=mem["repacks"] - mem_t0["repacks"], bin_moves=mem["bin_moves"] - mem_t0["bin_moves"],
+                        relayouts=mem["relayouts"] - mem_t0["relayouts"],
+                        settle_ms_per_step=1e3 * (mem["settle_seconds"] - mem_t0["settle_seconds"]) / args.steps)
+    log(f"[bench] pools after the timed laps: {mem['bin_moves']} bin moves, {mem['relayouts']} relayouts, "
+        f"{mem['new_pools']} new pools; in the timed laps {mem['timed']}")
     import numpy as np
     ms = np.zeros(9)
     cnt = np.zeros(9, dtype=np.int64)
@@ -142,7 +153,7 @@ def run_turbulence(args):
         ms_per_step=1e3 * wall / args.steps, prof=prof, fused_ms=float(fused_ms), alg_bytes=alg_bytes,
         achieved_gbs=achieved, field_ms_per_step=field_ms / args.steps,
         cell_updates_per_s=ncell * args.steps / (field_ms * 1e-3) if field_ms > 0 else None,
-        error_counters=[int(v) for v in err], brick=list(w.brick), B0=info["B0"], plain=plain,
+        error_counters=[int(v) for v in err], brick=list(w.brick), B0=info["B0"], plain=plain, pools=mem,
         autotune=dict(brick=tuned, fused_launch=launch))
 
 
@@ -672,6 +683,7 @@ def main():
         "fdtd_cell_updates_per_s": res["cell_updates_per_s"],
         "kernel_ms": res["prof"],
         "error_counters": res["error_counters"],
+        "pools": res["pools"],
     }
     if res["plain"] is not None:
         line["batched_vs_plain"] = dict(workload=line["config"]["workload"].replace(", batched form", ""),

@@ -260,6 +260,23 @@ int rk_bin_demand(rk_world* w, int32_t s, uint32_t* demand_dev, void* stream);
 int rk_repack(rk_world* w, int32_t s, int64_t new_capacity, float* const pos[3], float* const vel[3],
               uint64_t* id, int64_t* bin_start, uint32_t* bin_cap, void* stream);
 int rk_retry_append(rk_world* w, void* stream);
+/* Moves n listed bins of species s (device int32 bin indices) to new starts
+ * and capacities (device int64 / uint32, one per listed bin) inside the
+ * borrowed pool; the targets must be disjoint from every bin (the pool's
+ * spare tail).  Buffer addresses stay, so captured graphs stay valid.
+ * Replaces the reference's container growth on append
+ * (src/runko/pic/particle.h:471-509) for bricks that outgrow their bin. */
+int rk_move_bins(rk_world* w, int32_t s, int32_t n, const int32_t* bins, const int64_t* new_start,
+                 const uint32_t* new_cap, void* stream);
+/* In-place relayout of species s (synchronous): every bin b gets capacity
+ * new_cap[b] (host uint32[n_bricks], >= its stored count), bins packed in the
+ * order of their current starts; *new_start (host int64[n_bricks]) receives
+ * the new starts.  tmp is a caller device buffer of tmp_bytes (at least the
+ * tables, 28 B per brick, plus 32 B per particle of the largest bin; more
+ * means fewer passes).  RK_EOVERFLOW if the capacities exceed the pool.
+ * Also the in-place counterpart of particle.h:471-509: no second pool. */
+int rk_relayout(rk_world* w, int32_t s, const uint32_t* new_cap, void* tmp, int64_t tmp_bytes,
+                int64_t* new_start, void* stream);
 
 /* Copies of one tile's alive particles of species s in bin order into caller
  * device buffers (capacity >= count); *count_out set synchronously. */

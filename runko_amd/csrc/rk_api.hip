@@ -216,7 +216,9 @@ int rk_world_create(rk_world** out, const rk_world_desc* d)
     maxq = std::max(maxq, std::fabs(double(w->sp[s].charge)));
   }
   // every contribution is |q| * (a move < 1 cell) * weights <= 1: scale so that
-  // max|q| * jscale <= 2^40 (sums of 2^15 contributions stay far below 2^53)
+  // max|q| * jscale <= 2^40.  The int64 sums are exact up to 2^63 (2^23
+  // full-size contributions per node); converting a sum back to FP32 rounds
+  // once, like any FP32 result
   {
     int e = 0;
     if(maxq > 0.0 && std::isfinite(maxq)) (void)std::frexp(maxq, &e);  // maxq < 2^e
@@ -660,6 +662,22 @@ int rk_repack(rk_world* w, int32_t s, int64_t new_capacity, float* const pos[3],
   if(int rc = launch_repack(w, s, n, S(stream))) return rc;
   w->sp[s] = n;
   return RK_OK;
+}
+
+int rk_move_bins(rk_world* w, int32_t s, int32_t n, const int32_t* bins, const int64_t* new_start,
+                 const uint32_t* new_cap, void* stream)
+{
+  if(!w || s < 0 || s >= w->d.n_species) return fail(RK_EINVAL, "invalid species");
+  if(n < 0 || (n > 0 && (!bins || !new_start || !new_cap))) return fail(RK_EINVAL, "invalid bin list");
+  return launch_move_bins(w, s, n, bins, new_start, new_cap, S(stream));
+}
+
+int rk_relayout(rk_world* w, int32_t s, const uint32_t* new_cap, void* tmp, int64_t tmp_bytes, int64_t* new_start,
+                void* stream)
+{
+  if(!w || s < 0 || s >= w->d.n_species) return fail(RK_EINVAL, "invalid species");
+  if(!new_cap || !new_start) return fail(RK_EINVAL, "null argument");
+  return relayout_species(w, s, new_cap, tmp, tmp_bytes, new_start, S(stream));
 }
 
 int rk_retry_append(rk_world* w, void* stream)

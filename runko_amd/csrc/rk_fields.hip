@@ -615,6 +615,17 @@ int launch_fill(float* const F[3], int64_t n, float v, hipStream_t s)
   return RK_OK;
 }
 
+int launch_field_energy_partials(rk_world* w, const int r0[3], const int r1[3], double* partial, hipStream_t s)
+{
+  const Region r = make_region(r0, r1);
+  if(empty(r)) return RK_OK;
+  const int planes = r.r1[0] - r.r0[0];
+  hipLaunchKernelGGL(k_energy, dim3(1, planes), dim3(256), 0, s, w->g, r, w->E[0], w->E[1], w->E[2], w->B[0],
+                     w->B[1], w->B[2], partial);
+  RK_HIP(hipGetLastError());
+  return RK_OK;
+}
+
 int field_energies(rk_world* w, const int r0[3], const int r1[3], double out[2], hipStream_t s)
 {
   const Region r = make_region(r0, r1);
@@ -622,9 +633,7 @@ int field_energies(rk_world* w, const int r0[3], const int r1[3], double out[2],
   if(empty(r)) return RK_OK;
   const int planes = r.r1[0] - r.r0[0];
   if(w->red_cap < 2 * planes) return (set_error("reduction scratch too small"), RK_EINVAL);
-  hipLaunchKernelGGL(k_energy, dim3(1, planes), dim3(256), 0, s, w->g, r, w->E[0], w->E[1], w->E[2], w->B[0],
-                     w->B[1], w->B[2], w->red);
-  RK_HIP(hipGetLastError());
+  if(int rc = launch_field_energy_partials(w, r0, r1, w->red, s)) return rc;
   std::vector<double> h(size_t(2 * planes));
   RK_HIP(hipMemcpyAsync(h.data(), w->red, sizeof(double) * h.size(), hipMemcpyDeviceToHost, s));
   RK_HIP(hipStreamSynchronize(s));

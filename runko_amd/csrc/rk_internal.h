@@ -259,8 +259,16 @@ int launch_inject(rk_world* w, int tile, int s, int64_t n, const float* const po
                   const float* const vel[3], const uint64_t* id, hipStream_t st);
 int launch_retry(rk_world* w, hipStream_t s);
 int launch_repack(rk_world* w, int s, const rk::Species& nsp, hipStream_t st);
+int launch_move_bins(rk_world* w, int s, int n, const int32_t* bins, const int64_t* new_start,
+                     const uint32_t* new_cap, hipStream_t st);
+int relayout_species(rk_world* w, int s, const uint32_t* new_cap, void* tmp, int64_t tmp_bytes,
+                     int64_t* new_start, hipStream_t st);
 int launch_gather_tile(rk_world* w, int tile, int s, float* const pos[3], float* const vel[3], uint64_t* id,
                        const int64_t* offsets_dev, hipStream_t st);
 int gauss_residual(rk_world* w, int passes, double* out, hipStream_t s);
+// energy partial sums into a caller buffer (rk_energies and its allocation-free form)
+int launch_field_energy_partials(rk_world* w, const int r0[3], const int r1[3], double* partial, hipStream_t s);
+int kinetic_partial_count(rk_world* w, int tile);
+int launch_kinetic_partials(rk_world* w, int tile, int s, double* partial, hipStream_t st);
 
 }  // namespace rk

@@ -1790,6 +1790,55 @@ __global__ void __launch_bounds__(WG)
   }
 }
 
+// The seven arrays of a particle store (pool or staging buffer).
+struct PoolPtrs {
+  float* f[6];  // x y z ux uy uz
+  uint64_t* id;
+};
+
+// Segment copies: segment i moves len[i] particles from src at
+// src_off[i] - src_base to dst at dst_off[i] - dst_base.  One workgroup per
+// segment.  The source and destination ranges of one launch are disjoint.
+__global__ void __launch_bounds__(WG)
+  k_copy_segments(PoolPtrs src, PoolPtrs dst, const int64_t* __restrict__ src_off, int64_t src_base,
+                  const int64_t* __restrict__ dst_off, int64_t dst_base, const uint32_t* __restrict__ len, int nseg)
+{
+  for(int i = int(blockIdx.x); i < nseg; i += int(gridDim.x)) {
+    const int64_t so = src_off[i] - src_base, d0 = dst_off[i] - dst_base;
+    const uint32_t n = len[i];
+    for(uint32_t t = threadIdx.x; t < n; t += WG) {
+#pragma unroll
+      for(int d = 0; d < 6; ++d) dst.f[d][d0 + t] = src.f[d][so + t];
+      dst.id[d0 + t] = src.id[so + t];
+    }
+  }
+}
+
+// Moves the listed bins of one species to new places (disjoint from every
+// bin: the pool's spare arena).  A full bin keeps min(count, cap) particles;
+// its pending records are re-appended afterwards.
+__global__ void __launch_bounds__(WG)
+  k_move_bins(PoolPtrs p, int64_t* bin_start, uint32_t* bin_cap, uint32_t* bin_count, const int32_t* __restrict__ bins,
+              const int64_t* __restrict__ new_start, const uint32_t* __restrict__ new_cap, int n)
+{
+  for(int i = int(blockIdx.x); i < n; i += int(gridDim.x)) {
+    const int b = bins[i];
+    const int64_t so = bin_start[b], no = new_start[i];
+    const uint32_t c = min(bin_count[b], bin_cap[b]);
+    for(uint32_t t = threadIdx.x; t < c; t += WG) {
+#pragma unroll
+      for(int d = 0; d < 6; ++d) p.f[d][no + t] = p.f[d][so + t];
+      p.id[no + t] = p.id[so + t];
+    }
+    __syncthreads();
+    if(threadIdx.x == 0) {
+      bin_start[b] = no;
+      bin_cap[b] = new_cap[i];
+      bin_count[b] = c;
+    }
+  }
+}
+
 __global__ void __launch_bounds__(WG)
   k_gather_tile(SpParams sp, int s, int bin0, int nbins, const int64_t* offs, float* ox, float* oy, float* oz,
                 float* ux, float* uy, float* uz, uint64_t* oid)
@@ -2144,6 +2193,153 @@ int launch_repack(rk_world* w, int s, const Species& nsp, hipStream_t st)
   return RK_OK;
 }
 
+namespace {
+
+PoolPtrs pool_ptrs(const Species& p)
+{
+  PoolPtrs q {};
+  for(int d = 0; d < 3; ++d) {
+    q.f[d] = p.pos[d];
+    q.f[3 + d] = p.vel[d];
+  }
+  q.id = p.id;
+  return q;
+}
+
+}  // namespace
+
+int launch_move_bins(rk_world* w, int s, int n, const int32_t* bins, const int64_t* new_start, const uint32_t* new_cap,
+                     hipStream_t st)
+{
+  if(n <= 0) return RK_OK;
+  const Species& p = w->sp[s];
+  hipLaunchKernelGGL(k_move_bins, dim3(unsigned(std::min(n, 8192))), dim3(WG), 0, st, pool_ptrs(p), p.bin_start,
+                     p.bin_cap, p.bin_count, bins, new_start, new_cap, n);
+  RK_HIP(hipGetLastError());
+  return RK_OK;
+}
+
+// In-place relayout of species s (synchronous).  Bins are taken in the order
+// of their current starts.  Pass 1 packs them tightly (bin i at t_i = sum of
+// the stored counts before it, t_i <= its old start); pass 2 spreads them to
+// f_i = sum of the new capacities before it (f_i >= t_i).  Each pass moves
+// batches of whole bins through the staging buffer: pass 1 front to back
+// (a batch's target lies below every source not yet read), pass 2 back to
+// front (a batch's target lies above every source not yet read), so no
+// particle is overwritten before it has been copied.
+int relayout_species(rk_world* w, int s, const uint32_t* new_cap, void* tmp, int64_t tmp_bytes, int64_t* new_start,
+                     hipStream_t st)
+{
+  Species& p = w->sp[s];
+  const int nb = w->g.nbricks;
+  const size_t nbs = static_cast<size_t>(nb);
+  std::vector<int64_t> start(nbs, 0);
+  std::vector<uint32_t> cap(nbs, 0), cnt(nbs, 0);
+  RK_HIP(hipMemcpyAsync(start.data(), p.bin_start, sizeof(int64_t) * size_t(nb), hipMemcpyDeviceToHost, st));
+  RK_HIP(hipMemcpyAsync(cap.data(), p.bin_cap, sizeof(uint32_t) * size_t(nb), hipMemcpyDeviceToHost, st));
+  RK_HIP(hipMemcpyAsync(cnt.data(), p.bin_count, sizeof(uint32_t) * size_t(nb), hipMemcpyDeviceToHost, st));
+  RK_HIP(hipStreamSynchronize(st));
+  std::vector<int32_t> order(nbs);
+  for(int b = 0; b < nb; ++b) order[size_t(b)] = b;
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return start[size_t(a)] < start[size_t(b)]; });
+  // sorted tables: old start, tight start, final start, stored count
+  std::vector<int64_t> so(nbs), to(nbs), fo(nbs);
+  std::vector<uint32_t> n(nbs);
+  int64_t t = 0, f = 0, end = 0;
+  uint32_t nmax = 0;
+  for(int i = 0; i < nb; ++i) {
+    const int b = order[size_t(i)];
+    const uint32_t stored = std::min(cnt[size_t(b)], cap[size_t(b)]);
+    if(cap[size_t(b)] > 0 && start[size_t(b)] < end) {
+      set_error("relayout: bins of species " + std::to_string(s) + " overlap");
+      return RK_ESTATE;
+    }
+    if(new_cap[b] < stored) {
+      set_error("relayout: new capacity of bin " + std::to_string(b) + " below its " + std::to_string(stored) +
+                " stored particles");
+      return RK_EINVAL;
+    }
+    end = std::max(end, start[size_t(b)] + int64_t(cap[size_t(b)]));
+    so[size_t(i)] = start[size_t(b)];
+    to[size_t(i)] = t;
+    fo[size_t(i)] = f;
+    n[size_t(i)] = stored;
+    t += stored;
+    f += new_cap[b];
+    nmax = std::max(nmax, stored);
+  }
+  if(end > p.capacity || f > p.capacity) {
+    set_error("relayout: species " + std::to_string(s) + " needs " + std::to_string(f) + " slots, pool holds " +
+              std::to_string(p.capacity));
+    return RK_EOVERFLOW;
+  }
+  // staging: the three offset tables, the count table, then [7][W] particles
+  const int64_t table = int64_t(nb) * (3 * sizeof(int64_t) + sizeof(uint32_t));
+  const int64_t tab_al = (table + 255) / 256 * 256;
+  const int64_t W = tmp_bytes > tab_al ? (tmp_bytes - tab_al) / 32 : 0;
+  if(!tmp || W < int64_t(nmax) || W < 1) {
+    set_error("relayout: staging buffer of " + std::to_string(tmp_bytes) + " bytes is too small");
+    return RK_EINVAL;
+  }
+  char* base = static_cast<char*>(tmp);
+  int64_t* d_so = reinterpret_cast<int64_t*>(base);
+  int64_t* d_to = d_so + nb;
+  int64_t* d_fo = d_to + nb;
+  uint32_t* d_n = reinterpret_cast<uint32_t*>(d_fo + nb);
+  RK_HIP(hipMemcpyAsync(d_so, so.data(), sizeof(int64_t) * size_t(nb), hipMemcpyHostToDevice, st));
+  RK_HIP(hipMemcpyAsync(d_to, to.data(), sizeof(int64_t) * size_t(nb), hipMemcpyHostToDevice, st));
+  RK_HIP(hipMemcpyAsync(d_fo, fo.data(), sizeof(int64_t) * size_t(nb), hipMemcpyHostToDevice, st));
+  RK_HIP(hipMemcpyAsync(d_n, n.data(), sizeof(uint32_t) * size_t(nb), hipMemcpyHostToDevice, st));
+  PoolPtrs stage {};
+  float* fbase = reinterpret_cast<float*>(base + tab_al);
+  for(int d = 0; d < 6; ++d) stage.f[d] = fbase + int64_t(d) * W;
+  stage.id = reinterpret_cast<uint64_t*>(fbase + 6 * W);
+  const PoolPtrs pool = pool_ptrs(p);
+  auto contiguous = [&](const PoolPtrs& src, int64_t s0, const PoolPtrs& dst, int64_t d0, int64_t len) -> int {
+    for(int d = 0; d < 6; ++d)
+      RK_HIP(hipMemcpyAsync(dst.f[d] + d0, src.f[d] + s0, sizeof(float) * size_t(len), hipMemcpyDeviceToDevice, st));
+    RK_HIP(hipMemcpyAsync(dst.id + d0, src.id + s0, sizeof(uint64_t) * size_t(len), hipMemcpyDeviceToDevice, st));
+    return RK_OK;
+  };
+  // pass 1, front to back; the leading bins already packed stay
+  int i0 = 0;
+  while(i0 < nb && so[size_t(i0)] == to[size_t(i0)]) ++i0;
+  while(i0 < nb) {
+    int i1 = i0;
+    int64_t T = 0;
+    while(i1 < nb && T + n[size_t(i1)] <= W) T += n[size_t(i1++)];
+    const int ns = i1 - i0;
+    hipLaunchKernelGGL(k_copy_segments, dim3(unsigned(std::min(ns, 8192))), dim3(WG), 0, st, pool, stage, d_so + i0,
+                       int64_t(0), d_to + i0, to[size_t(i0)], d_n + i0, ns);
+    RK_HIP(hipGetLastError());
+    if(int rc = contiguous(stage, 0, pool, to[size_t(i0)], T)) return rc;
+    i0 = i1;
+  }
+  // pass 2, back to front
+  int i1 = nb;
+  while(i1 > 0) {
+    int j = i1;
+    int64_t T = 0;
+    while(j > 0 && T + n[size_t(j - 1)] <= W) T += n[size_t(--j)];
+    const int ns = i1 - j;
+    if(int rc = contiguous(pool, to[size_t(j)], stage, 0, T)) return rc;
+    hipLaunchKernelGGL(k_copy_segments, dim3(unsigned(std::min(ns, 8192))), dim3(WG), 0, st, stage, pool, d_to + j,
+                       to[size_t(j)], d_fo + j, int64_t(0), d_n + j, ns);
+    RK_HIP(hipGetLastError());
+    i1 = j;
+  }
+  for(int i = 0; i < nb; ++i) {
+    const int b = order[size_t(i)];
+    new_start[b] = fo[size_t(i)];
+    cnt[size_t(b)] = n[size_t(i)];
+  }
+  RK_HIP(hipMemcpyAsync(p.bin_start, new_start, sizeof(int64_t) * size_t(nb), hipMemcpyHostToDevice, st));
+  RK_HIP(hipMemcpyAsync(p.bin_cap, new_cap, sizeof(uint32_t) * size_t(nb), hipMemcpyHostToDevice, st));
+  RK_HIP(hipMemcpyAsync(p.bin_count, cnt.data(), sizeof(uint32_t) * size_t(nb), hipMemcpyHostToDevice, st));
+  RK_HIP(hipStreamSynchronize(st));
+  return RK_OK;
+}
+
 int launch_gather_tile(rk_world* w, int tile, int s, float* const pos[3], float* const vel[3], uint64_t* id,
                        const int64_t* offs, hipStream_t st)
 {
@@ -2154,16 +2350,30 @@ int launch_gather_tile(rk_world* w, int tile, int s, float* const pos[3], float*
   return RK_OK;
 }
 
-int kinetic_energies(rk_world* w, int tile, double* out, hipStream_t st)
+int kinetic_partial_count(rk_world* w, int tile)
+{
+  int b0, nb;
+  bin_range(w, tile, b0, nb);
+  return std::min(nb, 1024);
+}
+
+int launch_kinetic_partials(rk_world* w, int tile, int s, double* partial, hipStream_t st)
 {
   int b0, nb;
   bin_range(w, tile, b0, nb);
   const int grid = std::min(nb, 1024);
+  hipLaunchKernelGGL(k_kinetic, dim3(unsigned(grid)), dim3(WG), 0, st, make_sp(w), s, b0, nb, partial);
+  RK_HIP(hipGetLastError());
+  return RK_OK;
+}
+
+int kinetic_energies(rk_world* w, int tile, double* out, hipStream_t st)
+{
+  const int grid = kinetic_partial_count(w, tile);
   if(w->red_cap < grid) return (set_error("reduction scratch too small"), RK_EINVAL);
   std::vector<double> h(static_cast<size_t>(grid));
   for(int s = 0; s < w->d.n_species; ++s) {
-    hipLaunchKernelGGL(k_kinetic, dim3(unsigned(grid)), dim3(WG), 0, st, make_sp(w), s, b0, nb, w->red);
-    RK_HIP(hipGetLastError());
+    if(int rc = launch_kinetic_partials(w, tile, s, w->red, st)) return rc;
     RK_HIP(hipMemcpyAsync(h.data(), w->red, sizeof(double) * h.size(), hipMemcpyDeviceToHost, st));
     RK_HIP(hipStreamSynchronize(st));
     double acc = 0;

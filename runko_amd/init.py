@@ -26,7 +26,7 @@ import numpy as np
 import torch
 from scipy.special import kn
 
-from .world import HALO, World
+from .world import HALO, World, bin_caps
 
 
 def plasma_parameters(cfl, c_omp, ppc, theta, sigma):
@@ -65,22 +65,23 @@ def juttner_momenta(n, theta, gen, device):
     return torch.stack([u * (2 * X1 - 1), s * torch.cos(2 * math.pi * X2), s * torch.sin(2 * math.pi * X2)])
 
 
-def fill_uniform_plasma(world, ppc, theta, seed=1, chunk_bins=4096, slot_order="interleaved"):
+def fill_uniform_plasma(world, ppc, theta, seed=1, chunk_bins=4096, slot_order="interleaved", layout=None):
     """Every cell gets ppc particles of every species (all species at the same
     positions), bins laid out for exactly that occupancy plus slack.
 
     slot_order: "interleaved" stores slot p of a bin in cell p mod (cells per
     brick), so the 64 lanes of a wave sit in 64 different, neighbouring cells
     (LDS gathers and current atomics spread over the banks); "random" uses a
-    seeded random slot permutation."""
+    seeded random slot permutation.  layout: keyword arguments of
+    World.presize_all (slack, pad, arena) for a non-default bin layout."""
     if slot_order not in ("interleaved", "random"):
         raise ValueError(f"unknown slot_order {slot_order!r}")
     dev = world.dev
     br = world.brick
     cells = br[0] * br[1] * br[2]
     per_bin = cells * ppc
-    for s in range(world.nsp):
-        world.presize(s, np.full(world.nbricks, per_bin, dtype=np.int64))
+    world.presize_all({s: np.full(world.nbricks, per_bin, dtype=np.int64) for s in range(world.nsp)},
+                      **(layout or {}))
     gen = torch.Generator(device=dev)
     gen.manual_seed(int(seed))
     nbd = [world.n[a] // br[a] for a in range(3)]
@@ -159,16 +160,21 @@ def set_turbulent_field(world, B0, n_modes=8, dB_over_B0=1.0, seed=2):
 
 def turbulence_world(N=512, tile=32, ppc=16, theta=0.3, sigma=10.0, cfl=0.45, c_omp=10.0, n_modes=8,
                      dB_over_B0=1.0, seed=1, brick=None, mig_fraction=0.01, mailbox_fraction=0.12,
-                     slot_order="interleaved"):
-    """Decaying-turbulence box: N^3 cells in tile^3 tiles, periodic."""
+                     slot_order="interleaved", layout=None):
+    """Decaying-turbulence box: N^3 cells in tile^3 tiles, periodic.  layout:
+    keyword arguments of World.presize_all (slack, pad, arena)."""
     nt = N // tile
     q0, q1, B0 = plasma_parameters(cfl, c_omp, ppc, theta, sigma)
     w = World((tile,) * 3, (nt,) * 3, (0, 0, 0), (nt,) * 3, periodic=True, cfl=cfl, charges=[q0, q1],
               masses=[1.0, 1.0], current_filter="binomial2", brick=brick)
     set_turbulent_field(w, B0, n_modes=n_modes, dB_over_B0=dB_over_B0, seed=seed + 1)
-    fill_uniform_plasma(w, ppc, theta, seed=seed, slot_order=slot_order)
-    w.ensure_migration(max(1 << 16, int(mig_fraction * w.total_particles())))
-    w.ensure_mailbox(mailbox_fraction)
+    # record sets and mailboxes first: the pools then size their spare arenas
+    # from the device memory that is left
+    cells = tile ** 3 * nt ** 3
+    w.ensure_migration(max(1 << 16, int(mig_fraction * 2 * ppc * cells)))
+    br = w.brick
+    w.ensure_mailbox(mailbox_fraction, brick_capacity=2 * int(bin_caps([br[0] * br[1] * br[2] * ppc])[0]))
+    fill_uniform_plasma(w, ppc, theta, seed=seed, slot_order=slot_order, layout=layout)
     w.comm_local(1)
     w.comm_local(2)
     torch.cuda.synchronize()
@@ -222,8 +228,7 @@ def fill_shock_plasma(world, ppc, theta, gamma_flow, walloc, ramp_len, ramp_fact
     x0 = (tcx + world.toff[0]) * world.n[0] + lbx * br[0]
     csum = np.concatenate([[0], np.cumsum(cnt_x)])
     demand = (csum[x0 + br[0]] - csum[x0]) * br[1] * br[2]
-    for s in range(world.nsp):
-        world.presize(s, demand)
+    world.presize_all({s: demand for s in range(world.nsp)})
     gen = torch.Generator(device=dev)
     gen.manual_seed(int(seed))
     cl = torch.stack(torch.meshgrid(*(torch.arange(v, device=dev) for v in br), indexing="ij"), -1).reshape(-1, 3)

@@ -51,23 +51,75 @@ def _require_gpu():
         raise RuntimeError("runko_amd needs a ROCm GPU (no CPU fallback)")
 
 
-ARENA = 0.10          # spare pool capacity, fraction of the laid-out bins
-ARENA_MAX = 1 << 26   # at most 64 M particle slots
+SLOT_BYTES = 32       # one particle slot: pos[3], vel[3] float32 + id uint64
+ARENA_TARGET = 0.5    # spare pool capacity aimed at, fraction of the laid-out bins
+RESERVE_MIN = 16 << 30  # HBM left free after a layout (torch temporaries, relayout staging)
+RESERVE_FRAC = 0.04   # ... or this fraction of the device, whichever is larger
+STAGE_MAX = 2 << 30   # relayout staging buffer
+
+
+def round32(v):
+    return (np.asarray(v, dtype=np.int64) + 31) // 32 * 32
+
+
+def bin_caps(demand, slack=SLACK, pad=PAD):
+    """Bin capacities for a per-bin demand: demand * slack + pad, multiples
+    of 32; empty bins get none."""
+    d = np.asarray(demand, dtype=np.int64)
+    return np.where(d > 0, round32(np.ceil(d * slack).astype(np.int64) + pad), 0)
+
+
+def plan_arena(used, free_bytes, reserve_bytes, target=ARENA_TARGET):
+    """Spare slots per species for pools whose bins take `used` slots.
+
+    Every species aims at target * used spare slots.  When the device cannot
+    hold that beside the bins and the reserve, the slots that fit are shared
+    in proportion to `used`.  Raises if the bins themselves do not fit."""
+    used = np.asarray(used, dtype=np.int64)
+    total = int(used.sum())
+    budget = (int(free_bytes) - int(reserve_bytes)) // SLOT_BYTES - total
+    if budget < 0:
+        raise RuntimeError(f"runko_amd: particle bins need {total * SLOT_BYTES / 2**30:.1f} GiB, only "
+                           f"{(free_bytes - reserve_bytes) / 2**30:.1f} GiB of device memory are free beyond the "
+                           f"{reserve_bytes / 2**30:.1f} GiB reserve")
+    want = np.ceil(target * used).astype(np.int64)
+    if int(want.sum()) <= budget:
+        return want
+    return budget * used // max(total, 1)
+
+
+def plan_relayout(demand, caps, grown, capacity):
+    """New capacities of every bin for an in-place relayout of a pool of
+    `capacity` slots: bins whose demand exceeds their capacity get `grown`,
+    the others keep theirs; if that does not fit, every bin gets its demand
+    plus slack, then plus a pad, then exactly its demand.  Returns None when
+    even the demand exceeds the pool."""
+    demand = np.asarray(demand, dtype=np.int64)
+    caps = np.asarray(caps, dtype=np.int64)
+    full = demand > caps
+    for c in (np.where(full, grown, caps), bin_caps(demand), np.where(demand > 0, round32(demand + PAD), 0),
+              demand):
+        c = np.maximum(c, np.minimum(demand, caps))
+        if int(c.sum()) <= capacity:
+            return c
+    return None
 
 
 class _Pool:
-    def __init__(self, nb, cap_per_bin, dev):
+    def __init__(self, nb, cap_per_bin, dev, arena=None):
         caps = np.asarray(cap_per_bin, dtype=np.int64)
-        self.set_layout(caps, dev)
+        self.set_layout(caps, dev, arena)
 
-    def set_layout(self, caps, dev):
+    def set_layout(self, caps, dev, arena=None):
         starts = np.zeros_like(caps)
         starts[1:] = np.cumsum(caps)[:-1]
         # bins fill [0, used); the arena [used, capacity) takes bins that
-        # outgrow their place (World.settle_overflow moves them there without
-        # a new pool, so buffer addresses stay the same)
+        # outgrow their place (World.settle_overflow moves them there; when it
+        # is used up the pool is relaid out in place, so buffer addresses stay)
         self.used = int(caps.sum())
-        self.capacity = self.used + min(int(ARENA * self.used), ARENA_MAX) if self.used else 0
+        if arena is None:
+            arena = int(np.ceil(ARENA_TARGET * self.used))
+        self.capacity = self.used + int(arena) if self.used else 0
         self.start_h, self.cap_h = starts.copy(), caps.copy()
         n = max(self.capacity, 1)
         self.pos = torch.empty((3, n), dtype=torch.float32, device=dev)
@@ -75,6 +127,9 @@ class _Pool:
         self.id = torch.empty((n,), dtype=torch.int64, device=dev)
         self.start = torch.from_numpy(starts).to(dev)
         self.cap = torch.from_numpy(caps.astype(np.int32)).to(dev)
+
+    def nbytes(self):
+        return max(self.capacity, 1) * SLOT_BYTES
 
 
 class World:
@@ -123,6 +178,8 @@ class World:
         self.fused_launch = "grid"
         self.fused_workgroup = 512
         self.bin_moves = 0  # repacks done inside the arena (no new pool)
+        self.relayouts = 0  # in-place relayouts of a pool (arena used up)
+        self.new_pools = 0  # pools replaced by larger ones
         self.settle_seconds = 0.0  # wall time spent repacking and re-appending
         self.count = [torch.zeros((self.nbricks,), dtype=torch.int32, device=self.dev) for _ in range(self.nsp)]
         self.pools = [_Pool(self.nbricks, np.zeros(self.nbricks, dtype=np.int64), self.dev) for _ in range(self.nsp)]
@@ -181,18 +238,21 @@ class World:
             self._mig_cap[st] = c
             self.layout_version += 1
 
-    def ensure_mailbox(self, fraction=0.12):
+    def ensure_mailbox(self, fraction=0.12, brick_capacity=None):
         """Per-brick mailboxes for the fused particle step, sized to `fraction`
-        of the largest brick's total capacity (all species), and the per-brick
-        current staging slots (3 (br+3)^3 floats each) of the same step."""
+        of the largest brick's total capacity (all species; `brick_capacity`
+        gives it before the pools are laid out), and the per-brick current
+        staging slots (3 (br+3)^3 floats each) of the same step."""
         if self.jstage is None:
             stride = 3 * (self.brick[0] + 3) * (self.brick[1] + 3) * (self.brick[2] + 3)
             self.jstage = torch.empty((self.nbricks, stride), dtype=torch.float32, device=self.dev)
             torch.cuda.synchronize()
             L.check(L.lib().rk_world_set_current_staging(self.h, self.jstage.data_ptr(), stride))
             self.layout_version += 1
-        caps = sum(p.cap.to(torch.int64) for p in self.pools)
-        per = int(math.ceil(fraction * int(caps.max().item()))) if self.nsp else 0
+        if brick_capacity is None:
+            caps = sum(p.cap.to(torch.int64) for p in self.pools)
+            brick_capacity = int(caps.max().item()) if self.nsp else 0
+        per = int(math.ceil(fraction * int(brick_capacity)))
         per = max(64, (per + 31) // 32 * 32)
         if per <= self._mbox_per_brick:
             return
@@ -307,10 +367,16 @@ class World:
         return n.value
 
     def settle_overflow(self):
-        """Repack every species whose bins overflowed, then re-append the
-        pending records.  Synchronous.  A full bin grows to its demand plus
-        slack and at least 1.5x, so a bin that keeps filling (a compressed
-        layer) is repacked a logarithmic number of times."""
+        """Give every bin that overflowed room, then re-append the pending
+        records (synchronous; no particle is lost).
+
+        A full bin grows to its demand plus slack and at least 1.5x, so a bin
+        that keeps filling (a compressed layer) moves a logarithmic number of
+        times.  It moves into its pool's spare arena while the arena has room
+        (rk_move_bins); when the arena is used up the pool is relaid out in
+        place (rk_relayout), which reclaims the places moved bins left.  The
+        pool buffers are never reallocated unless even the relayout cannot
+        fit and the device has room for a larger pool."""
         t0 = time.perf_counter()
         for _ in range(4):
             if self.pending_overflow() == 0:
@@ -323,15 +389,17 @@ class World:
                 if not np.any(full):
                     continue
                 grown = np.maximum(np.ceil(demand * SLACK).astype(np.int64) + PAD, (3 * caps + 1) // 2)
-                grown = (grown + 31) // 32 * 32
+                grown = round32(grown)
                 need = int(grown[full].sum())
                 if pool.used + need <= pool.capacity:
                     self._move_bins(s, full, grown)
                 else:
-                    self._repack(s, np.where(full, grown, caps))
-            L.check(L.lib().rk_retry_append(self.h, _stream()))
+                    self._relayout(s, demand, grown)
+            rc = L.lib().rk_retry_append(self.h, _stream())
+            if rc != L.RK_EOVERFLOW:  # overflow again: the next pass gives those bins room
+                L.check(rc)
         if self.pending_overflow():
-            raise RuntimeError("runko_amd: particle bins still overflowing after repack")
+            raise RuntimeError("runko_amd: particle bins still overflowing after four relayouts")
         self.settle_seconds += time.perf_counter() - t0
         # the mailboxes keep their size (leavers past it spill into the shared
         # record sets); replacing them here would orphan pending records
@@ -341,47 +409,106 @@ class World:
         capacity `grown`; the other bins stay.  Same buffers: captured graphs
         stay valid."""
         pool = self.pools[s]
-        starts, caps = pool.start_h.copy(), pool.cap_h.copy()
         idx = np.nonzero(full)[0]
         g = grown[idx]
-        starts[idx] = pool.used + np.concatenate([[0], np.cumsum(g)[:-1]])
-        caps[idx] = g
-        new_start = torch.from_numpy(starts).to(self.dev)
-        new_cap = torch.from_numpy(caps.astype(np.int32)).to(self.dev)
-        # copies every bin to its (possibly new) place: unmoved bins onto
-        # themselves, moved ones into the arena (disjoint from every bin)
-        L.check(L.lib().rk_repack(self.h, s, pool.capacity, L.p3(pool.pos), L.p3(pool.vel), pool.id.data_ptr(),
-                                  new_start.data_ptr(), new_cap.data_ptr(), _stream()))
-        pool.start.copy_(new_start)
-        pool.cap.copy_(new_cap)
+        new = pool.used + np.concatenate([[0], np.cumsum(g)[:-1]]).astype(np.int64)
+        d_idx = torch.from_numpy(idx.astype(np.int32)).to(self.dev)
+        d_new = torch.from_numpy(new).to(self.dev)
+        d_cap = torch.from_numpy(g.astype(np.int32)).to(self.dev)
+        L.check(L.lib().rk_move_bins(self.h, s, len(idx), d_idx.data_ptr(), d_new.data_ptr(), d_cap.data_ptr(),
+                                     _stream()))
         torch.cuda.synchronize()
         pool.used += int(g.sum())
-        pool.start_h, pool.cap_h = starts, caps
-        L.check(L.lib().rk_world_set_species(self.h, s, pool.capacity, L.p3(pool.pos), L.p3(pool.vel),
-                                             pool.id.data_ptr(), pool.start.data_ptr(), pool.cap.data_ptr(),
-                                             self.count[s].data_ptr()))
+        pool.start_h[idx] = new
+        pool.cap_h[idx] = g
         self.repacks += 1
         self.bin_moves += 1
 
-    def _repack(self, s, newcaps):
-        newp = _Pool(self.nbricks, newcaps, self.dev)
+    def _relayout(self, s, demand, grown):
+        """Relays out species s in place with every bin that overflowed grown
+        (plan_relayout); falls back to a larger pool only if the relayout
+        cannot hold the demand and the device has room for one."""
+        pool = self.pools[s]
+        caps = plan_relayout(demand, pool.cap_h, grown, pool.capacity)
+        if caps is None:
+            self._grow_pool(s, demand)
+            return
+        free, total = torch.cuda.mem_get_info(self.dev)
+        stage_bytes = int(min(getattr(self, "stage_bytes", STAGE_MAX), max(free // 2, 1 << 20)))
+        stage = torch.empty((stage_bytes,), dtype=torch.uint8, device=self.dev)
+        caps32 = np.ascontiguousarray(caps.astype(np.uint32))
+        starts = np.zeros(self.nbricks, dtype=np.int64)
+        L.check(L.lib().rk_relayout(self.h, s, caps32.ctypes.data, stage.data_ptr(), stage_bytes, starts.ctypes.data,
+                                    _stream()))
+        del stage
+        pool.start_h, pool.cap_h = starts, caps.astype(np.int64)
+        pool.used = int(caps.sum())
+        self.repacks += 1
+        self.relayouts += 1
+
+    def _grow_pool(self, s, demand):
+        """A larger pool for species s: only when the device can hold it
+        beside the old one and the reserve."""
+        caps = bin_caps(demand)
+        used = int(caps.sum())
+        free, total = torch.cuda.mem_get_info(self.dev)
+        reserve = max(RESERVE_MIN, int(RESERVE_FRAC * total))
+        arena = plan_arena([used], free, reserve)[0] if free > reserve + used * SLOT_BYTES else -1
+        if arena < 0:
+            raise RuntimeError(f"runko_amd: species {s} holds {int(demand.sum())} particles, more than its pool's "
+                               f"{self.pools[s].capacity} slots, and the device has {free / 2**30:.1f} GiB free "
+                               f"(a larger pool needs {used * SLOT_BYTES / 2**30:.1f} GiB beside the "
+                               f"{reserve / 2**30:.1f} GiB reserve)")
+        self._repack(s, caps, arena)
+
+    def _repack(self, s, newcaps, arena=None):
+        newp = _Pool(self.nbricks, newcaps, self.dev, arena)
         L.check(L.lib().rk_repack(self.h, s, newp.capacity, L.p3(newp.pos), L.p3(newp.vel), newp.id.data_ptr(),
                                   newp.start.data_ptr(), newp.cap.data_ptr(), _stream()))
         torch.cuda.synchronize()
         self.pools[s] = newp
         self.layout_version += 1
         self.repacks += 1
+        self.new_pools += 1
 
     def presize(self, s, demand):
         """Lay out species s with bins sized for `demand` particles per bin
         (only while the species is empty)."""
-        if int(self.count[s].sum().item()) != 0:
-            raise RuntimeError("presize on a non-empty species")
-        caps = np.ceil(np.asarray(demand, dtype=np.float64) * SLACK).astype(np.int64)
-        caps = np.where(caps > 0, caps + PAD, 0)
-        caps = (caps + 31) // 32 * 32
-        self.pools[s] = _Pool(self.nbricks, caps, self.dev)
-        self._bind_species(s)
+        self.presize_all({s: demand})
+
+    def presize_all(self, demands, reserve_bytes=None, slack=SLACK, pad=PAD, arena=None):
+        """Lay out the species of `demands` ({species: per-bin demand}, all
+        empty) together: bins of demand * slack + pad, and spare arenas sized
+        from the free device memory (plan_arena) so that the pools never need
+        a second allocation while the arenas and in-place relayouts can hold
+        the particles.  arena: spare slots per species instead of the plan."""
+        for s in demands:
+            if int(self.count[s].sum().item()) != 0:
+                raise RuntimeError("presize on a non-empty species")
+        caps = {s: bin_caps(d, slack, pad) for s, d in demands.items()}
+        for s in demands:  # release the old (empty) pools first
+            self.pools[s] = _Pool(self.nbricks, np.zeros(self.nbricks, dtype=np.int64), self.dev)
+        import gc
+        gc.collect()
+        torch.cuda.empty_cache()
+        free, total = torch.cuda.mem_get_info(self.dev)
+        reserve = max(RESERVE_MIN, int(RESERVE_FRAC * total)) if reserve_bytes is None else int(reserve_bytes)
+        keys = list(caps)
+        arenas = plan_arena([int(caps[s].sum()) for s in keys], free, reserve)
+        if arena is not None:
+            arenas = [int(arena)] * len(keys)
+        for s, arena in zip(keys, arenas):
+            self.pools[s] = _Pool(self.nbricks, caps[s], self.dev, int(arena))
+            self._bind_species(s)
+
+    def memory_report(self):
+        """Pool occupancy and overflow statistics (host numbers)."""
+        out = dict(repacks=self.repacks, bin_moves=self.bin_moves, relayouts=self.relayouts, new_pools=self.new_pools,
+                   settle_seconds=self.settle_seconds, species=[])
+        for s, p in enumerate(self.pools):
+            out["species"].append(dict(particles=int(self.count[s].sum().item()), bins_used=p.used,
+                                       capacity=p.capacity, gib=p.nbytes() / 2**30))
+        return out
 
     def bricks_of(self, t, pos):
         """Brick index (world bin) of positions pos [3, n] (float32 tensor) in tile t."""
