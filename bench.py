@@ -160,36 +160,15 @@ def run_turbulence(args):
 def config0_inputs(seed=42):
     """64^3 cells, 4^3 tiles of 16^3, 8 ppc/species, theta 0.3, sigma 1, E = 0,
     B = B0 z (projects/pic-turbulence/pic.py conventions)."""
-    import numpy as np
-
-    from runko_amd.init import plasma_parameters
-    from runko_amd.sampling import sample_boosted_juttner_synge
-
-    nt, n, ppc, theta = (4, 4, 4), (16, 16, 16), 8, 0.3
-    q0, q1, B0 = plasma_parameters(0.45, 10.0, ppc, theta, 1.0)
-    rng = np.random.default_rng(seed)
-    tiles = {}
-    cells = np.stack(np.meshgrid(*(np.arange(v, dtype=np.float64) for v in n), indexing="ij"), -1).reshape(-1, 3)
-    for i in range(nt[0]):
-        for j in range(nt[1]):
-            for k in range(nt[2]):
-                lo = np.array([i * n[0], j * n[1], k * n[2]], dtype=np.float64)
-                pos = np.repeat(cells, ppc, 0) + lo + rng.random((len(cells) * ppc, 3))
-                parts = []
-                for s in range(2):
-                    ux, uy, uz = sample_boosted_juttner_synge(len(pos), theta, beta=0, gen=rng)
-                    parts.append(np.concatenate([pos, np.stack([ux, uy, uz], 1)], 1))
-                F = np.zeros((9,) + n)
-                F[5] = B0
-                tiles[(i, j, k)] = (F, parts)
-    return nt, n, [q0, q1], [1.0, 1.0], tiles
+    from runko_amd.workloads import pair_plasma_inputs
+    return pair_plasma_inputs(seed=seed)
 
 
 def run_config0_gpu(inputs, laps, execution):
     import torch
 
-    sys.path.insert(0, os.path.join(ROOT, "tests"))
-    from gpu_common import build_gpu, canonical_lap
+    from runko_amd.workloads import build_simulation as build_gpu
+    from runko_amd.workloads import turbulence_lap as canonical_lap
 
     nt, n, Q, M, tiles = inputs
     graph = execution == "graph"
@@ -217,10 +196,24 @@ def run_config0_gpu(inputs, laps, execution):
     return dict(particle_steps_per_s=npart * laps / dt, ms_per_step=1e3 * dt / laps, laps=laps, npart=npart)
 
 
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def run_cpu_baseline(inputs, seconds):
+    """The oracle (FP32 C++/OpenMP restatement of the reference's step, same
+    lap order) timed on this host's cores: built -O3 -march=native here when
+    g++ can, on every thread OpenMP offers and on one thread."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
 
+    native = oracle.use_native_build()
     nt, n, Q, M, tiles = inputs
     w = oracle.OracleWorld(nt, n, 0.45, Q, M, prec=32)
     for idx, (F, parts) in tiles.items():
@@ -231,17 +224,28 @@ def run_cpu_baseline(inputs, seconds):
     w.comm_local("emf_E")
     w.comm_local("emf_B")
     npart = sum(w.num_alive(t, s) for t in range(w.ntiles) for s in range(2))
+    threads = oracle.max_threads()
+
+    def timed(limit, max_laps, start):
+        laps, t0 = 0, time.perf_counter()
+        while True:
+            w.lap(start + laps)
+            laps += 1
+            if time.perf_counter() - t0 > limit or laps >= max_laps:
+                break
+        return laps, time.perf_counter() - t0
+
     w.lap(0)  # warm-up
-    laps, t0 = 0, time.perf_counter()
-    while True:
-        w.lap(laps + 1)
-        laps += 1
-        if time.perf_counter() - t0 > seconds or laps >= 200:
-            break
-    dt = time.perf_counter() - t0
-    return dict(value=npart * laps / dt, unit="particle-steps/s", cores=oracle.max_threads(), kind="port",
-                sample=f"oracle FP32 C++/OpenMP restatement, 64^3 cells in 4^3 tiles, {npart} particles, "
-                       f"{laps} full laps ({dt:.1f} s), same step order as the GPU run")
+    laps, dt = timed(seconds, 200, 1)
+    oracle.set_threads(1)
+    laps1, dt1 = timed(max(seconds / 3, 1.0), 20, 1 + laps)
+    oracle.set_threads(threads)
+    build = "-O3 -march=native" if native else "-O3 -march=x86-64-v3"
+    return dict(value=npart * laps / dt, unit="particle-steps/s", cores=threads, kind="port",
+                one_thread=npart * laps1 / dt1, cpu_model=cpu_model(), build=build,
+                sample=f"oracle FP32 C++/OpenMP restatement ({build}, -ffp-contract=off), 64^3 cells in 4^3 tiles, "
+                       f"{npart} particles: {laps} full laps on {threads} threads ({dt:.1f} s), then {laps1} laps on "
+                       f"1 thread ({dt1:.1f} s); same step order as the GPU run")
 
 
 # the stencil coefficients of the reference's 3D shock configuration

@@ -23,6 +23,22 @@ def build():
     subprocess.run(["make", "-s", "-C", _HERE], check=True)
 
 
+def use_native_build(timeout=180):
+    """Build liborc_native.so (-O3 -march=native for this host) and load it
+    instead of the portable liborc.so.  Returns False (keeping the portable
+    build) if the build fails or a library is already loaded."""
+    global _LIB
+    if _lib is not None:
+        return _LIB.endswith("liborc_native.so")
+    try:
+        subprocess.run(["make", "-s", "-B", "-C", _HERE, "liborc_native.so"], check=True, timeout=timeout,
+                       capture_output=True)
+    except (subprocess.SubprocessError, OSError):
+        return False
+    _LIB = os.path.join(_HERE, "liborc_native.so")
+    return True
+
+
 def lib():
     global _lib
     if _lib is None:

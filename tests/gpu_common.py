@@ -2,31 +2,13 @@
 
 import numpy as np
 
-import runko_amd as runko
+import runko_amd as runko  # noqa: F401
+from runko_amd import workloads
 
 CFL = 0.45
 
 
-def make_config(nt, n, charges, masses, filt="binomial2", cfl=CFL, execution="plain", pusher="boris",
-                stencil=None):
-    c = runko.Configuration(None)
-    c.n_tiles = list(nt)
-    c.n_cells_per_tile = list(n)
-    c.cfl = cfl
-    c.field_propagator = "fdtd2" if stencil is None else "stencil"
-    for key, v in (stencil or {}).items():
-        setattr(c, key, v)
-    c.particle_pusher = pusher
-    c.field_interpolator = "linear_1st"
-    c.current_depositer = "zigzag_1st_atomic"
-    c.current_filter = filt
-    c.tile_partitioning = "hilbert_curve"
-    c.n_laps = 10
-    c.execution = execution
-    for s, (q, m) in enumerate(zip(charges, masses)):
-        setattr(c, f"q{s}", q)
-        setattr(c, f"m{s}", m)
-    return c
+make_config = workloads.make_configuration
 
 
 def random_inputs(nt, n, nsp, seed, ppc=4, field_amp=0.05, umax=1.5):
@@ -51,29 +33,7 @@ def random_inputs(nt, n, nsp, seed, ppc=4, field_amp=0.05, umax=1.5):
     return tiles
 
 
-def build_gpu(nt, n, charges, masses, inputs, filt="binomial2", execution="plain", comm_prelude=True, pusher="boris",
-              stencil=None, brick_shape=None):
-    conf = make_config(nt, n, charges, masses, filt=filt, execution=execution, pusher=pusher, stencil=stencil)
-    if brick_shape is not None:
-        conf.brick_shape = brick_shape
-    grid = runko.TileGrid(conf)
-    for idx in grid.local_tile_indices():
-        F, parts = inputs[idx]
-        tile = runko.pic.threeD.Tile(idx, conf)
-        fs = [lambda x, y, z, a=F[c]: a for c in range(9)]
-        tile.batch_set_EBJ(*fs)
-        for s, st in enumerate(parts):
-            tile.batch_inject_to_cells(
-                s, lambda x, y, z, st=st: runko.pic.threeD.ParticleStateBatch(pos=(st[:, 0], st[:, 1], st[:, 2]),
-                                                                            vel=(st[:, 3], st[:, 4], st[:, 5])))
-        grid.add_tile(tile, idx)
-    sim = grid.configure_simulation(conf)
-    if comm_prelude:
-        def sync(x):
-            x.comm_local(runko.tools.comm_mode.emf_E)
-            x.comm_local(runko.tools.comm_mode.emf_B)
-        sim.prelude(sync)
-    return conf, grid, sim
+build_gpu = workloads.build_simulation
 
 
 def build_oracle(orc, nt, n, charges, masses, inputs, prec=32, filt="binomial2", comm_prelude=True, pusher="boris",
@@ -139,40 +99,6 @@ def gpu_fields(grid):
     return out
 
 
-def canonical_lap(sim, runko_mod=runko, io=False, sort_every=5):
+def canonical_lap(sim, io=False, sort_every=5):
     """projects/pic-turbulence/pic.py:187-221 without snapshot IO."""
-    cm = runko_mod.tools.comm_mode
-
-    def step(x):
-        x.grid_push_half_b()
-        x.comm_external(cm.emf_B)
-        x.comm_local(cm.emf_B)
-        x.prtcl_push()
-        x.prtcl_pack_outgoing()
-        x.comm_external(cm.pic_particle)
-        x.comm_local(cm.pic_particle)
-        if sort_every and sim.lap % sort_every == 0:
-            x.prtcl_sort()
-        x.prtcl_deposit_current()
-        x.comm_external(cm.emf_J)
-        x.comm_local(cm.emf_J_exchange)
-        x.comm_external(cm.emf_J)
-        x.comm_local(cm.emf_J)
-        x.grid_filter_current()
-        x.comm_external(cm.emf_J)
-        x.comm_local(cm.emf_J)
-        x.grid_filter_current()
-        x.grid_filter_current()
-        x.grid_push_half_b()
-        x.comm_external(cm.emf_B)
-        x.comm_local(cm.emf_B)
-        x.grid_push_e()
-        x.grid_add_current()
-        x.comm_external(cm.emf_E)
-        x.comm_local(cm.emf_E)
-        if io:
-            x.io_average_kinetic_energy()
-            x.io_average_B_energy_density()
-            x.io_average_E_energy_density()
-
-    return step
+    return workloads.turbulence_lap(sim, io=io, sort_every=sort_every)
