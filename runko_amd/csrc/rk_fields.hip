@@ -251,6 +251,9 @@ __global__ void __launch_bounds__(BX* BY)
 }
 
 // 27-tap (1,2,1)^3/64, taps in layout_right order, accumulator starting at 0.
+// Every coefficient is a power of two (1, 2, 4 or 8 over 64), so the product
+// c*P is exact and fma(c, P, acc) rounds exactly as acc + c*P does (unless
+// c*P falls below the smallest normal float, 2^-126).
 // Same 27-tap filter, each lane sweeping a column of FCH outputs along x: the
 // 3x3 (y, z) neighbourhood of three consecutive x planes stays in registers,
 // so every input is loaded once per column instead of three times.  The
@@ -287,7 +290,7 @@ __device__ inline void filter_columns(const Geo& g, const float* const (&S)[3], 
 #pragma unroll
         for(int b = 0; b < 3; ++b)
 #pragma unroll
-          for(int cc = 0; cc < 3; ++cc) acc = acc + ((w[a] * w[b] * w[cc]) * (1.0f / 64.0f)) * P[q][a][b][cc];
+          for(int cc = 0; cc < 3; ++cc) acc = __builtin_fmaf((w[a] * w[b] * w[cc]) * (1.0f / 64.0f), P[q][a][b][cc], acc);
       D[q][c] = acc;
     }
 #pragma unroll

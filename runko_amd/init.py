@@ -11,13 +11,13 @@ pools directly on the device:
     algorithm as runko/sample_thermal_distributions.py:7-127, drawn with
     torch's Philox generator (statistically, not bitwise, equal),
   * ids (tile_tag << 40) | ordinal, ordinals counted brick by brick,
-  * a guide field B0 along z plus n x n divergence-free sinusoidal dBx, dBy
-    modes with seeded random phases, normalised to rms(dB) = dB_over_B0 * B0;
+  * a guide field B0 along z plus n x n sinusoidal dBx, dBy modes with
+    seeded random phases (the discrete curl of a vector potential, so the
+    discrete div B is zero), normalised to rms(dB) = dB_over_B0 * B0;
   * for the reflecting-wall shock: a cold flow along -x (rest-frame
-    Juttner-Synge momenta boosted by the flow's Lorentz factor; the boost
-    does not re-weight the sample as the reference's flipping method does,
-    which matters only for warm flows), a density ramp in front of the wall,
-    the upstream perpendicular field and its motional E.
+    Juttner-Synge momenta boosted by the flow's Lorentz factor with the
+    reference's flipping method), a density ramp in front of the wall, the
+    upstream perpendicular field and its motional E.
 """
 
 import math
@@ -102,6 +102,12 @@ def fill_uniform_plasma(world, ppc, theta, seed=1, chunk_bins=4096, slot_order="
         nb = b1 - b0
         base = (origin[:, None, None, :] + cl[None, :, None, :]).expand(nb, cells, ppc, 3).to(torch.float64)
         pos = (base + torch.rand((nb, cells, ppc, 3), generator=gen, device=dev, dtype=torch.float64))
+        # keep each float32 position inside its cell (and so inside its brick):
+        # a draw just below the upper face can round up onto it
+        upper = (base + 1.0).to(torch.float32)
+        pos = pos.to(torch.float32)
+        pos = torch.where(pos >= upper, torch.nextafter(upper, torch.zeros_like(upper)), pos)
+        del upper
         if slot_order == "random":
             pos = pos.to(torch.float32).reshape(nb, per_bin, 3)[:, slot_perm, :]
         else:
@@ -122,32 +128,30 @@ def fill_uniform_plasma(world, ppc, theta, seed=1, chunk_bins=4096, slot_order="
 
 
 def set_turbulent_field(world, B0, n_modes=8, dB_over_B0=1.0, seed=2):
-    """Bz = B0 everywhere; dBx, dBy from A_z = sum_{m,n=1..n_modes} a_mn
-    sin(m k x + n k y + phi_mn) (k = 2 pi / L), so div dB = 0; every mode has
-    the same amplitude and the total is scaled to rms(dB) = dB_over_B0 * B0."""
+    """Bz = B0 everywhere; dBx, dBy are the discrete curl of
+    A_z = sum_{m,n=1..n_modes} a_mn sin(m k x + n k y + phi_mn) (k = 2 pi / L,
+    seeded phases) sampled on the Yee lattice (A_z on the z edges, Bx at
+    (i, j+1/2, k+1/2), By at (i+1/2, j, k+1/2)):
+        Bx(i, j+1/2) = A_z(i, j+1) - A_z(i, j),  By(i+1/2, j) = A_z(i, j) - A_z(i+1, j),
+    so the discrete div B vanishes up to rounding.  a_mn = 1 / |k_mn| with the
+    lattice wavenumber |k_mn|^2 = (2 sin(m k / 2))^2 + (2 sin(n k / 2))^2
+    gives every mode the same B amplitude; the total is scaled to
+    rms(dB) = dB_over_B0 * B0."""
     dev = world.dev
     N = world.N
     rng = np.random.default_rng(seed)
     phi = 2 * np.pi * rng.random((n_modes, n_modes))
     kx, ky = 2 * np.pi / N[0], 2 * np.pi / N[1]
-    ii = torch.arange(N[0], device=dev, dtype=torch.float64)
-    jj = torch.arange(N[1], device=dev, dtype=torch.float64)
-
-    def dB(xoff, yoff):
-        X = (ii + xoff)[:, None]
-        Y = (jj + yoff)[None, :]
-        bx = torch.zeros((N[0], N[1]), device=dev, dtype=torch.float64)
-        by = torch.zeros_like(bx)
-        for m in range(1, n_modes + 1):
-            for n in range(1, n_modes + 1):
-                kk = math.sqrt((m * kx) ** 2 + (n * ky) ** 2)
-                c = torch.cos(m * kx * X + n * ky * Y + float(phi[m - 1, n - 1]))
-                bx += (n * ky / kk) * c
-                by -= (m * kx / kk) * c
-        return bx, by
-
-    bx, _ = dB(0.0, 0.5)  # Bx at (i, j+1/2, k+1/2)
-    _, by = dB(0.5, 0.0)  # By at (i+1/2, j, k+1/2)
+    # A_z on the (periodic) nodes i = 0..N, j = 0..N
+    X = torch.arange(N[0] + 1, device=dev, dtype=torch.float64)[:, None]
+    Y = torch.arange(N[1] + 1, device=dev, dtype=torch.float64)[None, :]
+    A = torch.zeros((N[0] + 1, N[1] + 1), device=dev, dtype=torch.float64)
+    for m in range(1, n_modes + 1):
+        for n in range(1, n_modes + 1):
+            kd = math.sqrt((2 * math.sin(m * kx / 2)) ** 2 + (2 * math.sin(n * ky / 2)) ** 2)
+            A += torch.sin(m * kx * X + n * ky * Y + float(phi[m - 1, n - 1])) / kd
+    bx = A[:-1, 1:] - A[:-1, :-1]
+    by = A[:-1, :-1] - A[1:, :-1]
     rms = math.sqrt(float((bx**2 + by**2).mean()))
     scale = dB_over_B0 * B0 / rms
     inner = (slice(HALO, HALO + N[0]), slice(HALO, HALO + N[1]), slice(HALO, HALO + N[2]))
@@ -195,13 +199,20 @@ def shock_parameters(cfl, c_omp, ppc, gamma_flow, sigma, m0=1.0, m1=1.0):
     return q0, q1, B0, -beta * B0
 
 
-def boost_flow(u, gamma_flow):
-    """Rest-frame 4-velocities [3, n] boosted to a frame moving along -x with
-    Lorentz factor gamma_flow: u_x' = G (u_x - beta g), u_y, u_z unchanged."""
+def boost_flow(u, gamma_flow, gen):
+    """Rest-frame 4-velocities [3, n] as the flow of a frame moving along -x
+    with Lorentz factor gamma_flow, by Zenitani's flipping method
+    (runko/sample_thermal_distributions.py:106-111, direction '-x'): u_x
+    changes sign where -beta v_x > U (U uniform), which weights each
+    particle by the flux factor of the boosted frame, then
+    u_x' = -G (u_x + beta g)."""
     beta = math.sqrt(1.0 - 1.0 / gamma_flow**2)
     g = torch.sqrt(1.0 + (u * u).sum(0))
+    vx = u[0] / g
+    U = torch.rand(u.shape[1], generator=gen, device=u.device, dtype=u.dtype)
+    ux = torch.where(-beta * vx > U, -u[0], u[0])
     out = u.clone()
-    out[0] = gamma_flow * (u[0] - beta * g)
+    out[0] = -(gamma_flow * (ux + beta * g))
     return out
 
 
@@ -274,7 +285,8 @@ def fill_shock_plasma(world, ppc, theta, gamma_flow, walloc, ramp_len, ramp_fact
         for s in range(world.nsp):
             pool = world.pools[s]
             dst = pool.start[brick_of] + slot
-            u = boost_flow(juttner_momenta(total, theta, gen, dev).to(torch.float64), gamma_flow).to(torch.float32)
+            u = boost_flow(juttner_momenta(total, theta, gen, dev).to(torch.float64), gamma_flow,
+                           gen).to(torch.float32)
             for d in range(3):
                 pool.pos[d][dst] = pos[d]
                 pool.vel[d][dst] = u[d]

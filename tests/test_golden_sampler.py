@@ -54,3 +54,29 @@ def test_vectors_are_not_trivial():
     u = sampling.sample_boosted_juttner_synge(300, 0.3, beta=0.0, direction="x", gen=np.random.default_rng(12))
     assert not np.array_equal(np.stack(u), V["boosted0_u"])
     assert V["sobol0"].shape == (400,) and np.all(V["sobol0"] > 0)
+
+
+def test_boosted_sampler_matches_lab_frame_moments():
+    # the flipping method gives the lab-frame Juttner-Synge law
+    # exp(-G (gamma + beta u_x) / theta) d^3u of a flow along -x; its <u_x>
+    # and <gamma> by quadrature in (u_par, u_perp)
+    import math
+
+    from scipy import integrate
+
+    theta, G = 0.5, 3.0
+    beta = math.sqrt(1 - 1 / G**2)
+
+    def moment(f):
+        def dens(up, ux):
+            g = math.sqrt(1 + ux * ux + up * up)
+            return 2 * math.pi * up * math.exp(-(G * (g + beta * ux) - 1) / theta)
+        z = integrate.dblquad(dens, -40, 40, 0, 40, epsabs=1e-12)[0]
+        return integrate.dblquad(lambda up, ux: f(ux, up) * dens(up, ux), -40, 40, 0, 40, epsabs=1e-12)[0] / z
+
+    n = 1_000_000
+    ux, uy, uz = sampling.sample_boosted_juttner_synge(n, theta, Gamma=G, direction="-x",
+                                                       gen=np.random.default_rng(3))
+    g = np.sqrt(1 + ux**2 + uy**2 + uz**2)
+    for sample, expect in ((ux, moment(lambda a, b: a)), (g, moment(lambda a, b: math.sqrt(1 + a * a + b * b)))):
+        assert abs(sample.mean() - expect) < 5 * sample.std() / math.sqrt(n)

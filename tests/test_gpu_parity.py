@@ -213,6 +213,45 @@ def test_sort_keeps_particles(orc):
             assert np.array_equal(a, b)
 
 
+def test_sort_key_order_matches_oracle(orc):
+    # pic/tile.c++:417-438 -> particle.h:574-703: after sort_particles the
+    # storage order follows the haloed layout_right cell key.  The GPU sorts
+    # inside each brick bin; restricted to a bin, the oracle's sorted order
+    # must visit the same cells in the same sequence (ties inside a cell are
+    # unordered in both).  The injection order is a random permutation, so
+    # a sort that did nothing fails.
+    nt, n = (2, 1, 2), (12, 8, 10)
+    inputs = random_inputs(nt, n, 2, seed=21, ppc=4)
+    conf, grid, sim = build_gpu(nt, n, Q, M, inputs)
+    w = build_oracle(orc, nt, n, Q, M, inputs)
+    for tile in grid.tiles():
+        tile.sort_particles()
+    w.sort_particles(-1)
+    world = grid.world
+    checked = 0
+    for s in range(2):
+        counts = world.count[s].cpu().numpy()
+        for t in range(world.ntiles):
+            ost, oid = w.get_particles(t, s)
+            rank = {int(i): r for r, i in enumerate(oid)}
+            pos, vel, ids = world.tile_particles(t, s)
+            pos = pos.cpu().numpy()
+            ids = ids.cpu().numpy().view(np.uint64)
+            cells = np.floor(pos).astype(np.int64).T
+            assert sorted(ids.tolist()) == sorted(int(i) for i in oid)
+            o = 0
+            for b in range(world.bpt):
+                c = int(counts[t * world.bpt + b])
+                g_cells = [tuple(v) for v in cells[o:o + c]]
+                r = np.array([rank[int(i)] for i in ids[o:o + c]])
+                o_cells = [g_cells[k] for k in np.argsort(r, kind="stable")]
+                assert g_cells == o_cells, f"species {s} tile {t} bin {b}"
+                assert len(set(g_cells)) > 1
+                checked += c
+                o += c
+    assert checked == sum(len(p) for _, ps in inputs.values() for p in ps)
+
+
 def test_bins_grow_on_demand():
     # every particle of a tile injected into one brick: the bins repack and nothing is lost
     conf = make_config((1, 1, 1), (16, 16, 16), Q, M)
