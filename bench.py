@@ -266,45 +266,65 @@ SHOCK_INI_STENCIL = {
 def run_shock(args):
     """Reflecting-wall shock: nx x ny x nz cells in tile^3 tiles, gamma 10 cold
     flow along -x, 8 ppc/species upstream, density ramped 4x toward the wall
-    (512 cells), theta 1e-4, sigma 0.1 with B perpendicular to the flow.  The
-    pic-shock lap (edge conditions, reflect, the rest of the PIC step), on
-    the same world, back to back, in four forms: replayed from captured HIP
-    graphs (fused particle step with the wall inside), the same eagerly, the
-    batched form with the particle methods as separate launches, and the
-    plain per-tile form (one call per tile per method, each method
-    synchronous).  Bins that fill up are repacked inside the timed region
-    (counted in `repacks`)."""
+    (512 cells), theta 1e-4, sigma 0.1 with B perpendicular to the flow, the
+    plasma reaching the moving injector's front (walls, edge conditions and
+    injector of projects/pic-shock/pic.py:133-152).  The pic-shock lap (edge
+    conditions, reflect, the rest of the PIC step, then the injector every 50
+    laps), on the same world, back to back, in four forms: replayed from
+    captured HIP graphs (fused particle step with the wall inside), the same
+    eagerly, the batched form with the particle methods as separate launches,
+    and the plain per-tile form (one call per tile per method, each method
+    synchronous).  Bins that fill up are regrown inside the timed region
+    (counted in `repacks`); the injector runs inside it too (eagerly, between
+    replays)."""
     import gc
 
     import numpy as np
     import torch
 
-    from runko_amd.init import shock_world
+    from runko_amd.init import WorldInjector, shock_world
     from runko_amd.stepper import GraphLap, PlainShockLap, ShockLap
 
     gc.collect()
     torch.cuda.empty_cache()
     nx, ny, nz = args.shock_cells
+    injloc = args.shock_injloc if args.shock_injloc > 0 else None
     t0 = time.perf_counter()
     w, walls, bcs, par = shock_world(nx=nx, ny=ny, nz=nz, tile=args.tile, ppc=8, theta=1e-4, gamma_flow=10.0,
-                                     sigma=0.1, seed=3)
+                                     sigma=0.1, seed=3, injloc=injloc)
     npart = w.total_particles()
     per_tile = np.array([w.tile_count(t, 0) + w.tile_count(t, 1) for t in range(w.ntiles)])
     occupied = per_tile[per_tile > 0]
     log(f"[bench] shock {nx}x{ny}x{nz}, {w.ntiles} tiles, {npart} particles, tile occupancy "
-        f"{occupied.min()}..{occupied.max()} (+{int((per_tile == 0).sum())} empty), "
-        f"setup {time.perf_counter() - t0:.1f}s")
+        f"{occupied.min()}..{occupied.max()} ({occupied.max() / occupied.min():.1f}x, "
+        f"+{int((per_tile == 0).sum())} empty), setup {time.perf_counter() - t0:.1f}s")
+    beta = (1.0 - 1.0 / 100.0) ** 0.5
+    inj = None
+    if injloc is not None:
+        inj = WorldInjector(w, injloc=injloc, beta_flow=beta, ppc=8, theta=1e-4, gamma_flow=10.0, walloc=15.0,
+                            n_inj=50, Lx=float(nx))
     res = dict(workload=f"reflecting-wall shock, {nx}x{ny}x{nz} cells in {args.tile}^3 tiles, gamma 10 flow in -x, "
                         f"8 ppc/species upstream, 4x density ramp over 512 cells at the wall, theta 1e-4, "
-                        f"sigma 0.1 perpendicular B, conducting + upstream edge conditions, FP32",
+                        f"sigma 0.1 perpendicular B, conducting + upstream edge conditions, FP32"
+                        + (f", plasma up to the moving injector at x = {injloc} (every 50 laps)" if inj else ""),
                particles=npart, tiles=w.ntiles, tile_particles_min=int(occupied.min()),
-               tile_particles_max=int(occupied.max()), tiles_empty=int((per_tile == 0).sum()))
+               tile_particles_max=int(occupied.max()), occupancy_spread=float(occupied.max() / occupied.min()),
+               tiles_empty=int((per_tile == 0).sum()))
+    lapno = [0]
+
+    def after_lap():
+        lapno[0] += 1
+        if inj is not None:
+            inj.inject(lapno[0])
+
     fused = ShockLap(w, walls, bcs)
     fused()
     fused.settle()
+    after_lap()
     if not args.no_autotune:
         from runko_amd.autotune import tune_fused_launch
         best, times = tune_fused_launch(w, fused)
+        lapno[0] += 2  # the tuner ran one lap per launch form
         res["fused_launch"] = dict(chosen=best, k_fused_ms=times)
         log(f"[bench] shock fused launch: {best} ({', '.join(f'{k} {v:.2f} ms' for k, v in times.items())})")
     runs = [("graph", lambda: GraphLap(w, lap=fused), args.shock_steps),
@@ -316,19 +336,29 @@ def run_shock(args):
         lap = make()
         lap()
         lap.settle()
+        after_lap()
         torch.cuda.synchronize()
         r0, s0 = w.repacks, w.settle_seconds
+        e0, i0 = (inj.events, inj.injected) if inj else (0, 0)
+        n_start = w.total_particles()
         t1 = time.perf_counter()
         for i in range(steps):
             lap()
             lap.settle()
+            after_lap()
         torch.cuda.synchronize()
         dt = time.perf_counter() - t1
-        assert w.total_particles() == npart, "particle count changed (shock)"
-        res[name] = dict(steps=steps, ms_per_step=1e3 * dt / steps, particle_steps_per_s=npart * steps / dt,
-                         repacks=w.repacks - r0, repack_ms_per_step=1e3 * (w.settle_seconds - s0) / steps)
-        log(f"[bench] shock {name}: {1e3 * dt / steps:.2f} ms/step, {npart * steps / dt:.3e} p-s/s, "
-            f"{w.repacks - r0} repacks ({1e3 * (w.settle_seconds - s0) / steps:.2f} ms/step)")
+        events, added = (inj.events - e0, inj.injected - i0) if inj else (0, 0)
+        n_end = w.total_particles()
+        assert n_end == n_start + added, "particle count changed (shock)"
+        nmean = 0.5 * (n_start + n_end)
+        res[name] = dict(steps=steps, ms_per_step=1e3 * dt / steps, particle_steps_per_s=nmean * steps / dt,
+                         particles_start=n_start, particles_end=n_end, injector_events=events,
+                         injected=added, repacks=w.repacks - r0,
+                         repack_ms_per_step=1e3 * (w.settle_seconds - s0) / steps)
+        log(f"[bench] shock {name}: {1e3 * dt / steps:.2f} ms/step, {nmean * steps / dt:.3e} p-s/s, "
+            f"{w.repacks - r0} repacks ({1e3 * (w.settle_seconds - s0) / steps:.2f} ms/step), "
+            f"{events} injector events (+{added} particles)")
         if hasattr(lap, "recaptures"):
             res[name].update(recaptures=lap.recaptures, recapture_ms_per_step=1e3 * lap.recapture_seconds / steps)
             log(f"[bench] shock {name}: {lap.recaptures} graph recaptures "
@@ -337,8 +367,10 @@ def run_shock(args):
     if "plain_per_tile" in res:
         for name in ("graph", "batched_fused", "batched_unfused"):
             res[f"{name}_vs_plain"] = res["plain_per_tile"]["ms_per_step"] / res[name]["ms_per_step"]
+    res["laps_run"] = lapno[0]
     res["error_counters"] = [int(v) for v in w.error_counters()]
-    del runs, fused, w
+    res["pools"] = w.memory_report()
+    del runs, fused, w, inj
     gc.collect()
     torch.cuda.empty_cache()
     if not args.no_shock_schemes:
@@ -393,7 +425,9 @@ def run_ensemble(args):
 
     gc.collect()
     torch.cuda.empty_cache()
-    steps, every = args.ensemble_steps, args.ensemble_every
+    from runko_amd.diff import checkpoint_states, choose_checkpoints
+
+    steps = args.ensemble_steps
     t0 = time.perf_counter()
     prob = EnsembleProblem(n_boxes=args.ensemble_boxes, seed=5)
     box = prob.box
@@ -402,8 +436,17 @@ def run_ensemble(args):
     amp = torch.full((nb, prob.n_modes), 0.5, dtype=torch.float64, device=dev).requires_grad_(True)
     prob.loss(amp, 2, 1).backward()  # warm-up
     torch.cuda.synchronize()
-    log(f"[bench] ensemble: {nb} boxes, {box.np} particles, {box.ncell} cells, setup {time.perf_counter() - t0:.1f}s")
+    gc.collect()
+    torch.cuda.empty_cache()
+    state_bytes = sum(t.numel() * t.element_size() for t in box.empty_state())
+    free = torch.cuda.mem_get_info()[0]
+    # checkpoints of the rollout: flat if they fit 90 % of the free memory, else nested
+    every = args.ensemble_every if args.ensemble_every > 0 else choose_checkpoints(steps, state_bytes, 0.9 * free)
+    log(f"[bench] ensemble: {nb} boxes, {box.np} particles, {box.ncell} cells, setup {time.perf_counter() - t0:.1f}s; "
+        f"state {state_bytes / 2**30:.2f} GiB, {free / 2**30:.0f} GiB free: checkpoints {every} "
+        f"({checkpoint_states(steps, every)} states at the peak)")
     amp.grad = None
+    torch.cuda.reset_peak_memory_stats()
     t0 = time.perf_counter()
     E, B, x, u = prob.initial_state(amp)
     out = rollout(box, E, B, x, u, steps, every)
@@ -414,7 +457,8 @@ def run_ensemble(args):
     torch.cuda.synchronize()
     t2 = time.perf_counter()
     fwd, bwd = t1 - t0, t2 - t1
-    log(f"[bench] ensemble: forward {fwd:.2f}s, backward {bwd:.2f}s for {steps} steps")
+    log(f"[bench] ensemble: forward {fwd:.2f}s, backward {bwd:.2f}s for {steps} steps "
+        f"(peak {torch.cuda.max_memory_allocated() / 2**30:.0f} GiB)")
     state = tuple(t.detach().contiguous() for t in out)
     del out, loss, E, B, x, u
     spare = box.empty_state()
@@ -453,8 +497,9 @@ def run_ensemble(args):
     res = dict(
         workload=f"batched ensemble: {nb} periodic pair plasmas, sides from {{32,48,64,96}}, ppc/species from "
                  f"{{4,8,16}}, theta log-uniform [0.05,3], sigma log-uniform [0.1,10], 6 seeded dB modes each; "
-                 f"loss = sum of final magnetic energies after {steps} steps, checkpoints every {every}, FP32",
+                 f"loss = sum of final magnetic energies after {steps} steps, checkpoints {every}, FP32",
         boxes=nb, particles=box.np, cells=box.ncell, steps=steps, checkpoint_every=every,
+        checkpoint_states=checkpoint_states(steps, every), state_gib=state_bytes / 2**30,
         forward_s=fwd, backward_s=bwd, adjoint_over_forward=bwd / fwd,
         forward_ms_per_step=f_ms, adjoint_ms_per_step=a_ms, adjoint_step_over_forward_step=a_ms / f_ms,
         forward_particle_steps_per_s=box.np * steps / fwd,
@@ -573,14 +618,18 @@ def main():
     ap.add_argument("--no-shock-schemes", action="store_true",
                     help="skip the shock run with the reference configuration's schemes")
     ap.add_argument("--shock-cells", type=int, nargs=3, default=[2048, 128, 128], metavar=("NX", "NY", "NZ"))
-    ap.add_argument("--shock-steps", type=int, default=40)
+    ap.add_argument("--shock-steps", type=int, default=50)
+    ap.add_argument("--shock-injloc", type=float, default=1484.0,
+                    help="initial injector front of the shock (0: the plasma fills the box, no injector)")
     ap.add_argument("--shock-plain-steps", type=int, default=3)
     ap.add_argument("--no-ensemble", action="store_true", help="skip the batched ensemble workload")
     ap.add_argument("--only-ensemble", action="store_true",
                     help="run only the ensemble workload and print its result object")
     ap.add_argument("--ensemble-boxes", type=int, default=64)
-    ap.add_argument("--ensemble-steps", type=int, default=60)
-    ap.add_argument("--ensemble-every", type=int, default=8)
+    ap.add_argument("--ensemble-steps", type=int, default=300)
+    ap.add_argument("--ensemble-every", type=int, default=0,
+                    help="flat checkpoint spacing of the ensemble rollout (0: chosen from free memory, nested if "
+                         "flat does not fit)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="also run plain copies of MIB MiB (scale for rocprof byte counters)")
     args = ap.parse_args()

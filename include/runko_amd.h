@@ -129,8 +129,8 @@ int rk_world_set_current_staging(rk_world* w, float* buf, int64_t stride);
  * (particle_faraday.h:37-109); stencil = null for the fdtd2 B push, else 45
  * floats [axis x,y,z][row 0..2][column 0..4], the StencilAxisCoeffs matrices
  * with M[0][0] normalised (stencil_coefficients.h:42-60), for push_b_stencil
- * (emf/yee_lattice_stencil.c++:17-297).  The fused particle step implements
- * boris only (RK_ESTATE otherwise). */
+ * (emf/yee_lattice_stencil.c++:17-297).  The fused particle step runs every
+ * pusher, with or without reflecting walls. */
 int rk_world_set_schemes(rk_world* w, int32_t particle_pusher, const float* stencil);
 
 /* Reflecting wall (conducting piston) normal to x, pic::reflector_wall
@@ -293,6 +293,14 @@ int rk_outgoing(rk_world* w, float* const pos[3], float* const vel[3], uint64_t*
  * out[0] = sum(E^2)/2, out[1] = sum(B^2)/2, out[2+s] = sum_alive(gamma-1).
  * Synchronous.  src/runko/emf/yee_lattice.c++:383-428, src/runko/pic/particle.c++:351-377 */
 int rk_energies(rk_world* w, int32_t tile, double* out, void* stream);
+/* rk_energies without allocation, host copy or synchronisation: the same
+ * values (bit for bit) written to out_dev (device double[2 + n_species]) on
+ * `stream`.  No counterpart in the reference: it is for callers that cannot
+ * allocate (HIP graph capture, serving loops).  Its partial sums live in one
+ * 160 KiB device array per GPU (at most 8192 x planes per region, held
+ * until the process ends), resolved when the first world is created on the
+ * device.  Calls that overlap on one GPU race on that array. */
+int rk_energies_device(rk_world* w, int32_t tile, double* out_dev, void* stream);
 
 /* Gauss-law residual (FP64): rho = first-order (CIC) charge on the nodes where
  * div E is centred, optionally passed through the same number of binomial

@@ -192,43 +192,130 @@ class DiffEnsemble(DiffBox):
         return x[:, self.p0[i]:self.p0[i] + self.nps[i]]
 
 
+def checkpoint_levels(every):
+    """Normalised checkpoint spacing: an int K (a checkpoint every K steps,
+    each segment's states recomputed and kept whole in the backward pass) or
+    a decreasing tuple (K1, K2, ...): checkpoints every K1 steps in the
+    forward pass, and each K1 segment recomputed with checkpoints every K2
+    steps, and so on, the last level kept whole.  Peak memory is about
+    steps/K1 + K1/K2 + ... + K_last states; every level below the first
+    adds one forward pass of recomputation."""
+    levels = (int(every),) if np.isscalar(every) else tuple(int(k) for k in every)
+    if not levels or any(k < 1 for k in levels) or any(a < b for a, b in zip(levels, levels[1:])):
+        raise ValueError("rollout: checkpoint spacing must be a positive int or a decreasing tuple of them")
+    return levels
+
+
+def checkpoint_states(steps, every):
+    """States held at the peak of a rollout's backward pass (the checkpoints
+    of every level plus the last level's whole segment)."""
+    levels = checkpoint_levels(every)
+    n, out = int(steps), 0
+    for k in levels:
+        out += -(-n // k)
+        n = min(k, n)
+    return out + n
+
+
+def choose_checkpoints(steps, state_bytes, budget_bytes, reserve_states=4, max_levels=4):
+    """The checkpoint spacing with the least recomputation whose peak states
+    (plus `reserve_states` working states) fit budget_bytes: flat when it
+    fits, else the fewest levels that fit (each level below the first costs
+    one more forward pass), and among those the spacing holding the fewest
+    states."""
+    steps = int(steps)
+    fit = int(budget_bytes // max(int(state_bytes), 1)) - reserve_states
+    memo = {}
+
+    def best(n, depth):
+        # (states, spacing) for a segment of n steps with `depth` levels
+        if depth == 0 or n <= 1:
+            return n, ()
+        key = (n, depth)
+        if key not in memo:
+            out = (n, ())  # kept whole
+            for k in range(1, n):
+                c, sub = best(k, depth - 1)
+                c += -(-n // k)
+                if c < out[0]:
+                    out = (c, (k,) + sub)
+            memo[key] = out
+        return memo[key]
+
+    for depth in range(1, max_levels + 1):
+        c, spacing = best(steps, depth)
+        if c <= fit:
+            return spacing[0] if len(spacing) == 1 else spacing
+    raise RuntimeError(f"rollout: {budget_bytes / 2**30:.1f} GiB hold {fit} states of {state_bytes / 2**30:.2f} "
+                       f"GiB beside the working ones; no spacing of up to {max_levels} levels fits")
+
+
+def _adjoint_segment(box, s0, n, levels, lam):
+    """lam holds the adjoint of the state n steps after s0; on return it
+    holds the adjoint of s0.  The forward states are recomputed from s0:
+    kept whole when no level is left, else with checkpoints every levels[0]
+    steps, each sub-segment handled by the next level."""
+    if not levels or levels[0] >= n:
+        states = [s0]
+        for _ in range(n - 1):
+            states.append(box.step(states[-1]))
+        for st in reversed(states):
+            box.step_adjoint(st, lam, lam)
+        return
+    k = levels[0]
+    ckpts = [s0]
+    s = s0
+    for t in range(k, n, k):
+        for i in range(k):
+            s = box.step(s)
+        ckpts.append(s)
+    del s
+    for i in reversed(range(len(ckpts))):
+        _adjoint_segment(box, ckpts[i], min(k, n - i * k), levels[1:], lam)
+        ckpts[i] = None
+
+
 class _Rollout(torch.autograd.Function):
     @staticmethod
     def forward(ctx, box, steps, every, E, B, x, u):
+        levels = checkpoint_levels(every)
         s = tuple(t.detach().contiguous().clone() for t in (E, B, x, u))
         spare = box.empty_state()
         ckpts = []
         for t in range(steps):
-            if t % every == 0:
+            if t % levels[0] == 0:
                 ckpts.append(tuple(v.clone() for v in s))
             box.step(s, spare)
             s, spare = spare, s
-        ctx.box, ctx.steps, ctx.every, ctx.ckpts = box, steps, every, ckpts
+        ctx.box, ctx.steps, ctx.levels, ctx.ckpts = box, steps, levels, ckpts
         return s
 
     @staticmethod
     def backward(ctx, gE, gB, gx, gu):
-        box, steps, every = ctx.box, ctx.steps, ctx.every
+        box, steps, levels = ctx.box, ctx.steps, ctx.levels
         ref = box.empty_state()
         lam = tuple(torch.zeros_like(r) if g is None else g.detach().contiguous().clone()
                     for g, r in zip((gE, gB, gx, gu), ref))
+        del ref
+        k = levels[0]
         for seg in reversed(range(len(ctx.ckpts))):
-            t0 = seg * every
-            t1 = min(t0 + every, steps)
-            states = [ctx.ckpts[seg]]
-            for _ in range(t0, t1 - 1):
-                states.append(box.step(states[-1]))
-            for st in reversed(states):
-                box.step_adjoint(st, lam, lam)
+            t0 = seg * k
+            n = min(t0 + k, steps) - t0
+            s0 = ctx.ckpts[seg]
             ctx.ckpts[seg] = None
+            _adjoint_segment(box, s0, n, levels[1:], lam)
+            del s0
         return (None, None, None) + lam
 
 
 def rollout(box, E, B, x, u, steps, checkpoint_every=20):
-    """State after `steps` steps, differentiable w.r.t. E, B, x, u."""
-    if steps < 1 or checkpoint_every < 1:
+    """State after `steps` steps, differentiable w.r.t. E, B, x, u.
+    checkpoint_every: an int, or a decreasing tuple for nested checkpoints
+    (checkpoint_levels)."""
+    if steps < 1:
         raise ValueError("rollout: steps and checkpoint_every must be positive")
-    return _Rollout.apply(box, int(steps), int(checkpoint_every), E, B, x, u)
+    checkpoint_levels(checkpoint_every)
+    return _Rollout.apply(box, int(steps), checkpoint_every, E, B, x, u)
 
 
 # ---------------------------------------------------------------------- the workload

@@ -216,19 +216,24 @@ def boost_flow(u, gamma_flow, gen):
     return out
 
 
-def fill_shock_plasma(world, ppc, theta, gamma_flow, walloc, ramp_len, ramp_factor, seed=1, chunk_bins=2048):
+def fill_shock_plasma(world, ppc, theta, gamma_flow, walloc, ramp_len, ramp_factor, seed=1, chunk_bins=2048,
+                      injloc=None):
     """Cold flow along -x in front of a wall at x = walloc: ppc particles per
     cell per species upstream, ramp_factor * ppc at the wall falling linearly
     to ppc over ramp_len cells (a compressed layer, so tiles near the wall hold
-    several times the upstream occupancy), none behind the wall.  Positions
-    uniform in each cell (x >= walloc), species 1 on top of species 0, rest
-    frame momenta from the Juttner-Synge law at theta boosted by gamma_flow."""
+    several times the upstream occupancy), none behind the wall, and none from
+    the injector front on (cells from ceil(injloc), the whole-cell rule of
+    pic/tile.c++:239-249; None: the plasma fills the box).  Positions uniform
+    in each cell (x >= walloc), species 1 on top of species 0, rest frame
+    momenta from the Juttner-Synge law at theta boosted by gamma_flow."""
     dev = world.dev
     br = world.brick
     N = world.N
     xc = np.arange(N[0], dtype=np.float64)
     factor = 1.0 + (ramp_factor - 1.0) * np.clip(1.0 - (xc - walloc) / ramp_len, 0.0, 1.0)
     cnt_x = np.where(xc + 1.0 <= walloc, 0, np.rint(ppc * factor)).astype(np.int64)
+    if injloc is not None:
+        cnt_x[xc >= math.ceil(injloc)] = 0
     nbd = [world.n[a] // br[a] for a in range(3)]
     # particles per brick: bricks are tile-major, layout_right inside the tile
     b = np.arange(world.nbricks)
@@ -298,7 +303,7 @@ def fill_shock_plasma(world, ppc, theta, gamma_flow, walloc, ramp_len, ramp_fact
 
 def shock_world(nx=2048, ny=128, nz=128, tile=32, ppc=8, theta=1e-4, gamma_flow=10.0, sigma=0.1, cfl=0.45,
                 c_omp=10.0, walloc=15.0, ramp_len=512.0, ramp_factor=4.0, seed=1, brick=None, mig_fraction=0.12,
-                pusher="boris", stencil=None, current_filter="binomial2"):
+                pusher="boris", stencil=None, current_filter="binomial2", injloc=None):
     """Reflecting-wall shock set-up: nx x ny x nz cells in tile^3 tiles, a cold
     gamma_flow flow along -x with a perpendicular field (B0 along z, motional
     E_y), a wall at walloc with the conducting and upstream edge conditions
@@ -317,7 +322,7 @@ def shock_world(nx=2048, ny=128, nz=128, tile=32, ppc=8, theta=1e-4, gamma_flow=
     w.F.zero_()
     w.F[0][1][inner] = float(Ey)
     w.F[1][2][inner] = float(B0)
-    fill_shock_plasma(w, ppc, theta, gamma_flow, walloc, ramp_len, ramp_factor, seed=seed)
+    fill_shock_plasma(w, ppc, theta, gamma_flow, walloc, ramp_len, ramp_factor, seed=seed, injloc=injloc)
     w.ensure_migration(max(1 << 16, int(mig_fraction * w.total_particles())))
     w.comm_local(1)
     w.comm_local(2)
@@ -329,3 +334,110 @@ def shock_world(nx=2048, ny=128, nz=128, tile=32, ppc=8, theta=1e-4, gamma_flow=
     upstream = L.EdgeBC(0, 1, float(np.float32(Lx - 5)), (0.0, float(np.float32(Ey)), 0.0),
                         (0.0, 0.0, float(np.float32(B0))), (0.0, 0.0, 0.0), 0b111, 0b111, 0b111)
     return w, [wall], [conducting, upstream], dict(q0=q0, q1=q1, B0=B0, Ey=Ey)
+
+
+class _StripeSink:
+    """Stands in for the tiles of a simulation: records the stripe that
+    MovingInjector.inject asks the tiles to fill."""
+
+    def __init__(self):
+        self.stripe = None
+
+    def batch_inject_in_x_stripe(self, species, pgen, x_left, x_right):
+        self.stripe = (x_left, x_right)
+
+
+class _LapCounter:
+    def __init__(self, lap, sink):
+        self.lap = lap
+        self._sink = sink
+
+    def local_tiles(self):
+        return [self._sink]
+
+
+class WorldInjector:
+    """The moving injector of runko/moving_injector.py:24-54 for a world
+    stepped as a whole.  The front bookkeeping is runko_amd.MovingInjector's
+    own (every n_inj laps the stripe [max(injloc - beta_flow n_inj cfl,
+    walloc), injloc + beta_inj n_inj cfl), the front then at its right edge,
+    and no more injection once the edge reaches Lx - Lx_margin); the stripe
+    is filled on the GPU: the whole cells it touches (pic/tile.c++:239-249),
+    ppc passes of one particle per cell per species, positions uniform in the
+    cell, species 1 on top of species 0, rest-frame Juttner-Synge momenta at
+    theta boosted along -x by gamma_flow (flipping method), as the shock
+    driver's generators draw them (projects/pic-shock/pic.py:155-168).  Ids
+    are (tile tag << 40) | a per-tile, per-species counter that starts at
+    2^39, above every ordinal the set-up uses."""
+
+    ORDINAL0 = 1 << 39
+
+    def __init__(self, world, injloc, beta_flow, ppc, theta, gamma_flow, walloc=0.0, beta_inj=1.0, n_inj=50,
+                 Lx=None, Lx_margin=10.0, seed=11):
+        from .moving_injector import MovingInjector
+        self.w = world
+        self.front = MovingInjector(injloc=injloc, beta_inj=beta_inj, beta_flow=beta_flow, cfl=world.cfl, n_inj=n_inj,
+                                    walloc=walloc, Lx_margin=Lx_margin, Lx=Lx)
+        self.ppc, self.theta, self.gamma_flow = int(ppc), float(theta), float(gamma_flow)
+        self.gen = torch.Generator(device=world.dev)
+        self.gen.manual_seed(int(seed))
+        self.next = np.full((world.ntiles, world.nsp), self.ORDINAL0, dtype=np.int64)
+        self.injected = 0
+        self.events = 0
+        self.stripes = []
+
+    @property
+    def injloc(self):
+        return self.front.injloc
+
+    @property
+    def moving(self):
+        return self.front.moving
+
+    def inject(self, lap):
+        """Call once per lap with the lap counter; returns the number of
+        particles injected."""
+        sink = _StripeSink()
+        self.front.inject(_LapCounter(lap, sink), [(s, None) for s in range(self.w.nsp)], 1)
+        if sink.stripe is None:
+            return 0
+        n = self.fill_stripe(*sink.stripe)
+        self.stripes.append(sink.stripe)
+        self.injected += n
+        self.events += 1
+        return n
+
+    def fill_stripe(self, x_left, x_right):
+        w = self.w
+        dev = w.dev
+        N = w.N
+        i0 = max(int(math.floor(x_left)), 0)
+        i1 = min(int(math.ceil(x_right)), N[0])
+        total = 0
+        if i0 >= i1:
+            return 0
+        n1, n2 = w.n[1], w.n[2]
+        for tx in range(i0 // w.n[0], (i1 - 1) // w.n[0] + 1):
+            a0, a1 = max(i0, tx * w.n[0]), min(i1, (tx + 1) * w.n[0])
+            for ty in range(w.nt[1]):
+                for tz in range(w.nt[2]):
+                    t = (tx * w.nt[1] + ty) * w.nt[2] + tz
+                    cx = torch.arange(a0, a1, device=dev, dtype=torch.float64)
+                    cy = torch.arange(ty * n1, (ty + 1) * n1, device=dev, dtype=torch.float64)
+                    cz = torch.arange(tz * n2, (tz + 1) * n2, device=dev, dtype=torch.float64)
+                    base = torch.stack(torch.meshgrid(cx, cy, cz, indexing="ij"), 0).reshape(3, 1, -1)
+                    base = base.expand(3, self.ppc, base.shape[2]).reshape(3, -1)  # pass-major, like ppc calls
+                    pos = base + torch.rand(base.shape, generator=self.gen, device=dev, dtype=torch.float64)
+                    upper = (base + 1.0).to(torch.float32)
+                    pos = pos.to(torch.float32)
+                    pos = torch.where(pos >= upper, torch.nextafter(upper, torch.zeros_like(upper)), pos)
+                    m = pos.shape[1]
+                    w.ensure_migration(m)
+                    for s in range(w.nsp):
+                        u = boost_flow(juttner_momenta(m, self.theta, self.gen, dev).to(torch.float64),
+                                       self.gamma_flow, self.gen).to(torch.float32)
+                        ids = (int(t) << 40) | (int(self.next[t, s]) + torch.arange(m, device=dev, dtype=torch.int64))
+                        self.next[t, s] += m
+                        w.inject(t, s, pos, u, ids)
+                    total += w.nsp * m
+        return total

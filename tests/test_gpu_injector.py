@@ -122,3 +122,84 @@ def test_moving_injector_fills_grid_tiles():
     # stripe [8.9, 9.8): cell 8 and 9, twice
     assert [t.number_of_particles(0) for t in grid.tiles()] == [16 + 2 * 32, 0, 0, 0]
     assert grid.world.total_particles() == 16 + 2 * 32 + 16
+
+
+def test_world_injector_fills_the_moving_stripes():
+    # the world-level injector: MovingInjector's front bookkeeping, the stripe
+    # filled on the GPU with ppc particles per touched cell and species
+    import math
+
+    import torch
+
+    from runko_amd.init import WorldInjector, shock_world
+
+    w, walls, bcs, par = shock_world(nx=128, ny=32, nz=32, tile=16, ppc=2, theta=1e-4, gamma_flow=10.0, sigma=0.1,
+                                     walloc=15.0, ramp_len=40.0, ramp_factor=4.0, seed=3, injloc=60.0)
+    # set-up: nothing at or beyond the front's cell ceil(60) = 60
+    for s in range(2):
+        for t in range(w.ntiles):
+            p, _, _ = w.tile_particles(t, s)
+            if p.numel():
+                assert float(p[0].max()) < 60.0
+    beta = math.sqrt(1 - 1 / 100.0)
+    inj = WorldInjector(w, injloc=60.0, beta_flow=beta, ppc=2, theta=1e-4, gamma_flow=10.0, walloc=15.0, n_inj=5,
+                        Lx=128.0)
+    ref = runko.MovingInjector(injloc=60.0, beta_inj=1.0, beta_flow=beta, cfl=0.45, n_inj=5, walloc=15.0, Lx=128.0)
+
+    class Rec:
+        def __init__(self):
+            self.calls = []
+
+        def batch_inject_in_x_stripe(self, s, pgen, a, b):
+            self.calls.append((a, b))
+
+    class Sim:
+        def __init__(self, lap, rec):
+            self.lap, self.rec = lap, rec
+
+        def local_tiles(self):
+            return [self.rec]
+
+    before = w.total_particles()
+    for lap in range(0, 400):
+        rec = Rec()
+        ref.inject(Sim(lap, rec), [(0, None), (1, None)], 2)
+        n = inj.inject(lap)
+        if rec.calls:
+            a, b = rec.calls[0]
+            assert inj.stripes[-1] == (a, b)
+            cells = math.ceil(b) - math.floor(a)
+            assert n == cells * 32 * 32 * 2 * 2, (lap, a, b, n)
+        else:
+            assert n == 0
+    assert not inj.moving and ref.injloc == inj.injloc and inj.events == len(inj.stripes) > 10
+    assert w.total_particles() == before + inj.injected
+    # the injected particles: ids above 2^39 ordinal, tile-tagged, unique;
+    # one per cell and pass, species 1 on top of species 0, cold flow along -x
+    got = {}
+    for s in range(2):
+        ps, vs, ids = [], [], []
+        for t in range(w.ntiles):
+            p, v, i = w.tile_particles(t, s)
+            sel = (i & ((1 << 40) - 1)) >= (1 << 39)
+            assert torch.all((i[sel] >> 40) == t)
+            ps.append(p[:, sel].cpu().numpy())
+            vs.append(v[:, sel].cpu().numpy())
+            ids.append(i[sel].cpu().numpy())
+        p, v, i = np.concatenate(ps, 1), np.concatenate(vs, 1), np.concatenate(ids)
+        assert len(np.unique(i)) == len(i) == inj.injected // 2
+        assert np.all(v[0] < 0) and abs(v[0].mean() + 10.0 * beta) < 0.01
+        o = np.argsort(i)
+        got[s] = (p[:, o], i[o])
+        lo = min(a for a, b in inj.stripes)
+        assert p[0].min() >= math.floor(lo)
+    assert np.array_equal(got[0][1], got[1][1]) and np.array_equal(got[0][0], got[1][0])
+    # per event and cell: exactly ppc particles per species
+    cells = np.floor(got[0][0]).astype(np.int64)
+    key = (cells[0] * 32 + cells[1]) * 32 + cells[2]
+    cnt = np.bincount(key)
+    expect = np.zeros_like(cnt)
+    for a, b in inj.stripes:
+        for x in range(math.floor(a), math.ceil(b)):
+            expect[x * 1024:(x + 1) * 1024] += 2
+    assert np.array_equal(cnt, expect[:len(cnt)]) and expect[len(cnt):].sum() == 0
