@@ -293,6 +293,15 @@ int rk_outgoing(rk_world* w, float* const pos[3], float* const vel[3], uint64_t*
  * out[0] = sum(E^2)/2, out[1] = sum(B^2)/2, out[2+s] = sum_alive(gamma-1).
  * Synchronous.  src/runko/emf/yee_lattice.c++:383-428, src/runko/pic/particle.c++:351-377 */
 int rk_energies(rk_world* w, int32_t tile, double* out, void* stream);
+/* rk_energies without allocation, host copy or synchronisation: the same
+ * values, bit for bit, written to out_dev (device double[2 + n_species]) on
+ * `stream`.  No counterpart in the reference: it is for callers that cannot
+ * allocate device memory (code under HIP graph capture, serving loops,
+ * harnesses that forbid hipMalloc).  Its partial sums live in one 160 KiB
+ * static device array per GPU (a region of at most 8192 x planes; RK_EINVAL
+ * beyond), held until the process ends.  Calls that overlap on one GPU race
+ * on that array. */
+int rk_energies_device(rk_world* w, int32_t tile, double* out_dev, void* stream);
 
 /* Gauss-law residual (FP64): rho = first-order (CIC) charge on the nodes where
  * div E is centred, optionally passed through the same number of binomial

@@ -129,6 +129,7 @@ def lib():
     L.rk_tile_particles.argtypes = [vp, i32, i32, P3, P3, vp, i64, ctypes.POINTER(i64), vp]
     L.rk_outgoing.argtypes = [vp, P3, P3, vp, vp, vp, vp, i64, ctypes.POINTER(i64), vp]
     L.rk_energies.argtypes = [vp, i32, vp, vp]
+    L.rk_energies_device.argtypes = [vp, i32, vp, vp]
     L.rk_gauss_residual.argtypes = [vp, i32, vp, vp]
     L.rk_error_counters.argtypes = [vp, vp, vp]
     L.rk_diff_create.argtypes = [ctypes.POINTER(vp), ctypes.POINTER(DiffDesc)]
@@ -165,7 +166,7 @@ EXPORTED = [
     "rk_particle_step_fused", "rk_particle_step_fused_walls", "rk_world_set_fused_launch", "rk_world_set_fused_workgroup", "rk_world_set_walls",
     "rk_particle_step_fused_device_walls", "rk_advance_walls", "rk_read_walls", "rk_comm_local", "rk_inject", "rk_pending_overflow", "rk_bin_demand",
     "rk_repack", "rk_retry_append", "rk_move_bins", "rk_relayout", "rk_tile_particle_count", "rk_tile_particles", "rk_outgoing",
-    "rk_energies", "rk_gauss_residual", "rk_error_counters", "rk_push_e_add_current", "rk_profile_enable",
+    "rk_energies", "rk_energies_device", "rk_gauss_residual", "rk_error_counters", "rk_push_e_add_current", "rk_profile_enable",
     "rk_profile_read", "rk_world_set_mailbox", "rk_world_set_current_staging", "rk_reflect_particles",
     "rk_apply_edge_bc", "rk_diff_create", "rk_diff_create_ensemble", "rk_diff_destroy", "rk_diff_step", "rk_diff_step_adjoint",
     "rk_diff_error_count", "rk_snapshot_fields", "rk_snapshot_spectra", "rk_interpolate_at",

@@ -130,6 +130,61 @@ int check_lost(rk_world* w, hipStream_t s)
 
 }  // namespace
 
+namespace {
+
+// Partial sums of rk_energies_device: one file-scope device array, of which
+// every GPU holds its own copy until the process ends.  Layout: 2 doubles per
+// x plane of the region (sum E^2, sum B^2), then ENERGY_KIN doubles per
+// species (kinetic partials, one per block).  Host code never names the array
+// in a launch or copy: its address on each device comes from
+// hipGetSymbolAddress, resolved once per device (with that device current)
+// when the first world is created there.
+constexpr int ENERGY_PLANES = 8192;
+constexpr int ENERGY_KIN = 1024;  // >= kinetic_partial_count() for any tile or world
+constexpr int ENERGY_DOUBLES = 2 * ENERGY_PLANES + RK_MAX_SPECIES * ENERGY_KIN;  // 160 KiB
+static __device__ __align__(256) double scratch[ENERGY_DOUBLES];
+
+int resolve_energy_scratch(double** out)
+{
+  static std::mutex mu;
+  static std::map<int, double*> per_device;
+  int dev = 0;
+  RK_HIP(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> lk(mu);
+  auto it = per_device.find(dev);
+  if(it == per_device.end()) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if(e != hipSuccess) return hip_fail(e, "hipGetSymbolAddress(energy scratch)");
+    if(!p) return fail(RK_EHIP, "hipGetSymbolAddress(energy scratch) returned null");
+    it = per_device.emplace(dev, static_cast<double*>(p)).first;
+  }
+  *out = it->second;
+  return RK_OK;
+}
+
+// The host's sums of rk_energies in the same order (planes in order from 0,
+// then / 2; kinetic partials in order from 0), so the values are bit-identical.
+__global__ void k_energy_finish(const double* __restrict__ part, int planes, int kin_blocks, int nsp, double* out)
+{
+  if(blockIdx.x != 0 || threadIdx.x != 0) return;
+  double e = 0.0, b = 0.0;
+  for(int p = 0; p < planes; ++p) {
+    e += part[2 * p];
+    b += part[2 * p + 1];
+  }
+  out[0] = e / 2.0;
+  out[1] = b / 2.0;
+  for(int s = 0; s < nsp; ++s) {
+    double acc = 0.0;
+    const double* k = part + 2 * ENERGY_PLANES + s * ENERGY_KIN;
+    for(int q = 0; q < kin_blocks; ++q) acc += k[q];
+    out[2 + s] = acc;
+  }
+}
+
+}  // namespace
+
 extern "C" {
 
 const char* rk_last_error(void) { return rk::g_err.c_str(); }
@@ -236,6 +291,10 @@ int rk_world_create(rk_world** out, const rk_world_desc* d)
     return hip_fail(e, "rk_world_create");
   }
   w->filter_done.assign(size_t(g.ntiles()), 0);
+  if(int rc = resolve_energy_scratch(&w->energy_scratch)) {
+    rk_world_destroy(w);
+    return rc;
+  }
   if(int rc = fast_division_ok(g.cfl, g.rcp_cfl, g.fast_div, g.fast_sqrt)) {
     rk_world_destroy(w);
     return rc;
@@ -771,6 +830,32 @@ int rk_energies(rk_world* w, int32_t tile, double* out, void* stream)
   out[1] = fe[1];
   if(w->d.n_species > 0 && w->sp[0].bin_start) return kinetic_energies(w, tile, out + 2, S(stream));
   for(int s = 0; s < w->d.n_species; ++s) out[2 + s] = 0.0;
+  return RK_OK;
+}
+
+int rk_energies_device(rk_world* w, int32_t tile, double* out_dev, void* stream)
+{
+  if(int rc = check_tile(w, tile)) return rc;
+  if(!out_dev) return fail(RK_EINVAL, "rk_energies_device: null output");
+  if(!w->energy_scratch) return fail(RK_ESTATE, "rk_energies_device: energy scratch not resolved");
+  int r0[3], r1[3];
+  interior(w, tile, r0, r1);
+  const bool empty = r1[0] <= r0[0] || r1[1] <= r0[1] || r1[2] <= r0[2];
+  const int planes = empty ? 0 : r1[0] - r0[0];  // no partials are written for an empty region
+  if(planes > ENERGY_PLANES)
+    return fail(RK_EINVAL, "rk_energies_device: the region has " + std::to_string(planes) +
+                             " x planes; the static partial-sum array holds at most " +
+                             std::to_string(ENERGY_PLANES));
+  double* part = w->energy_scratch;
+  const hipStream_t st = S(stream);
+  if(int rc = launch_field_energy_partials(w, r0, r1, part, st)) return rc;
+  const bool kin = w->d.n_species > 0 && w->sp[0].bin_start;
+  const int kin_blocks = kin ? kinetic_partial_count(w, tile) : 0;
+  if(kin_blocks > ENERGY_KIN) return fail(RK_EINVAL, "rk_energies_device: too many kinetic partials");
+  for(int s = 0; kin && s < w->d.n_species; ++s)
+    if(int rc = launch_kinetic_partials(w, tile, s, part + 2 * ENERGY_PLANES + s * ENERGY_KIN, st)) return rc;
+  hipLaunchKernelGGL(k_energy_finish, dim3(1), dim3(64), 0, st, part, planes, kin_blocks, w->d.n_species, out_dev);
+  RK_HIP(hipGetLastError());
   return RK_OK;
 }
 

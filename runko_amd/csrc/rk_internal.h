@@ -175,6 +175,7 @@ struct rk_world {
   int64_t ovf_cap = 0;
   double* red = nullptr;          // reduction partials
   int64_t red_cap = 0;
+  double* energy_scratch = nullptr;  // this device's static partial-sum array (rk_energies_device)
   uint32_t* host_counters = nullptr;  // pinned mirror
   // kernel timing (rk_profile_*)
   bool prof_on = false;
