@@ -52,6 +52,17 @@ def profiled_traffic(args):
     return None
 
 
+def release_device_memory():
+    """Return the cached blocks of a finished workload to the device, so the
+    next workload (and the library's own allocations) find them free."""
+    import gc
+
+    import torch
+    gc.collect()
+    torch.cuda.synchronize()
+    torch.cuda.empty_cache()
+
+
 def run_turbulence(args):
     import torch
 
@@ -97,6 +108,37 @@ def run_turbulence(args):
         best, times = tune_fused_launch(w, lap)
         launch = dict(chosen=best, k_fused_ms=times)
         log(f"[bench] fused launch: {best} ({', '.join(f'{k} {v:.2f} ms' for k, v in times.items())})")
+    graph = None
+    if not args.no_graph:
+        # the same lap replayed from captured HIP graphs (one launch per lap,
+        # two graphs alternating since three filter passes swap J and its
+        # target), with the synchronous overflow check after every lap: the
+        # headline form
+        from runko_amd.stepper import GraphLap
+        try:
+            gl = GraphLap(w)
+        except RuntimeError as e:  # capture refused: report the eager form only
+            log(f"[bench] graph capture failed ({e}); headline from the eager laps")
+            gl = None
+        if gl is not None:
+            for _ in range(2):
+                gl()
+                gl.settle()
+            torch.cuda.synchronize()
+            mem_g0 = w.memory_report()
+            tg = time.perf_counter()
+            for i in range(args.steps):
+                gl()
+                gl.settle()
+            torch.cuda.synchronize()
+            gwall = time.perf_counter() - tg
+            mem_g1 = w.memory_report()
+            graph = dict(steps=args.steps, wall=gwall, ms_per_step=1e3 * gwall / args.steps,
+                         particle_steps_per_s=npart * args.steps / gwall, recaptures=gl.recaptures,
+                         bin_moves=mem_g1["bin_moves"] - mem_g0["bin_moves"],
+                         relayouts=mem_g1["relayouts"] - mem_g0["relayouts"])
+            log(f"[bench] graph laps: {graph['ms_per_step']:.2f} ms/step ({graph})")
+            del gl
     torch.cuda.synchronize()
     mem_t0 = w.memory_report()
     _lib.check(_lib.lib().rk_profile_enable(w.h, 1))
@@ -124,6 +166,30 @@ def run_turbulence(args):
     prof = {name: {"ms_total": float(ms[i]), "launches": int(cnt[i])} for i, name in enumerate(_lib.PROF_CATEGORIES)}
     assert w.total_particles() == npart, "particle count changed"
     err = w.error_counters()
+    sort_effect = None
+    if not args.no_sort_check:
+        # the reference driver cell-sorts every bin every 5 laps
+        # (projects/pic-turbulence/pic.py:198-199): time that sort once, and
+        # the fused kernel on the lap right after it against the timed laps'
+        # average (bins are kept compact every step, so only the in-bin order
+        # changes)
+        ea, eb = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        ea.record()
+        w.call("rk_sort_particles", -1)
+        eb.record()
+        torch.cuda.synchronize()
+        _lib.check(_lib.lib().rk_profile_enable(w.h, 1))
+        lap()
+        lap.settle()
+        ms2, cnt2 = np.zeros(9), np.zeros(9, dtype=np.int64)
+        _lib.check(_lib.lib().rk_profile_read(w.h, ms2.ctypes.data, cnt2.ctypes.data))
+        _lib.check(_lib.lib().rk_profile_enable(w.h, 0))
+        sort_effect = dict(sort_ms=ea.elapsed_time(eb), k_fused_ms_after_sort=float(ms2[0] / max(cnt2[0], 1)),
+                           k_fused_ms_timed_laps=float(ms[0] / max(cnt[0], 1)),
+                           sort_ms_per_lap_every_5=ea.elapsed_time(eb) / 5)
+        log(f"[bench] in-bin cell sort: {sort_effect}")
+        assert w.total_particles() == npart, "particle count changed (sort)"
     plain = None
     if args.plain_steps > 0:
         # the same world advanced in the plain per-tile form (one call per tile
@@ -153,7 +219,8 @@ def run_turbulence(args):
         ms_per_step=1e3 * wall / args.steps, prof=prof, fused_ms=float(fused_ms), alg_bytes=alg_bytes,
         achieved_gbs=achieved, field_ms_per_step=field_ms / args.steps,
         cell_updates_per_s=ncell * args.steps / (field_ms * 1e-3) if field_ms > 0 else None,
-        error_counters=[int(v) for v in err], brick=list(w.brick), B0=info["B0"], plain=plain, pools=mem,
+        error_counters=[int(v) for v in err], brick=list(w.brick), B0=info["B0"], plain=plain, pools=mem, graph=graph,
+        sort_effect=sort_effect,
         autotune=dict(brick=tuned, fused_launch=launch))
 
 
@@ -170,6 +237,7 @@ def run_config0_gpu(inputs, laps, execution):
     from runko_amd.workloads import build_simulation as build_gpu
     from runko_amd.workloads import turbulence_lap as canonical_lap
 
+    release_device_memory()
     nt, n, Q, M, tiles = inputs
     graph = execution == "graph"
     conf, grid, sim = build_gpu(nt, n, Q, M, tiles, execution="batched" if graph else execution)
@@ -196,6 +264,26 @@ def run_config0_gpu(inputs, laps, execution):
     return dict(particle_steps_per_s=npart * laps / dt, ms_per_step=1e3 * dt / laps, laps=laps, npart=npart)
 
 
+def run_pair_plasma(inputs=None):
+    """The 64^3 pair plasma through the Simulation API: batched, replayed from
+    HIP graphs, and plain per tile."""
+    if inputs is None:
+        inputs = config0_inputs()
+    b = run_config0_gpu(inputs, laps=20, execution="batched")
+    gph = run_config0_gpu(inputs, laps=100, execution="graph")
+    p = run_config0_gpu(inputs, laps=5, execution="plain")
+    step_bytes = 48.0 * b["npart"] + 228.0 * 64 ** 3
+    log(f"[bench] 64^3: graph {gph['particle_steps_per_s']:.3e}, batched {b['particle_steps_per_s']:.3e}, "
+        f"plain {p['particle_steps_per_s']:.3e} p-s/s")
+    return dict(
+        workload="3D periodic pair plasma, 64^3 cells in 4^3 tiles of 16^3, 8 ppc/species, theta 0.3, "
+                 "sigma 1 (Simulation API, canonical lap)",
+        batched=b, graph=gph, plain_per_tile=p,
+        graph_vs_plain=gph["particle_steps_per_s"] / p["particle_steps_per_s"],
+        batched_vs_plain=b["particle_steps_per_s"] / p["particle_steps_per_s"],
+        graph_step_roofline_frac=step_bytes / (gph["ms_per_step"] * 1e-3) / 1e9 / HBM_PEAK_GBS)
+
+
 def cpu_model():
     try:
         for line in open("/proc/cpuinfo"):
@@ -206,15 +294,19 @@ def cpu_model():
     return "unknown"
 
 
-def run_cpu_baseline(inputs, seconds):
+def run_cpu_baseline(seconds):
     """The oracle (FP32 C++/OpenMP restatement of the reference's step, same
-    lap order) timed on this host's cores: built -O3 -march=native here when
-    g++ can, on every thread OpenMP offers and on one thread."""
+    lap order) timed on this host's cores on a sample of the headline
+    workload (a 64^3 periodic sub-box of the turbulence run: 2^3 tiles of
+    32^3 cells, 16 ppc/species, theta 0.3, sigma 10, 8x8 dB modes): built
+    -O3 -march=native here when g++ can, on every thread OpenMP offers and
+    on one thread."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
+    from runko_amd.workloads import turbulence_sample_inputs
 
     native = oracle.use_native_build()
-    nt, n, Q, M, tiles = inputs
+    nt, n, Q, M, tiles = turbulence_sample_inputs()
     w = oracle.OracleWorld(nt, n, 0.45, Q, M, prec=32)
     for idx, (F, parts) in tiles.items():
         t = w.tile_index(*idx)
@@ -243,9 +335,10 @@ def run_cpu_baseline(inputs, seconds):
     build = "-O3 -march=native" if native else "-O3 -march=x86-64-v3"
     return dict(value=npart * laps / dt, unit="particle-steps/s", cores=threads, kind="port",
                 one_thread=npart * laps1 / dt1, cpu_model=cpu_model(), build=build,
-                sample=f"oracle FP32 C++/OpenMP restatement ({build}, -ffp-contract=off), 64^3 cells in 4^3 tiles, "
-                       f"{npart} particles: {laps} full laps on {threads} threads ({dt:.1f} s), then {laps1} laps on "
-                       f"1 thread ({dt1:.1f} s); same step order as the GPU run")
+                sample=f"oracle FP32 C++/OpenMP restatement ({build}, -ffp-contract=off) on a 64^3 periodic sub-box of "
+                       f"the turbulence workload (2^3 tiles of 32^3 cells, 16 ppc/species, theta 0.3, sigma 10, 8x8 "
+                       f"dB modes), {npart} particles: {laps} full laps on {threads} threads ({dt:.1f} s), then "
+                       f"{laps1} laps on 1 thread ({dt1:.1f} s); same step order as the GPU run")
 
 
 # the stencil coefficients of the reference's 3D shock configuration
@@ -600,6 +693,12 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-graph", action="store_true",
+                    help="time only the eager batched laps (no HIP-graph replay of the turbulence lap)")
+    ap.add_argument("--only-pair", action="store_true",
+                    help="run only the 64^3 pair-plasma legs (batched, graph, plain per tile)")
+    ap.add_argument("--no-sort-check", action="store_true",
+                    help="skip timing the in-bin cell sort and the fused kernel after it")
     ap.add_argument("--plain-steps", type=int, default=1,
                     help="steps of the plain per-tile form on the same world (0: skip)")
     ap.add_argument("--slot-order", default="interleaved", choices=["interleaved", "random"],
@@ -646,6 +745,9 @@ def main():
     if args.only_shock:
         print(json.dumps(run_shock(args)), flush=True)
         return
+    if args.only_pair:
+        print(json.dumps(run_pair_plasma()), flush=True)
+        return
     if args.only_ensemble:
         print(json.dumps(run_ensemble(args)), flush=True)
         return
@@ -654,34 +756,28 @@ def main():
     res = run_turbulence(args)
     log(f"[bench] batched turbulence: {res['ms_per_step']:.2f} ms/step, fused kernel {res['fused_ms']:.2f} ms, "
         f"{res['achieved_gbs']:.0f} GB/s algorithmic")
-    value = res["npart"] * res["steps"] / res["wall"]
+    release_device_memory()
+    # headline: the graph-replayed batched lap when it ran, else the eager one
+    head = res["graph"] if res["graph"] is not None else dict(wall=res["wall"], ms_per_step=res["ms_per_step"])
+    value = res["npart"] * res["steps"] / head["wall"]
 
     extra = {}
     inputs = None
-    if not args.no_plain or not args.no_cpu_baseline:
+    if not args.no_plain:
         inputs = config0_inputs()
     if not args.no_plain:
-        b = run_config0_gpu(inputs, laps=20, execution="batched")
-        gph = run_config0_gpu(inputs, laps=100, execution="graph")
-        p = run_config0_gpu(inputs, laps=5, execution="plain")
-        step_bytes = 48.0 * b["npart"] + 228.0 * 64 ** 3
-        extra["pair_plasma_64cubed"] = dict(
-            workload="3D periodic pair plasma, 64^3 cells in 4^3 tiles of 16^3, 8 ppc/species, theta 0.3, "
-                     "sigma 1 (Simulation API, canonical lap)",
-            batched=b, graph=gph, plain_per_tile=p,
-            graph_vs_plain=gph["particle_steps_per_s"] / p["particle_steps_per_s"],
-            batched_vs_plain=b["particle_steps_per_s"] / p["particle_steps_per_s"],
-            graph_step_roofline_frac=step_bytes / (gph["ms_per_step"] * 1e-3) / 1e9 / HBM_PEAK_GBS)
-        log(f"[bench] 64^3: graph {gph['particle_steps_per_s']:.3e}, batched {b['particle_steps_per_s']:.3e}, "
-            f"plain {p['particle_steps_per_s']:.3e} p-s/s")
+        extra["pair_plasma_64cubed"] = run_pair_plasma(inputs)
     if not args.no_shock:
+        release_device_memory()
         extra["shock"] = run_shock(args)
     if not args.no_adjoint:
+        release_device_memory()
         extra["differentiation"] = run_differentiation(args)
         d = extra["differentiation"]
         log(f"[bench] adjoint/forward {d['adjoint_over_forward']:.2f} (rollout with recompute), "
             f"step kernels {d['forward_ms_per_step']:.2f} / {d['adjoint_ms_per_step']:.2f} ms")
     if not args.no_ensemble:
+        release_device_memory()
         extra["ensemble"] = run_ensemble(args)
         d = extra["ensemble"]
         log(f"[bench] ensemble adjoint/forward {d['adjoint_over_forward']:.2f}, step kernels "
@@ -689,7 +785,7 @@ def main():
             f"{d['ensemble_vs_per_box']:.2f}x the per-box launches")
     cpu = None
     if not args.no_cpu_baseline:
-        cpu = run_cpu_baseline(inputs, args.cpu_seconds)
+        cpu = run_cpu_baseline(args.cpu_seconds)
         log(f"[bench] cpu baseline {cpu['value']:.3e} particle-steps/s on {cpu['cores']} threads")
 
     line = {
@@ -699,7 +795,7 @@ def main():
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": res["ms_per_step"],
+        "ms_per_step": head["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -707,7 +803,8 @@ def main():
         "data": "synthetic (seeded on-GPU initial conditions)",
         "config": {
             "workload": f"3D decaying turbulence, {args.cells}^3 cells in {args.tile}^3-cell tiles, "
-                        f"{args.ppc} particles/cell/species, theta 0.3, sigma 10, 8x8 dB modes, batched form",
+                        f"{args.ppc} particles/cell/species, theta 0.3, sigma 10, 8x8 dB modes, batched form"
+                        + (" replayed from HIP graphs" if res["graph"] is not None else ""),
             "particles": res["npart"],
             "cells": res["ncell"],
             "brick": res["brick"],
@@ -726,22 +823,25 @@ def main():
         },
         "step_roofline": {
             "algorithmic_bytes_per_step": 48.0 * res["npart"] + 228.0 * res["ncell"],
-            "achieved": (48.0 * res["npart"] + 228.0 * res["ncell"]) / (res["ms_per_step"] * 1e-3) / 1e9,
+            "achieved": (48.0 * res["npart"] + 228.0 * res["ncell"]) / (head["ms_per_step"] * 1e-3) / 1e9,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
-            "frac": (48.0 * res["npart"] + 228.0 * res["ncell"]) / (res["ms_per_step"] * 1e-3) / 1e9 / HBM_PEAK_GBS,
+            "frac": (48.0 * res["npart"] + 228.0 * res["ncell"]) / (head["ms_per_step"] * 1e-3) / 1e9 / HBM_PEAK_GBS,
         },
+        "eager_batched": dict(ms_per_step=res["ms_per_step"], particle_steps_per_s=res["npart"] * res["steps"] / res["wall"]),
         "autotune": res["autotune"],
         "cpu_baseline": cpu,
         "fdtd_cell_updates_per_s": res["cell_updates_per_s"],
         "kernel_ms": res["prof"],
         "error_counters": res["error_counters"],
         "pools": res["pools"],
+        "sort_effect": res["sort_effect"],
     }
     if res["plain"] is not None:
-        line["batched_vs_plain"] = dict(workload=line["config"]["workload"].replace(", batched form", ""),
-                                        batched_ms_per_step=res["ms_per_step"], plain=res["plain"],
-                                        ratio=res["plain"]["ms_per_step"] / res["ms_per_step"])
+        line["batched_vs_plain"] = dict(workload=line["config"]["workload"].split(", batched form")[0],
+                                        batched_ms_per_step=res["ms_per_step"], graph_ms_per_step=head["ms_per_step"],
+                                        plain=res["plain"], ratio=res["plain"]["ms_per_step"] / res["ms_per_step"],
+                                        graph_ratio=res["plain"]["ms_per_step"] / head["ms_per_step"])
     line.update(extra)
     print(json.dumps(line), flush=True)
 

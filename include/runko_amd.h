@@ -173,6 +173,17 @@ int rk_push_e(rk_world* w, int32_t tile, void* stream);        /* E += cfl curl 
 int rk_add_current(rk_world* w, int32_t tile, void* stream);   /* E -= J */
 int rk_filter_current(rk_world* w, int32_t tile, void* stream);/* binomial (1,2,1)^3 pass */
 
+/* Batched form of the driver's filter sequence (projects/pic-turbulence/pic.py:208-212:
+ * filter_current, comm_local(emf_J), filter_current, filter_current on every
+ * tile): `passes` filter passes over every tile, starting from a current J
+ * ghost layer (comm_local(emf_J) after the deposit), with the J ghost layer
+ * synced between passes.  On a periodic world with three binomial2 passes
+ * the three run as one launch that reads J and writes the result once; the
+ * interior equals the passes run one by one bit for bit.  On return the J
+ * ghost layer of a periodic world holds the periodic images of the filtered
+ * interior. */
+int rk_filter_current_passes(rk_world* w, int32_t passes, void* stream);
+
 /* ---- pic::Tile methods (src/runko/pic/tile.c++:324-438, tile_communication.c++:66-96) ---- */
 int rk_push_particles(rk_world* w, int32_t tile, void* stream);  /* interpolation + Boris */
 int rk_deposit_current(rk_world* w, int32_t tile, void* stream); /* zigzag, first order */

@@ -108,6 +108,7 @@ def lib():
         f = getattr(L, name)
         f.argtypes = [vp, i32, vp]
     L.rk_particle_step_fused.argtypes = [vp, vp]
+    L.rk_filter_current_passes.argtypes = [vp, i32, vp]
     L.rk_particle_step_fused_walls.argtypes = [vp, vp, i32, vp]
     L.rk_world_set_fused_launch.argtypes = [vp, i32]
     L.rk_world_set_fused_workgroup.argtypes = [vp, i32]
@@ -163,7 +164,7 @@ EXPORTED = [
     "rk_world_num_bricks", "rk_world_set_fields", "rk_world_j_pointer", "rk_world_set_species",
     "rk_world_set_migration", "rk_push_half_b", "rk_push_e", "rk_add_current", "rk_filter_current",
     "rk_push_particles", "rk_deposit_current", "rk_pack_outgoing_particles", "rk_sort_particles",
-    "rk_particle_step_fused", "rk_particle_step_fused_walls", "rk_world_set_fused_launch", "rk_world_set_fused_workgroup", "rk_world_set_walls",
+    "rk_filter_current_passes", "rk_particle_step_fused", "rk_particle_step_fused_walls", "rk_world_set_fused_launch", "rk_world_set_fused_workgroup", "rk_world_set_walls",
     "rk_particle_step_fused_device_walls", "rk_advance_walls", "rk_read_walls", "rk_comm_local", "rk_inject", "rk_pending_overflow", "rk_bin_demand",
     "rk_repack", "rk_retry_append", "rk_move_bins", "rk_relayout", "rk_tile_particle_count", "rk_tile_particles", "rk_outgoing",
     "rk_energies", "rk_energies_device", "rk_gauss_residual", "rk_error_counters", "rk_push_e_add_current", "rk_profile_enable",

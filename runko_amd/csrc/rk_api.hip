@@ -270,15 +270,15 @@ int rk_world_create(rk_world** out, const rk_world_desc* d)
     w->sp[s].charge = static_cast<float>(d->charge[s]);
     maxq = std::max(maxq, std::fabs(double(w->sp[s].charge)));
   }
-  // every contribution is |q| * (a move < 1 cell) * weights <= 1: scale so that
-  // max|q| * jscale <= 2^40.  The int64 sums are exact up to 2^63 (2^23
-  // full-size contributions per node); converting a sum back to FP32 rounds
-  // once, like any FP32 result
+  // every contribution is |q| * (a move < 1 cell) * weights <= 1: scale so
+  // that max|q| * jscale <= 2^30 (a contribution rounds to an int32); the
+  // int64 window sums are exact up to 2^63, and converting a sum back to FP32
+  // rounds once, like any FP32 result
   {
     int e = 0;
     if(maxq > 0.0 && std::isfinite(maxq)) (void)std::frexp(maxq, &e);  // maxq < 2^e
-    g.jscale = std::ldexp(1.0, 40 - e);
-    g.jinv = std::ldexp(1.0, e - 40);
+    g.jscale = std::ldexp(1.0, 30 - e);
+    g.jinv = std::ldexp(1.0, e - 30);
   }
   hipError_t e = hipMalloc(reinterpret_cast<void**>(&w->counters), sizeof(uint32_t) * C_NUM);
   if(e == hipSuccess) e = hipMemset(w->counters, 0, sizeof(uint32_t) * C_NUM);
@@ -505,6 +505,33 @@ int rk_filter_current(rk_world* w, int32_t tile, void* stream)
   if(++w->filter_done_count == w->g.ntiles()) {
     swap();
     w->filter_open = false;
+  }
+  return RK_OK;
+}
+
+int rk_filter_current_passes(rk_world* w, int32_t passes, void* stream)
+{
+  if(int rc = check_tile(w, -1)) return rc;
+  if(passes < 0) return fail(RK_EINVAL, "filter_current_passes: negative pass count");
+  if(w->filter_open)
+    return fail(RK_ESTATE, "filter_current: a per-tile filter pass is open; finish it on every tile first");
+  if(passes == 3 && w->g.periodic && w->d.current_filter == 0) {
+    close_j(w);
+    {
+      ProfScope ps(w, 6, S(stream));
+      if(int rc = launch_filter_x3(w, S(stream))) return rc;
+    }
+    for(int d = 0; d < 3; ++d) std::swap(w->J[d], w->Jf[d]);
+    // the new J's ghost layer: periodic images of the filtered interior
+    ProfScope ps(w, 7, S(stream));
+    return launch_ghost_copy(w, w->J, S(stream));
+  }
+  for(int p = 0; p < passes; ++p) {
+    if(p > 0 && w->g.periodic) {
+      ProfScope ps(w, 7, S(stream));
+      if(int rc = launch_ghost_copy(w, w->J, S(stream))) return rc;
+    }
+    if(int rc = rk_filter_current(w, -1, stream)) return rc;
   }
   return RK_OK;
 }

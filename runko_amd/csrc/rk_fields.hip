@@ -376,6 +376,159 @@ __global__ void __launch_bounds__(BX* BY)
   sep_columns(g, S, D, i0, i1, j, k);
 }
 
+// Three binomial2 passes in one launch, for a periodic world whose J ghost
+// layer is current (3 deep, as after a ghost copy): every interior node of the
+// third pass depends on inputs at most 3 nodes away, so the result equals the
+// passes run one by one (with or without a ghost copy between them) bit for
+// bit.  A workgroup owns an F3Y x F3Z (y, z) column of output nodes and marches
+// along x over F3XS planes.  Pass level k (1..3) covers its output's (y, z)
+// extent widened by 3-k nodes per side.  When a plane of level k-1 arrives,
+// every level-k node reads its 3x3 (y, z) neighbourhood from LDS once and
+// feeds the three 27-tap sums that take this plane: it completes the node
+// one plane behind (third x offset), continues the node of this plane
+// (second) and starts the node one plane ahead (first).  Each sum thus takes
+// its taps in the order of filter_columns (x offset, then y, then z, from 0),
+// and a level's completed planes reach the next level through LDS (planes
+// double-buffered per level, one barrier per step).  The x segment is at
+// most F3XS planes and shorter on small worlds, so that the launch still has
+// a few workgroups per CU.
+constexpr int F3Y = 8, F3Z = 32, F3T = 256, F3XS = 128;
+constexpr int F3IY = F3Y + 6, F3IZ = F3Z + 6, F3IN = F3IY * F3IZ, F3IM = (F3IN + F3T - 1) / F3T;
+
+template<int K>
+struct F3Level {
+  static constexpr int WY = F3Y + 2 * (3 - K), WZ = F3Z + 2 * (3 - K), NP = WY * WZ;
+  static constexpr int NM = (NP + F3T - 1) / F3T;  // nodes per thread
+  static constexpr int SZ = WZ + 2;                // row length of the level below
+};
+
+struct F3Chains {
+  float s1, s2;  // sums of the nodes one plane ahead (one x offset taken) and of this plane (two taken)
+};
+
+// one level-K position (py, pz): take plane `src` (row length SZ) of the level below
+template<int K>
+__device__ inline float f3_take(const float* src, int py, int pz, F3Chains& ch)
+{
+  constexpr int SZ = F3Level<K>::SZ;
+  const float w[3] = {1.0f, 2.0f, 1.0f};
+  float v[3][3];
+#pragma unroll
+  for(int b = 0; b < 3; ++b)
+#pragma unroll
+    for(int c = 0; c < 3; ++c) v[b][c] = src[(py + b) * SZ + pz + c];
+  float out = ch.s2, mid = ch.s1, head = 0.0f;
+#pragma unroll
+  for(int b = 0; b < 3; ++b)
+#pragma unroll
+    for(int c = 0; c < 3; ++c) {
+      out = __builtin_fmaf((w[2] * w[b] * w[c]) * (1.0f / 64.0f), v[b][c], out);
+      mid = __builtin_fmaf((w[1] * w[b] * w[c]) * (1.0f / 64.0f), v[b][c], mid);
+      head = __builtin_fmaf((w[0] * w[b] * w[c]) * (1.0f / 64.0f), v[b][c], head);
+    }
+  ch.s2 = mid;
+  ch.s1 = head;
+  return out;
+}
+
+__global__ void __launch_bounds__(F3T)
+  k_filter27_x3(Geo g, int xseg, const float* __restrict__ S0, const float* __restrict__ S1,
+                const float* __restrict__ S2, float* __restrict__ D0, float* __restrict__ D1, float* __restrict__ D2)
+{
+  __shared__ float In[2][3][F3IN];
+  __shared__ float L1[2][3][F3Level<1>::NP];
+  __shared__ float L2[2][3][F3Level<2>::NP];
+  const int tid = int(threadIdx.x);
+  const int z0 = HALO + int(blockIdx.x) * F3Z, y0 = HALO + int(blockIdx.y) * F3Y;
+  const int x0 = HALO + int(blockIdx.z) * xseg;
+  const int xs = min(xseg, HALO + g.N[0] - x0);
+  const float* const S[3] = {S0, S1, S2};
+  float* const D[3] = {D0, D1, D2};
+  // input plane q (0 .. xs+5) is haloed plane x0 - 3 + q; rows and columns
+  // past the lattice (partial workgroups) read its last node and feed only
+  // outputs that are not stored
+  uint32_t ioff[F3IM];
+  bool iok[F3IM];
+#pragma unroll
+  for(int m = 0; m < F3IM; ++m) {
+    const int e = tid + m * F3T;
+    iok[m] = e < F3IN;
+    const int r = e / F3IZ, c = e - r * F3IZ;
+    const int yy = min(y0 - HALO + r, g.L[1] - 1), zz = min(z0 - HALO + c, g.L[2] - 1);
+    ioff[m] = uint32_t(yy) * uint32_t(g.L[2]) + uint32_t(zz);
+  }
+  const uint32_t plane = uint32_t(g.L[1]) * uint32_t(g.L[2]);
+  float pre[3][F3IM];
+  auto load = [&](int q) {
+    const uint32_t base = uint32_t(x0 - HALO + q) * plane;
+#pragma unroll
+    for(int comp = 0; comp < 3; ++comp)
+#pragma unroll
+      for(int m = 0; m < F3IM; ++m)
+        if(iok[m]) pre[comp][m] = S[comp][base + ioff[m]];
+  };
+  F3Chains c1[3][F3Level<1>::NM] = {}, c2[3][F3Level<2>::NM] = {}, c3[3][F3Level<3>::NM] = {};
+  const int nin = xs + 6;  // input planes
+  load(0);
+  for(int s = 0; s < xs + 9; ++s) {
+    const int cur = s & 1, prev = cur ^ 1;
+    if(s < nin) {
+#pragma unroll
+      for(int comp = 0; comp < 3; ++comp)
+#pragma unroll
+        for(int m = 0; m < F3IM; ++m)
+          if(iok[m]) In[cur][comp][tid + m * F3T] = pre[comp][m];
+      if(s + 1 < nin) load(s + 1);
+    }
+    // level 1 takes input plane s-1 and completes its node s-2 (input-relative index)
+    if(s >= 1 && s <= nin) {
+#pragma unroll
+      for(int m = 0; m < F3Level<1>::NM; ++m) {
+        const int p = tid + m * F3T;
+        if(p < F3Level<1>::NP) {
+          const int py = p / F3Level<1>::WZ, pz = p - py * F3Level<1>::WZ;
+#pragma unroll
+          for(int comp = 0; comp < 3; ++comp)
+            L1[cur][comp][p] = f3_take<1>(In[prev][comp], py, pz, c1[comp][m]);
+        }
+      }
+    }
+    // level 2 takes level-1 node s-3 and completes its node s-4
+    if(s >= 3 && s <= nin + 2) {
+#pragma unroll
+      for(int m = 0; m < F3Level<2>::NM; ++m) {
+        const int p = tid + m * F3T;
+        if(p < F3Level<2>::NP) {
+          const int py = p / F3Level<2>::WZ, pz = p - py * F3Level<2>::WZ;
+#pragma unroll
+          for(int comp = 0; comp < 3; ++comp)
+            L2[cur][comp][p] = f3_take<2>(L1[prev][comp], py, pz, c2[comp][m]);
+        }
+      }
+    }
+    // level 3 takes level-2 node s-5 and completes output node s-6 (haloed x0 - 9 + s)
+    if(s >= 5) {
+#pragma unroll
+      for(int m = 0; m < F3Level<3>::NM; ++m) {
+        const int p = tid + m * F3T;
+        if(p < F3Level<3>::NP) {
+          const int py = p / F3Level<3>::WZ, pz = p - py * F3Level<3>::WZ;
+          float o[3];
+#pragma unroll
+          for(int comp = 0; comp < 3; ++comp) o[comp] = f3_take<3>(L2[prev][comp], py, pz, c3[comp][m]);
+          const int y = y0 + py, z = z0 + pz;
+          if(s >= 9 && y < HALO + g.N[1] && z < HALO + g.N[2]) {
+            const uint32_t q = g.at(x0 - 9 + s, y, z);
+#pragma unroll
+            for(int comp = 0; comp < 3; ++comp) D[comp][q] = o[comp];
+          }
+        }
+      }
+    }
+    __syncthreads();
+  }
+}
+
 __device__ inline int wrap_idx(int c, int N) { return c < HALO ? c + N : (c >= N + HALO ? c - N : c); }
 
 // periodic ghost copy: every ghost node takes the value of its interior image
@@ -588,6 +741,20 @@ int launch_filter(rk_world* w, const int r0[3], const int r1[3], hipStream_t s)
     hipLaunchKernelGGL(k_filter_sep, grid, dim3(BX, BY), 0, s, w->g, r, w->J[0], w->J[1], w->J[2], w->Jf[0],
                        w->Jf[1], w->Jf[2]);
   }
+  RK_HIP(hipGetLastError());
+  return RK_OK;
+}
+
+int launch_filter_x3(rk_world* w, hipStream_t s)
+{
+  const Geo& g = w->g;
+  const int64_t cols = int64_t((g.N[2] + F3Z - 1) / F3Z) * ((g.N[1] + F3Y - 1) / F3Y);
+  int xseg = F3XS;
+  while(xseg > 4 && cols * ((g.N[0] + xseg - 1) / xseg) < 512) xseg /= 2;  // 2 workgroups per CU
+  const dim3 grid(unsigned((g.N[2] + F3Z - 1) / F3Z), unsigned((g.N[1] + F3Y - 1) / F3Y),
+                  unsigned((g.N[0] + xseg - 1) / xseg));
+  hipLaunchKernelGGL(k_filter27_x3, grid, dim3(F3T), 0, s, g, xseg, w->J[0], w->J[1], w->J[2], w->Jf[0], w->Jf[1],
+                     w->Jf[2]);
   RK_HIP(hipGetLastError());
   return RK_OK;
 }

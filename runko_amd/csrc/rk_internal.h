@@ -62,7 +62,7 @@ struct Geo {
                       //    corrections on their ranges (both flags set only when the forms were
                       //    verified bit-identical to the IEEE operations for every float of the
                       //    range at world creation)
-  double jscale;      // fixed-point scale of the fused current windows: 2^40 / (max|q| rounded up to 2^k)
+  double jscale;      // fixed-point scale of the fused current windows: 2^30 / (max|q| rounded up to 2^k)
   double jinv;        // 1 / jscale (exact)
   uint32_t br_magic[3];  // ceil(2^32 / br) for br > 1: c / br == umulhi(c, magic) for 0 <= c < 2^16
   int pusher;            // 0 boris, 1 higuera_cary, 2 faraday (pic/tile.c++:74-86)
@@ -262,6 +262,8 @@ int launch_retry(rk_world* w, hipStream_t s);
 int launch_repack(rk_world* w, int s, const rk::Species& nsp, hipStream_t st);
 int launch_move_bins(rk_world* w, int s, int n, const int32_t* bins, const int64_t* new_start,
                      const uint32_t* new_cap, hipStream_t st);
+// three binomial2 passes J -> Jf over the interior of a periodic world (J ghosts current)
+int launch_filter_x3(rk_world* w, hipStream_t s);
 int relayout_species(rk_world* w, int s, const uint32_t* new_cap, void* tmp, int64_t tmp_bytes,
                      int64_t* new_start, hipStream_t st);
 int launch_gather_tile(rk_world* w, int tile, int s, float* const pos[3], float* const vel[3], uint64_t* id,

@@ -1223,13 +1223,20 @@ __device__ inline bool gather6(const Shape<BR>& S, const float* lds, int li, int
 {
   if(!(li >= 0 && lj >= 0 && lk >= 0 && li + 1 < S.M(0) && lj + 1 < S.M(1) && lk + 1 < S.M(2))) return false;
   const int V = S.MV();
-  const float* p = lds + (li * S.M(1) + lj) * S.M(2) + lk;
+  const int p = (li * S.M(1) + lj) * S.M(2) + lk;
   float v[6][8];
 #pragma unroll
-  for(int c = 0; c < 8; ++c) {
-    const int o = ((c >> 2) * S.M(1) + ((c >> 1) & 1)) * S.M(2) + (c & 1);
+  for(int comp = 0; comp < 6; ++comp) {
+    // the component's base index stays one register: its 8 corners (at most
+    // M1*M2 + M2 + 1 floats apart) are then immediate offsets of LDS reads
+    const __attribute__((address_space(3))) float* pc =
+      (const __attribute__((address_space(3))) float*)(lds + p + comp * V);
+    asm("" : "+v"(pc));
 #pragma unroll
-    for(int comp = 0; comp < 6; ++comp) v[comp][c] = p[comp * V + o];
+    for(int c = 0; c < 8; ++c) {
+      const int o = ((c >> 2) * S.M(1) + ((c >> 1) & 1)) * S.M(2) + (c & 1);
+      v[comp][c] = pc[o];
+    }
   }
   E[0] = lerp3(d, v[0]);
   E[1] = lerp3(d, v[1]);
@@ -1267,16 +1274,18 @@ __device__ inline SegVals segment_values(float Fx, float Fy, float Fz, float Wx,
 }
 
 // The fused step accumulates each brick's current in LDS as 64-bit fixed
-// point: value * jscale (a power of two with max|q| * jscale = 2^40) rounded
-// to the nearest integer.  Integer adds are exact and order-free (the window
-// is bit-reproducible), and ds_add_u64 runs about twice as fast as ds_add_f64
-// on gfx950.  |value * jscale| <= 2^41 < 2^51, so fma(value, jscale, 1.5*2^52)
-// leaves the rounded integer in the low mantissa bits; subtracting the bit
-// pattern of 1.5*2^52 yields it as int64.
-__device__ inline uint64_t to_fixed(float v, double jscale)
+// point.  The deposit runs with the charge scaled by jscale (a power of two
+// with max|q| * jscale <= 2^30), so every contribution is value * jscale
+// exactly and lies in [-2^30, 2^30]; it is rounded to an integer (half-way
+// cases up) by v_cvt_rpi_i32_f32 and sign-extended.  Integer adds are exact
+// and order-free (the window is bit-reproducible), and ds_add_u64 runs about
+// twice as fast as ds_add_f64 on gfx950.  The rounding unit is 2^-30 max|q|,
+// far below the float rounding of a node's current.
+__device__ inline uint64_t to_fixed(float v)
 {
-  const double t = fma(double(v), jscale, 0x1.8p52);
-  return uint64_t(__double_as_longlong(t)) - uint64_t(0x4338000000000000ull);
+  int r;
+  asm("v_cvt_rpi_i32_f32 %0, %1" : "=v"(r) : "v"(v));
+  return uint64_t(int64_t(r));
 }
 
 __device__ inline void lds_add_u64(uint64_t* p, uint64_t v)
@@ -1284,55 +1293,64 @@ __device__ inline void lds_add_u64(uint64_t* p, uint64_t v)
   __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
-// add segment a -- plus segment b when both share their cell (summed in FP32,
-// one rounding as one of the reference's FP32 atomic adds) -- at window node
-// base p
+// add the 12 contributions of one zigzag segment at window node base p
 template<int BR>
-__device__ inline void emit_window(const Shape<BR>& S, uint64_t* p, const SegVals& a, const SegVals& b, bool with_b,
-                                   double jscale)
+__device__ inline void emit_window(const Shape<BR>& S, uint64_t* p, const SegVals& a)
 {
   const int sx = S.D(1) * S.D(2), sy = S.D(2), V = S.DV();
   uint64_t* px = p;
   uint64_t* py = p + V;
   uint64_t* pz = p + 2 * V;
-  auto val = [&](int k) { return to_fixed(with_b ? a.v[k] + b.v[k] : a.v[k], jscale); };
-  lds_add_u64(px, val(0));
-  lds_add_u64(py, val(1));
-  lds_add_u64(pz, val(2));
-  lds_add_u64(py + sx, val(3));
-  lds_add_u64(pz + sx, val(4));
-  lds_add_u64(px + sy, val(5));
-  lds_add_u64(pz + sy, val(6));
-  lds_add_u64(px + 1, val(7));
-  lds_add_u64(py + 1, val(8));
-  lds_add_u64(px + sy + 1, val(9));
-  lds_add_u64(py + sx + 1, val(10));
-  lds_add_u64(pz + sx + sy, val(11));
+  lds_add_u64(px, to_fixed(a.v[0]));
+  lds_add_u64(py, to_fixed(a.v[1]));
+  lds_add_u64(pz, to_fixed(a.v[2]));
+  lds_add_u64(py + sx, to_fixed(a.v[3]));
+  lds_add_u64(pz + sx, to_fixed(a.v[4]));
+  lds_add_u64(px + sy, to_fixed(a.v[5]));
+  lds_add_u64(pz + sy, to_fixed(a.v[6]));
+  lds_add_u64(px + 1, to_fixed(a.v[7]));
+  lds_add_u64(py + 1, to_fixed(a.v[8]));
+  lds_add_u64(px + sy + 1, to_fixed(a.v[9]));
+  lds_add_u64(py + sx + 1, to_fixed(a.v[10]));
+  lds_add_u64(pz + sx + sy, to_fixed(a.v[11]));
 }
 
-// Deposit into the brick's current window.  off[d] maps a tile-haloed node
-// index of the owning tile to a window index (periodic image included).
-// False when a node falls outside the window (the caller then uses HBM).
-// A particle that stays in its cell (i1 == i2) adds both segments in one
-// pass; only cell-crossing lanes issue the second pass.
+// Deposit into the brick's current window; z carries the scaled charge.
+// off[d] maps a tile-haloed node index of the owning tile to a window index
+// (periodic image included).  False when a node falls outside the window
+// (the caller then uses HBM).  Both segments are added as the reference's
+// atomic deposit adds them, one after the other (the second at the same
+// nodes when the particle stays in its cell).
 template<int BR>
-__device__ inline bool deposit_win(const Shape<BR>& S, uint64_t* ldsJ, const Zig& z, const int off[3], double jscale)
+__device__ inline bool deposit_win(const Shape<BR>& S, uint64_t* ldsJ, const Zig& z, const int off[3])
 {
   int a[3], b[3];
-  bool in = true, same = true;
+  bool in = true;
 #pragma unroll
   for(int d = 0; d < 3; ++d) {
     a[d] = z.i1[d] + off[d];
     b[d] = z.i2[d] + off[d];
     in = in && a[d] >= 0 && b[d] >= 0 && a[d] + 1 < S.D(d) && b[d] + 1 < S.D(d);
-    same = same && z.i1[d] == z.i2[d];
   }
   if(!in) return false;
-  const SegVals s1 = segment_values(z.Fx1, z.Fy1, z.Fz1, z.Wx1, z.Wy1, z.Wz1);
-  const SegVals s2 = segment_values(z.Fx2, z.Fy2, z.Fz2, z.Wx2, z.Wy2, z.Wz2);
-  emit_window(S, ldsJ + (a[0] * S.D(1) + a[1]) * S.D(2) + a[2], s1, s2, same, jscale);
-  if(!same) emit_window(S, ldsJ + (b[0] * S.D(1) + b[1]) * S.D(2) + b[2], s2, s2, false, jscale);
+  emit_window(S, ldsJ + (a[0] * S.D(1) + a[1]) * S.D(2) + a[2],
+              segment_values(z.Fx1, z.Fy1, z.Fz1, z.Wx1, z.Wy1, z.Wz1));
+  emit_window(S, ldsJ + (b[0] * S.D(1) + b[1]) * S.D(2) + b[2],
+              segment_values(z.Fx2, z.Fy2, z.Fz2, z.Wx2, z.Wy2, z.Wz2));
   return true;
+}
+
+// the segment weights of a deposit set up with the charge scaled by a power
+// of two, scaled back by its inverse (exact)
+__device__ inline Zig unscaled(Zig z, float inv)
+{
+  z.Fx1 *= inv;
+  z.Fy1 *= inv;
+  z.Fz1 *= inv;
+  z.Fx2 *= inv;
+  z.Fy2 *= inv;
+  z.Fz2 *= inv;
+  return z;
 }
 
 // same contributions straight to HBM (global atomics), for nodes outside the window
@@ -1358,6 +1376,14 @@ __device__ inline float fmod_near(float x, float L)
   if(ax < L) return x;
   if(ax < 2.0f * L) return x - copysignf(L, x);
   return fmodf(x, L);
+}
+
+// Element q of a bin array through a 32-bit byte offset from the bin's
+// (uniform) base, so the access is one scalar base plus one lane offset.
+template<class T>
+__device__ inline T& at32(T* base, uint32_t q)
+{
+  return *reinterpret_cast<T*>(reinterpret_cast<char*>(base) + q * uint32_t(sizeof(T)));
 }
 
 // Fused batched step of one brick: gather + Boris + classification + zigzag
@@ -1431,6 +1457,8 @@ __device__ __forceinline__ void fused_brick(const Geo& g, const SpParams& sp, co
     if(count == 0) continue;
     const float qm = sp.qm[s];
     const float charge = sp.charge[s];
+    // the deposit's charge in the window's fixed-point unit (exact: a power of two)
+    const float qscaled = charge * float(g.jscale), jinv_f = float(g.jinv);
     float* const P0 = sp.pos[s][0] + start;
     float* const P1 = sp.pos[s][1] + start;
     float* const P2 = sp.pos[s][2] + start;
@@ -1450,13 +1478,13 @@ __device__ __forceinline__ void fused_brick(const Geo& g, const SpParams& sp, co
     uint64_t nid = 0;
     if(int(threadIdx.x) < count) {
       const uint32_t q0 = threadIdx.x;
-      npos[0] = P0[q0];
-      npos[1] = P1[q0];
-      npos[2] = P2[q0];
-      nvel[0] = U0[q0];
-      nvel[1] = U1[q0];
-      nvel[2] = U2[q0];
-      nid = ID[q0];
+      npos[0] = at32(P0, q0);
+      npos[1] = at32(P1, q0);
+      npos[2] = at32(P2, q0);
+      nvel[0] = at32(U0, q0);
+      nvel[1] = at32(U1, q0);
+      nvel[2] = at32(U2, q0);
+      nid = at32(ID, q0);
     }
     for(int base = 0; base < count; base += WGS) {
       const int t = base + int(threadIdx.x);
@@ -1466,13 +1494,13 @@ __device__ __forceinline__ void fused_brick(const Geo& g, const SpParams& sp, co
       int dir = 13;
       if(t + WGS < count) {
         const uint32_t qn = uint32_t(t + WGS);
-        npos[0] = P0[qn];
-        npos[1] = P1[qn];
-        npos[2] = P2[qn];
-        nvel[0] = U0[qn];
-        nvel[1] = U1[qn];
-        nvel[2] = U2[qn];
-        nid = ID[qn];
+        npos[0] = at32(P0, qn);
+        npos[1] = at32(P1, qn);
+        npos[2] = at32(P2, qn);
+        nvel[0] = at32(U0, qn);
+        nvel[1] = at32(U1, qn);
+        nvel[2] = at32(U2, qn);
+        nid = at32(ID, qn);
       }
       if(t < count) {
         float pr[3], dd[3];
@@ -1516,18 +1544,17 @@ __device__ __forceinline__ void fused_brick(const Geo& g, const SpParams& sp, co
           int lb[3];
 #pragma unroll
           for(int d = 0; d < 3; ++d) {
-            int c = int(floorf(pos[d] - Bk.origo[d])) - HALO;
-            c = c < 0 ? 0 : (c >= g.n[d] ? g.n[d] - 1 : c);
-            lb[d] = S.div(c, g, d);
             x2[d] = pos[d] - Bk.origo[d];
+            const int c = min(max(int(floorf(x2[d])) - HALO, 0), g.n[d] - 1);
+            lb[d] = BR > 0 ? int(uint32_t(c) / uint32_t(BR)) : S.div(c, g, d);
             off[d] = coff[d];
             dtoffh[d] = Bk.toffh[d];
           }
           lv1 = lb[0] != Bk.lb[0] || lb[1] != Bk.lb[1] || lb[2] != Bk.lb[2];
         }
         // zigzag deposit in the frame of the tile that owns the particle after migration
-        const Zig z = zigzag_setup<FAST>(g, x2, vel, charge, g.cfl);
-        if(!deposit_win(S, ldsJ, z, off, g.jscale)) deposit_global(g, f, z, dtoffh, &counters[C_STENCIL]);
+        const Zig z = zigzag_setup<FAST>(g, x2, vel, qscaled, g.cfl);
+        if(!deposit_win(S, ldsJ, z, off)) deposit_global(g, f, unscaled(z, jinv_f), dtoffh, &counters[C_STENCIL]);
       }
       // leavers take consecutive slots of this brick's mailbox (rank from a
       // block scan: no global atomics); past its capacity they spill into
@@ -1568,13 +1595,13 @@ __device__ __forceinline__ void fused_brick(const Geo& g, const SpParams& sp, co
       }
       if(stays) {
         const uint32_t d = uint32_t(dst);
-        P0[d] = pos[0];
-        P1[d] = pos[1];
-        P2[d] = pos[2];
-        U0[d] = vel[0];
-        U1[d] = vel[1];
-        U2[d] = vel[2];
-        if(d != uint32_t(t)) ID[d] = id;
+        at32(P0, d) = pos[0];
+        at32(P1, d) = pos[1];
+        at32(P2, d) = pos[2];
+        at32(U0, d) = vel[0];
+        at32(U1, d) = vel[1];
+        at32(U2, d) = vel[2];
+        if(d != uint32_t(t)) at32(ID, d) = id;
       }
       wr += min(WGS, count - base) - (tot - kept_tot) - dead_tot;
       nrec += (int64_t(nrec) + tot <= m.per_brick) ? tot : int(m.per_brick - nrec);

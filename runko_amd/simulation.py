@@ -14,7 +14,10 @@ Two execution modes share that interface:
               is recognised and run as one fused kernel (rk_particle_step_fused).
               Bins are re-established every step there, which is what the
               periodic sort provides for memory locality, so a prtcl_sort inside
-              that phase needs no pass of its own.
+              that phase needs no pass of its own.  The filter sequence
+              filter_current, comm_local(emf_J), filter_current, filter_current
+              (or three filter_current in a row) runs as one three-pass
+              launch (rk_filter_current_passes) on a periodic binomial2 world.
 """
 
 import ctypes
@@ -66,6 +69,7 @@ class Simulation:
         self._total_interval_time = 0
         self._pending = []
         self._pending_walls = None
+        self._pending_filter = ""  # batched filter sequence so far: "f" filter_current, "c" comm_local(emf_J)
         self._warned = set()
         self._emf_writer = None
         self._prtcl_writers = {}
@@ -171,6 +175,21 @@ class Simulation:
         ops, self._pending = self._pending, []
         for op in ops:
             self._run_unfused(op)
+        self._flush_filter()
+
+    def _flush_filter(self):
+        seq, self._pending_filter = self._pending_filter, ""
+        w = self._world
+        for head in ("fcff", "fff"):
+            if seq.startswith(head) and w.periodic and w.current_filter == "binomial2":
+                w.filter_current_passes(3)
+                seq = seq[len(head):]
+                break
+        for op in seq:
+            if op == "f":
+                w.call("rk_filter_current", -1)
+            else:
+                w.comm_local(int(comm_mode.emf_J))
 
     def _particle_op(self, op):
         if self._pending and op in _NEXT.get(self._pending[-1], ()):
@@ -231,10 +250,15 @@ class Simulation:
                     for bc in self._tile_grid.tiles()[0]._edge_bcs:
                         L.check(L.lib().rk_apply_edge_bc(self._world.h, -1, ctypes.byref(bc._struct()),
                                                          int(vargs[0]), _stream()))
+                elif batched and m == "filter_current" and not vargs:
+                    if self._tile_grid.tiles()[0].current_filter is None:
+                        raise RuntimeError("Trying to filter current without specifying `current_filter`!")
+                    ops, self._pending = self._pending, []
+                    for op in ops:
+                        self._run_unfused(op)
+                    self._pending_filter += "f"
                 elif batched and m in _RK_TILE and not vargs:
                     self._flush()
-                    if m == "filter_current" and self._tile_grid.tiles()[0].current_filter is None:
-                        raise RuntimeError("Trying to filter current without specifying `current_filter`!")
                     self._world.call(_RK_TILE[m], -1)
                 else:
                     self._flush()
@@ -256,6 +280,8 @@ class Simulation:
                 for mode in vargs:
                     if batched and mode == comm_mode.pic_particle and self._pending:
                         self._particle_op("comm")
+                    elif batched and mode == comm_mode.emf_J and self._pending_filter:
+                        self._pending_filter += "c"
                     else:
                         self._flush()
                         self._world.comm_local(int(mode.value))

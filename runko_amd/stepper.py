@@ -38,10 +38,8 @@ class BatchedLap:
         w.particle_step_fused()
         w.comm_local(int(comm_mode.emf_J_exchange))
         w.comm_local(int(comm_mode.emf_J))
-        for p in range(self.filter_passes):
-            w.call("rk_filter_current", -1)
-            if p == 0 and self.filter_passes > 1:
-                w.comm_local(int(comm_mode.emf_J))
+        # filter, comm J, filter, filter (pic.py:208-212) as one call
+        w.filter_current_passes(self.filter_passes)
         w.call("rk_push_half_b", -1)
         w.comm_local(int(comm_mode.emf_B))
         w.call("rk_push_e_add_current", -1)
@@ -226,10 +224,15 @@ class ShockLap:
         w.comm_local(int(comm_mode.emf_J_exchange))
         w.comm_local(int(comm_mode.emf_J))
         self._bcs(comm_mode.emf_J)
-        for p in range(self.filter_passes):
-            if p > 0 and p % 3 == 0:
-                w.comm_local(int(comm_mode.emf_J))
-            self._each("rk_filter_current")
+        if self.tiles is None and self.filter_passes == 3 and w.periodic and w.current_filter == "binomial2":
+            # three passes with no ghost sync between them: one launch reading
+            # J and writing the result once (the same interior bit for bit)
+            w.filter_current_passes(3)
+        else:
+            for p in range(self.filter_passes):
+                if p > 0 and p % 3 == 0:
+                    w.comm_local(int(comm_mode.emf_J))
+                self._each("rk_filter_current")
         self._bcs(comm_mode.emf_J)
         self._each("rk_push_half_b")
         self._bcs(comm_mode.emf_B)

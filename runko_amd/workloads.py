@@ -42,6 +42,52 @@ def pair_plasma_inputs(nt=(4, 4, 4), n=(16, 16, 16), ppc=8, theta=0.3, sigma=1.0
     return tuple(nt), tuple(n), [q0, q1], [1.0, 1.0], tiles
 
 
+def turbulence_sample_inputs(nt=(2, 2, 2), n=(32, 32, 32), ppc=16, theta=0.3, sigma=10.0, c_omp=10.0, n_modes=8,
+                             dB_over_B0=1.0, seed=1):
+    """A periodic sub-box of the decaying-turbulence workload, in the layout of
+    pair_plasma_inputs: its 32^3-cell tiles, 16 ppc per species, theta 0.3,
+    sigma 10 (q and B0 of the full box), Bz = B0 plus the 8x8 dBx, dBy modes
+    of init.set_turbulent_field laid over the sub-box (so they stay periodic
+    in it).  The defaults (2^3 tiles, 8.39e6 particles) are the CPU timing
+    sample of the 512^3 workload."""
+    q0, q1, B0 = plasma_parameters(CFL, c_omp, ppc, theta, sigma)
+    N = [a * b for a, b in zip(nt, n)]
+    rng = np.random.default_rng(seed)
+    phi = 2 * np.pi * rng.random((n_modes, n_modes))
+    kx, ky = 2 * np.pi / N[0], 2 * np.pi / N[1]
+    X = np.arange(N[0] + 1, dtype=np.float64)[:, None]
+    Y = np.arange(N[1] + 1, dtype=np.float64)[None, :]
+    A = np.zeros((N[0] + 1, N[1] + 1))
+    for m in range(1, n_modes + 1):
+        for k in range(1, n_modes + 1):
+            kd = np.sqrt((2 * np.sin(m * kx / 2)) ** 2 + (2 * np.sin(k * ky / 2)) ** 2)
+            A += np.sin(m * kx * X + k * ky * Y + phi[m - 1, k - 1]) / kd
+    bx = A[:-1, 1:] - A[:-1, :-1]
+    by = A[:-1, :-1] - A[1:, :-1]
+    scale = dB_over_B0 * B0 / np.sqrt(np.mean(bx ** 2 + by ** 2))
+    bx = (scale * bx).astype(np.float32).astype(np.float64)
+    by = (scale * by).astype(np.float32).astype(np.float64)
+    cells = np.stack(np.meshgrid(*(np.arange(v, dtype=np.float64) for v in n), indexing="ij"), -1).reshape(-1, 3)
+    tiles = {}
+    for i in range(nt[0]):
+        for j in range(nt[1]):
+            for k in range(nt[2]):
+                lo = np.array([i * n[0], j * n[1], k * n[2]], dtype=np.float64)
+                pos = np.repeat(cells, ppc, 0) + lo + rng.random((len(cells) * ppc, 3))
+                parts = []
+                for s in range(2):
+                    ux, uy, uz = sample_boosted_juttner_synge(len(pos), theta, beta=0, gen=rng)
+                    parts.append(np.concatenate([pos, np.stack([ux, uy, uz], 1)], 1))
+                F = np.zeros((9,) + tuple(n))
+                sx = slice(i * n[0], (i + 1) * n[0])
+                sy = slice(j * n[1], (j + 1) * n[1])
+                F[3] = bx[sx, sy][:, :, None]
+                F[4] = by[sx, sy][:, :, None]
+                F[5] = B0
+                tiles[(i, j, k)] = (F, parts)
+    return tuple(nt), tuple(n), [q0, q1], [1.0, 1.0], tiles
+
+
 def make_configuration(nt, n, charges, masses, filt="binomial2", cfl=CFL, execution="plain", pusher="boris",
                        stencil=None):
     c = Configuration(None)

@@ -46,6 +46,13 @@ def _stream():
     return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
 
 
+def free_device_bytes(dev):
+    """Device memory a new tensor can take: free on the device plus what
+    torch's caching allocator holds without using it."""
+    free, _ = torch.cuda.mem_get_info(dev)
+    return int(free) + int(torch.cuda.memory_reserved(dev)) - int(torch.cuda.memory_allocated(dev))
+
+
 def _require_gpu():
     if not torch.cuda.is_available():
         raise RuntimeError("runko_amd needs a ROCm GPU (no CPU fallback)")
@@ -158,7 +165,8 @@ class World:
             d.brick[a] = self.brick[a]
         d.periodic = 1 if periodic else 0
         d.n_species = self.nsp
-        d.current_filter = {"binomial2": 0, "binomial2_unrolled": 1}[current_filter or "binomial2"]
+        self.current_filter = current_filter or "binomial2"
+        d.current_filter = {"binomial2": 0, "binomial2_unrolled": 1}[self.current_filter]
         d.cfl = self.cfl
         for s in range(self.nsp):
             d.charge[s] = self.charges[s]
@@ -306,6 +314,12 @@ class World:
             return
         L.check(rc)
 
+    def filter_current_passes(self, passes):
+        """`passes` filter passes over every tile from a current J ghost layer,
+        the ghost layer synced between passes (rk_filter_current_passes: one
+        launch for three binomial2 passes on a periodic world)."""
+        L.check(L.lib().rk_filter_current_passes(self.h, int(passes), _stream()))
+
     def comm_local(self, mode):
         rc = L.lib().rk_comm_local(self.h, int(mode), _stream())
         if rc == L.RK_EOVERFLOW:
@@ -433,7 +447,7 @@ class World:
         if caps is None:
             self._grow_pool(s, demand)
             return
-        free, total = torch.cuda.mem_get_info(self.dev)
+        free = free_device_bytes(self.dev)
         stage_bytes = int(min(getattr(self, "stage_bytes", STAGE_MAX), max(free // 2, 1 << 20)))
         stage = torch.empty((stage_bytes,), dtype=torch.uint8, device=self.dev)
         caps32 = np.ascontiguousarray(caps.astype(np.uint32))
@@ -451,7 +465,8 @@ class World:
         beside the old one and the reserve."""
         caps = bin_caps(demand)
         used = int(caps.sum())
-        free, total = torch.cuda.mem_get_info(self.dev)
+        _, total = torch.cuda.mem_get_info(self.dev)
+        free = free_device_bytes(self.dev)
         reserve = max(RESERVE_MIN, int(RESERVE_FRAC * total))
         arena = plan_arena([used], free, reserve)[0] if free > reserve + used * SLOT_BYTES else -1
         if arena < 0:
