@@ -75,6 +75,7 @@ def lib():
         L.orc_num_outgoing.restype = ctypes.c_long
         L.orc_get_outgoing.argtypes = [vp, ip, vp, vp, vp, vp]
         L.orc_energies.argtypes = [vp, vp]
+        L.orc_gauss_residual.argtypes = [vp, ip, vp]
         L.orc_max_threads.restype = ip
         L.orc_set_threads.argtypes = [ip]
         _lib = L
@@ -244,6 +245,14 @@ class OracleWorld:
     def energies(self):
         out = np.zeros(2 + self.nsp)
         _check(lib().orc_energies(self._h, out.ctypes.data))
+        return out
+
+    def gauss_residual(self, passes):
+        """(max |div E - F^passes(rho)|, rms F^passes(rho), rms div E) in FP64
+        on the periodic grid, rho the cloud-in-cell charge (the definition of
+        the GPU library's rk_gauss_residual, restated on this world)."""
+        out = np.zeros(3)
+        _check(lib().orc_gauss_residual(self._h, int(passes), out.ctypes.data))
         return out
 
     def all_particles(self, s):

@@ -1553,4 +1553,89 @@ int orc_energies(void* hp, double* out)
   }); });
 }
 
+// Gauss-law residual in FP64 on the periodic grid of all tiles: rho = first-
+// order (cloud-in-cell) charge of the alive particles at the lattice nodes,
+// cell weights from p = x - (tile mins - 3) in the lattice precision and
+// floor(p), as the deposit indexes cells; F^passes(rho) with the (1,2,1)^3/64
+// filter; div E at node i = (Ex[i]-Ex[i-1]) + (Ey[j]-Ey[j-1]) + (Ez[k]-Ez[k-1]).
+// out[0] = max |div E - F(rho)|, out[1] = rms F(rho), out[2] = rms div E.
+// The reference has no such diagnostic; this is the GPU library's
+// rk_gauss_residual restated on the oracle's own state.
+int orc_gauss_residual(void* hp, int passes, double* out)
+{
+  auto* h = static_cast<Handle*>(hp);
+  return guard([&] { dispatch(h, [&](auto& w) { WT;
+    const int N[3] = {w.nt[0] * w.n[0], w.nt[1] * w.n[1], w.nt[2] * w.n[2]};
+    const std::size_t V = std::size_t(N[0]) * N[1] * N[2];
+    auto gi = [&](long i, long j, long k) {
+      i = ((i % N[0]) + N[0]) % N[0];
+      j = ((j % N[1]) + N[1]) % N[1];
+      k = ((k % N[2]) + N[2]) % N[2];
+      return (std::size_t(i) * N[1] + std::size_t(j)) * N[2] + std::size_t(k);
+    };
+    std::vector<double> rho(V, 0.0), tmp(V, 0.0), Ex(V), Ey(V), Ez(V);
+    for(const auto& t : w.tiles) {
+      T origo[3];
+      tile_origo(t, origo);
+      const auto& l = t.lat;
+      for(int i = 0; i < l.n[0]; ++i)
+        for(int j = 0; j < l.n[1]; ++j)
+          for(int k = 0; k < l.n[2]; ++k) {
+            const std::size_t q = l.at(i + H, j + H, k + H);
+            const std::size_t g = gi(long(t.idx[0]) * w.n[0] + i, long(t.idx[1]) * w.n[1] + j,
+                                     long(t.idx[2]) * w.n[2] + k);
+            Ex[g] = double(l.E[0][q]);
+            Ey[g] = double(l.E[1][q]);
+            Ez[g] = double(l.E[2][q]);
+          }
+      for(const auto& c : t.sp) {
+        for(std::size_t p = 0; p < c.size(); ++p) {
+          if(c.id[p] == DEAD) continue;
+          long I[3];
+          double f1[3];
+          for(int d = 0; d < 3; ++d) {
+            const T pr = c.x[d][p] - origo[d];
+            const T f = std::floor(pr);
+            I[d] = long(t.idx[d]) * w.n[d] + long(f) - H;
+            f1[d] = double(pr) - double(f);
+          }
+          for(int a = 0; a < 2; ++a)
+            for(int b = 0; b < 2; ++b)
+              for(int e = 0; e < 2; ++e) {
+                const double wgt = (a ? f1[0] : 1.0 - f1[0]) * (b ? f1[1] : 1.0 - f1[1]) * (e ? f1[2] : 1.0 - f1[2]);
+                rho[gi(I[0] + a, I[1] + b, I[2] + e)] += c.charge * wgt;
+              }
+        }
+      }
+    }
+    const double wt[3] = {1.0, 2.0, 1.0};
+    for(int pass = 0; pass < passes; ++pass) {
+      for(long i = 0; i < N[0]; ++i)
+        for(long j = 0; j < N[1]; ++j)
+          for(long k = 0; k < N[2]; ++k) {
+            double acc = 0.0;
+            for(int a = 0; a < 3; ++a)
+              for(int b = 0; b < 3; ++b)
+                for(int e = 0; e < 3; ++e)
+                  acc += wt[a] * wt[b] * wt[e] / 64.0 * rho[gi(i + a - 1, j + b - 1, k + e - 1)];
+            tmp[gi(i, j, k)] = acc;
+          }
+      std::swap(rho, tmp);
+    }
+    double mx = 0.0, r2 = 0.0, d2 = 0.0;
+    for(long i = 0; i < N[0]; ++i)
+      for(long j = 0; j < N[1]; ++j)
+        for(long k = 0; k < N[2]; ++k) {
+          const std::size_t g = gi(i, j, k);
+          const double div = (Ex[g] - Ex[gi(i - 1, j, k)]) + (Ey[g] - Ey[gi(i, j - 1, k)]) + (Ez[g] - Ez[gi(i, j, k - 1)]);
+          mx = std::max(mx, std::fabs(div - rho[g]));
+          r2 += rho[g] * rho[g];
+          d2 += div * div;
+        }
+    out[0] = mx;
+    out[1] = std::sqrt(r2 / double(V));
+    out[2] = std::sqrt(d2 / double(V));
+  }); });
+}
+
 }  // extern "C"

@@ -353,6 +353,43 @@ def test_gauss_residual(execution):
     assert g[0] < 0.05 * g[1]
 
 
+@pytest.mark.parametrize("execution", ["plain", "batched"])
+def test_gauss_residual_matches_fp64_oracle(orc, execution):
+    # The same neutral start and laps on the GPU (FP32) and in the FP64 oracle:
+    # each world's residual div E - F^3 rho on its own state.  The residual is
+    # the 3D zigzag defect, a property of the scheme (not of rounding), so the
+    # FP32 GPU run must reproduce the FP64 oracle's residual; its size
+    # relative to rms rho is printed (the reference's scheme keeps no Gauss
+    # law, so no bound like 1e-5 of rms rho can hold).
+    nt, n = (2, 2, 2), (8, 8, 8)
+    inputs = random_inputs(nt, n, 2, seed=12, ppc=4, umax=1.0)
+    for F, parts in inputs.values():
+        F[0:3] = 0.0
+        parts[1][:, :3] = parts[0][:, :3]
+    conf, grid, sim = build_gpu(nt, n, Q, M, inputs, execution=execution)
+    w64 = build_oracle(orc, nt, n, Q, M, inputs, prec=64)
+    w32 = build_oracle(orc, nt, n, Q, M, inputs)
+    assert w64.gauss_residual(3)[0] < 1e-15
+    step = canonical_lap(sim)
+    for lap in range(6):
+        sim.for_one_lap(step)
+        w64.lap(lap)
+        w32.lap(lap)
+    g = grid.world.gauss_residual(3)
+    o64, o32 = w64.gauss_residual(3), w32.gauss_residual(3)
+    print(f"gauss residual {execution}: GPU {g}, FP64 oracle {o64}, FP32 oracle {o32}; "
+          f"max residual / rms rho: GPU {g[0] / g[1]:.3e}, FP64 oracle {o64[0] / o64[1]:.3e}")
+    # rms rho and rms div E agree within the FP32 bar; the max residual sits
+    # at one node where the defect peaks, and there the J rounding differs
+    # (the GPU rounds each contribution to 2^-30 of max|q| and sums exactly,
+    # the oracle sums in FP32 or FP64): measured 0.56 % from the FP64 oracle
+    # and 0.66 % from the FP32 one, while the two oracles differ by 0.1 %
+    assert g[0] == pytest.approx(o64[0], rel=2e-2)
+    assert g[0] == pytest.approx(o32[0], rel=2e-2)
+    assert g[1] == pytest.approx(o64[1], rel=1e-6)
+    assert g[2] == pytest.approx(o64[2], rel=1e-6)
+
+
 def test_graph_lap_matches_eager():
     # two identical worlds: one advanced eagerly, one by graph replays with
     # eager work (energies, syncs) between the replays; every lap compared
