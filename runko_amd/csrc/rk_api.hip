@@ -664,10 +664,7 @@ int fused_step(rk_world* w, const rk_reflector_wall* walls, int32_t n_walls, boo
   }
   ProfScope ps(w, 4, s);
   if(int rc = launch_append_mailbox(w, s)) return rc;
-  if(int rc = launch_append(w, 0, s)) return rc;
-  if(int rc = launch_append(w, 1, s)) return rc;
-  if(int rc = launch_reset_counter(w, C_MIG0, s)) return rc;
-  return launch_reset_counter(w, C_MIG1, s);
+  return launch_append_records(w, s);
 }
 }  // namespace
 

@@ -533,13 +533,28 @@ __device__ inline int wrap_idx(int c, int N) { return c < HALO ? c + N : (c >= N
 
 // periodic ghost copy: every ghost node takes the value of its interior image
 // (sources are interior nodes, destinations ghost nodes, so the copies are
-// independent).  k_ghost_rows copies the whole z rows whose i or j lies in
-// the ghost shell: 2*HALO*L1 rows with a ghost i, then N0*2*HALO rows with an
-// interior i and a ghost j, one workgroup per row.  k_ghost_z copies the z
-// ghosts of the interior rows, one lane per ghost node.
-__global__ void __launch_bounds__(64) k_ghost_rows(Geo g, float* F0, float* F1, float* F2)
+// independent and one launch does them all).  The first nrows workgroups
+// copy the whole z rows whose i or j lies in the ghost shell: 2*HALO*L1 rows
+// with a ghost i, then N0*2*HALO rows with an interior i and a ghost j, one
+// workgroup per row.  The workgroups after them copy the z ghosts of the
+// interior rows, one lane per ghost node.
+__global__ void __launch_bounds__(64) k_ghost_copy(Geo g, int nrows, float* F0, float* F1, float* F2)
 {
-  int b = int(blockIdx.x), i, j;
+  int b = int(blockIdx.x);
+  if(b >= nrows) {
+    const int64_t t = int64_t(b - nrows) * 64 + threadIdx.x;
+    if(t >= int64_t(g.N[0]) * g.N[1] * 2 * HALO) return;
+    const int h = int(t % (2 * HALO));
+    const int64_t row = t / (2 * HALO);
+    const int i = HALO + int(row / g.N[1]), j = HALO + int(row % g.N[1]);
+    const int k = h < HALO ? h : g.N[2] + h;
+    const uint32_t d = g.at(i, j, k), s = g.at(i, j, wrap_idx(k, g.N[2]));
+    F0[d] = F0[s];
+    F1[d] = F1[s];
+    F2[d] = F2[s];
+    return;
+  }
+  int i, j;
   const int nA = 2 * HALO * g.L[1];
   if(b < nA) {
     const int ii = b / g.L[1];
@@ -558,20 +573,6 @@ __global__ void __launch_bounds__(64) k_ghost_rows(Geo g, float* F0, float* F1, 
     F1[d] = F1[s];
     F2[d] = F2[s];
   }
-}
-
-__global__ void __launch_bounds__(256) k_ghost_z(Geo g, float* F0, float* F1, float* F2)
-{
-  const int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if(t >= int64_t(g.N[0]) * g.N[1] * 2 * HALO) return;
-  const int h = int(t % (2 * HALO));
-  const int64_t row = t / (2 * HALO);
-  const int i = HALO + int(row / g.N[1]), j = HALO + int(row % g.N[1]);
-  const int k = h < HALO ? h : g.N[2] + h;
-  const uint32_t d = g.at(i, j, k), s = g.at(i, j, wrap_idx(k, g.N[2]));
-  F0[d] = F0[s];
-  F1[d] = F1[s];
-  F2[d] = F2[s];
 }
 
 // images of an interior coordinate c in the ghost shell: c-N (c >= N) and c+N (c < 2*HALO)
@@ -762,10 +763,9 @@ int launch_filter_x3(rk_world* w, hipStream_t s)
 int launch_ghost_copy(rk_world* w, float* const F[3], hipStream_t s)
 {
   const Geo& g = w->g;
-  hipLaunchKernelGGL(k_ghost_rows, dim3(unsigned(2 * HALO * g.L[1] + g.N[0] * 2 * HALO)), dim3(64), 0, s, g, F[0],
-                     F[1], F[2]);
+  const int nrows = 2 * HALO * g.L[1] + g.N[0] * 2 * HALO;
   const int64_t nz = int64_t(g.N[0]) * g.N[1] * 2 * HALO;
-  hipLaunchKernelGGL(k_ghost_z, dim3(unsigned((nz + 255) / 256)), dim3(256), 0, s, g, F[0], F[1], F[2]);
+  hipLaunchKernelGGL(k_ghost_copy, dim3(unsigned(nrows + (nz + 63) / 64)), dim3(64), 0, s, g, nrows, F[0], F[1], F[2]);
   RK_HIP(hipGetLastError());
   return RK_OK;
 }

@@ -130,6 +130,7 @@ enum Counter {
   C_MIGLOST = 3,     // records that did not fit the record buffer (particle kept in place)
   C_STENCIL = 4,     // particles whose stencil left the haloed lattice
   C_SLOWCOMPACT = 5, // bins compacted on the slow path
+  C_APPDONE = 6,     // workgroups done in k_append_records (the last one clears C_MIG0/1 and itself)
   C_QUEUE = 8,       // 8 per-XCD brick queues of the persistent fused step
   C_NUM = 16
 };
@@ -244,6 +245,7 @@ int launch_deposit(rk_world* w, int tile, hipStream_t s);
 int launch_pack(rk_world* w, int tile, hipStream_t s);
 int launch_append(rk_world* w, int set, hipStream_t s);
 int launch_append_mailbox(rk_world* w, hipStream_t s);
+int launch_append_records(rk_world* w, hipStream_t s);  // sets 0 and 1, then clears their counters
 int launch_current_gather(rk_world* w, hipStream_t s);
 int verify_fast_division(float cfl, float rcp, uint64_t mismatches[2]);
 int upload_mig_table(rk_world* w);

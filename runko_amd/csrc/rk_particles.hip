@@ -1106,6 +1106,30 @@ __global__ void __launch_bounds__(WG)
   }
 }
 
+// Append the records of sets 0 and 1 (blockIdx.y is the set), then clear
+// both record counters: every workgroup reads its set's count before it
+// signs off on C_APPDONE, and the last workgroup to sign off clears C_MIG0,
+// C_MIG1 and C_APPDONE, so the counters are zero for the next step without
+// separate launches.
+__global__ void __launch_bounds__(WG)
+  k_append_records(Geo g, SpParams sp, MigPtrs m, uint32_t* counters, uint32_t* ovf, int64_t ovf_cap)
+{
+  const int set = int(blockIdx.y);
+  const int n = int(min(int64_t(counters[set == 0 ? C_MIG0 : C_MIG1]), m.cap[set]));
+  for(int e = int(blockIdx.x * blockDim.x + threadIdx.x); e < n; e += int(gridDim.x * blockDim.x))
+    append_record(g, sp, m, set, uint32_t(e), counters, ovf, ovf_cap);
+  __syncthreads();
+  if(threadIdx.x == 0) {
+    __threadfence();
+    const uint32_t done = atomicAdd(&counters[C_APPDONE], 1u);
+    if(done == gridDim.x * gridDim.y - 1) {
+      counters[C_MIG0] = 0;
+      counters[C_MIG1] = 0;
+      counters[C_APPDONE] = 0;
+    }
+  }
+}
+
 // Global brick coordinate of bin b along axis d.
 __device__ inline int global_brick(const Geo& g, int b, int d)
 {
@@ -1673,7 +1697,14 @@ __global__ void k_reset_queues(uint32_t* counters)
 __global__ void __launch_bounds__(128)
   k_current_gather(Geo g, const float* __restrict__ stage, int64_t stride, float* J0, float* J1, float* J2)
 {
-  const int Jn = int(blockIdx.x), In = int(blockIdx.y);
+  // Workgroups are dealt to the 8 XCDs round-robin by linear index; XCD x
+  // takes the x-th contiguous share of the rows instead, so the rows that
+  // share a window's cache lines (neighbouring Jn) go through one L2.
+  const int total = int(gridDim.x * gridDim.y);
+  const int lin = int(blockIdx.y * gridDim.x + blockIdx.x);
+  const int xcd = lin % 8, q8 = total / 8, r8 = total % 8;
+  const int row = xcd * q8 + min(xcd, r8) + lin / 8;
+  const int Jn = row % int(gridDim.x), In = row / int(gridDim.x);
   const int D[3] = {g.br[0] + 3, g.br[1] + 3, g.br[2] + 3};
   const int V = D[0] * D[1] * D[2];
   const int Nb[3] = {g.nt[0] * g.nb[0], g.nt[1] * g.nb[1], g.nt[2] * g.nb[2]};
@@ -1690,6 +1721,70 @@ __global__ void __launch_bounds__(128)
     if(o >= D[1]) break;
     by[ny] = gb;
     oy[ny++] = o;
+  }
+  if(nx <= 2 && ny <= 2 && D[2] <= 2 * g.br[2]) {
+    // every node lies in at most two windows per axis: the (up to 8) windows
+    // of a node are addressed first and their 24 values loaded together,
+    // then summed in the order of the general loop below (bit-identical)
+    int64_t wxy[2][2];
+#pragma unroll
+    for(int a = 0; a < 2; ++a)
+#pragma unroll
+      for(int c = 0; c < 2; ++c) {
+        const int ga = a < nx ? bx[a] : 0, gc = c < ny ? by[c] : 0;
+        const int tx = ga / g.nb[0], lx = ga - tx * g.nb[0];
+        const int ty = gc / g.nb[1], ly = gc - ty * g.nb[1];
+        const int64_t bin0 = int64_t(g.tile_lin(tx, ty, 0)) * g.bpt + (lx * g.nb[1] + ly) * g.nb[2];
+        wxy[a][c] = bin0 * stride + ((a < nx ? ox[a] : 0) * D[1] + (c < ny ? oy[c] : 0)) * D[2];
+      }
+    for(int K = int(threadIdx.x); K < g.L[2]; K += int(blockDim.x)) {
+      int64_t wz[2];
+      bool vz[2];
+      const int gz1 = K >= 2 ? min((K - 2) / g.br[2], Nb[2] - 1) : -1;
+#pragma unroll
+      for(int zi = 0; zi < 2; ++zi) {
+        const int gz = gz1 - zi;
+        const int oz = K - (2 + gz * g.br[2]);
+        vz[zi] = gz >= 0 && oz < D[2];
+        const int tz = vz[zi] ? gz / g.nb[2] : 0, lz = vz[zi] ? gz - tz * g.nb[2] : 0;
+        wz[zi] = (int64_t(tz) * g.bpt + lz) * stride + oz;
+      }
+      vz[1] = vz[1] && vz[0];
+      float v[2][2][2][3];
+#pragma unroll
+      for(int zi = 0; zi < 2; ++zi)
+#pragma unroll
+        for(int a = 0; a < 2; ++a)
+#pragma unroll
+          for(int c = 0; c < 2; ++c) {
+            // unused windows read the first staged value (in bounds) and are
+            // not added: the loads stay unconditional, so all 24 are in
+            // flight together
+            const bool ok = vz[zi] && a < nx && c < ny;
+            const float* w = stage + (ok ? wxy[a][c] + wz[zi] : 0);
+            v[zi][a][c][0] = w[0];
+            v[zi][a][c][1] = w[V];
+            v[zi][a][c][2] = w[2 * V];
+          }
+      float a0 = 0.0f, a1 = 0.0f, a2 = 0.0f;
+#pragma unroll
+      for(int zi = 0; zi < 2; ++zi)
+#pragma unroll
+        for(int a = 0; a < 2; ++a)
+#pragma unroll
+          for(int c = 0; c < 2; ++c) {
+            const bool ok = vz[zi] && a < nx && c < ny;
+            const float s0 = a0 + v[zi][a][c][0], s1 = a1 + v[zi][a][c][1], s2 = a2 + v[zi][a][c][2];
+            a0 = ok ? s0 : a0;
+            a1 = ok ? s1 : a1;
+            a2 = ok ? s2 : a2;
+          }
+      const uint32_t q = g.at(In, Jn, K);
+      J0[q] = J0[q] + a0;
+      J1[q] = J1[q] + a1;
+      J2[q] = J2[q] + a2;
+    }
+    return;
   }
   for(int K = int(threadIdx.x); K < g.L[2]; K += int(blockDim.x)) {
     float a0 = 0.0f, a1 = 0.0f, a2 = 0.0f;
@@ -2050,6 +2145,14 @@ int launch_append_mailbox(rk_world* w, hipStream_t s)
   return RK_OK;
 }
 
+int launch_append_records(rk_world* w, hipStream_t s)
+{
+  hipLaunchKernelGGL(k_append_records, dim3(512, 2), dim3(WG), 0, s, w->g, make_sp(w), make_mig(w), w->counters,
+                     w->ovf_list, w->ovf_cap);
+  RK_HIP(hipGetLastError());
+  return RK_OK;
+}
+
 int launch_reset_counter(rk_world* w, int which, hipStream_t s)
 {
   hipLaunchKernelGGL(k_reset, dim3(1), dim3(64), 0, s, w->counters, which);
@@ -2078,9 +2181,10 @@ int launch_fused(rk_world* w, hipStream_t s, const WallSet* walls, const rk_refl
     return RK_EINVAL;
   }
   if(g.pusher != 0) {
-    // alternative pushers (their own arithmetic in IEEE forms), 8^3 bricks or
-    // the generic shape; on 8^3 bricks the deposit and classification take
-    // the verified sqrt/rcp forms, which give the IEEE results
+    // Higuera-Cary or Faraday, 8^3 bricks or the generic shape; on 8^3 bricks
+    // with both fast forms verified, the pusher, the deposit and the
+    // classification take the verified sqrt/rcp forms (the IEEE results),
+    // otherwise the IEEE operations
     const bool hc = g.pusher == 1;
     if(ws.n > 0) {
       if(cube8 && lvl == 2) hc ? go(k_fused<8, 2, 1, 1>) : go(k_fused<8, 2, 1, 2>);

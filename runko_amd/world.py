@@ -412,8 +412,9 @@ class World:
             rc = L.lib().rk_retry_append(self.h, _stream())
             if rc != L.RK_EOVERFLOW:  # overflow again: the next pass gives those bins room
                 L.check(rc)
-        if self.pending_overflow():
-            raise RuntimeError("runko_amd: particle bins still overflowing after four relayouts")
+        else:
+            if self.pending_overflow():
+                raise RuntimeError("runko_amd: particle bins still overflowing after four relayouts")
         self.settle_seconds += time.perf_counter() - t0
         # the mailboxes keep their size (leavers past it spill into the shared
         # record sets); replacing them here would orphan pending records
