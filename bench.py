@@ -224,6 +224,76 @@ def run_turbulence(args):
         autotune=dict(brick=tuned, fused_launch=launch))
 
 
+def run_turbulence_long(args):
+    """Configuration 2 at its stated length: the 512^3 turbulence world for
+    args.long_run laps of the batched lap replayed from HIP graphs, with the
+    overflow settle after every lap.  Every 50 laps (and at the end): field
+    and kinetic energies (FP64 reductions), the alive particle count and the
+    error counters; every 250 laps the Gauss residual (its FP64 charge
+    deposit takes about 1 s at this size).  The diagnostics are timed apart
+    from the laps; the run stops at the first error."""
+    import numpy as np
+    import torch
+
+    from runko_amd.init import turbulence_world
+    from runko_amd.stepper import GraphLap
+
+    N, tile, ppc = args.cells, args.tile, args.ppc
+    t0 = time.time()
+    w, info = turbulence_world(N=N, tile=tile, ppc=ppc, theta=0.3, sigma=10.0, cfl=0.45, c_omp=10.0, n_modes=8,
+                               dB_over_B0=1.0, seed=1, mig_fraction=args.mig_fraction, slot_order=args.slot_order)
+    npart = w.total_particles()
+    log(f"[long] turbulence {N}^3, {npart} particles, set-up {time.time() - t0:.1f}s")
+    gl = GraphLap(w)
+    hist = []
+
+    diag = [0.0]
+
+    def sample(lap, wall):
+        td = time.perf_counter()
+        e = w.energies()
+        err = [int(v) for v in w.error_counters()]
+        alive = w.total_particles()
+        rec = dict(lap=lap, wall=wall, field_E=float(e[0]), field_B=float(e[1]),
+                   kinetic=[float(v) for v in e[2:]], particles=alive, errors=err,
+                   bin_moves=w.bin_moves, relayouts=w.relayouts, recaptures=gl.recaptures)
+        if lap % 250 == 0 or lap == args.long_run:
+            g = w.gauss_residual(3)
+            rec["gauss_max_over_rms_rho"] = float(g[0] / g[1]) if g[1] > 0 else 0.0
+        torch.cuda.synchronize()
+        diag[0] += time.perf_counter() - td
+        hist.append(rec)
+        log(f"[long] lap {lap}: field energy B {rec['field_B']:.6e}, E {rec['field_E']:.6e}; kinetic "
+            f"{', '.join(f'{k:.6e}' for k in rec['kinetic'])}; particles {alive}, errors {err}, "
+            f"bin moves {w.bin_moves}, relayouts {w.relayouts}, {wall:.1f}s")
+        return alive == npart and err == [0, 0]
+
+    torch.cuda.synchronize()
+    ok = sample(0, 0.0)
+    diag[0] = 0.0
+    tl = time.perf_counter()
+    laps = 0
+    settle_s = w.settle_seconds
+    for i in range(1, args.long_run + 1):
+        gl()
+        gl.settle()
+        laps = i
+        if i % 50 == 0 or i == args.long_run:
+            torch.cuda.synchronize()
+            if not sample(i, time.perf_counter() - tl):
+                ok = False
+                break
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - tl - diag[0]
+    out = dict(workload=f"3D decaying turbulence, {N}^3 cells in {tile}^3-cell tiles, {ppc} particles/cell/species, "
+                        f"theta 0.3, sigma 10, 8x8 dB modes, batched lap from HIP graphs",
+               laps=laps, ok=ok, particles=npart, wall_s=wall, ms_per_lap=1e3 * wall / max(laps, 1),
+               settle_ms_per_lap=1e3 * (w.settle_seconds - settle_s) / max(laps, 1),
+               diagnostics_s=diag[0], bin_moves=w.bin_moves, relayouts=w.relayouts, history=hist)
+    print(json.dumps(out), flush=True)
+    return 0 if ok else 1
+
+
 def config0_inputs(seed=42):
     """64^3 cells, 4^3 tiles of 16^3, 8 ppc/species, theta 0.3, sigma 1, E = 0,
     B = B0 z (projects/pic-turbulence/pic.py conventions)."""
@@ -729,6 +799,8 @@ def main():
     ap.add_argument("--ensemble-every", type=int, default=0,
                     help="flat checkpoint spacing of the ensemble rollout (0: chosen from free memory, nested if "
                          "flat does not fit)")
+    ap.add_argument("--long-run", type=int, default=0, metavar="LAPS",
+                    help="only the 512^3 turbulence world for LAPS graph laps with diagnostics every 50 laps")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="also run plain copies of MIB MiB (scale for rocprof byte counters)")
     args = ap.parse_args()
@@ -751,6 +823,8 @@ def main():
     if args.only_ensemble:
         print(json.dumps(run_ensemble(args)), flush=True)
         return
+    if args.long_run:
+        sys.exit(run_turbulence_long(args))
     if args.copy_probe:
         run_copy_probe(args.copy_probe)
     res = run_turbulence(args)

@@ -199,8 +199,9 @@ class Simulation:
                 self._pending = []
                 w = self._world
                 w.particle_step_fused(walls)
-                if w.pending_overflow():
-                    w.settle_overflow()
+                n = w.pending_overflow()
+                if n:
+                    w.settle_overflow(n)
             return
         self._flush()
         if op == "push":

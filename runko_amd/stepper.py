@@ -48,8 +48,9 @@ class BatchedLap:
     def settle(self):
         """Synchronous check after a lap: bins that overflowed are repacked and
         their pending particles appended (none are lost)."""
-        if self.w.pending_overflow():
-            self.w.settle_overflow()
+        n = self.w.pending_overflow()
+        if n:
+            self.w.settle_overflow(n)
 
 
 class PlainLap:
@@ -84,8 +85,9 @@ class PlainLap:
         torch.cuda.synchronize()
 
     def settle(self):
-        if self.w.pending_overflow():
-            self.w.settle_overflow()
+        n = self.w.pending_overflow()
+        if n:
+            self.w.settle_overflow(n)
 
 
 class GraphLap:
@@ -148,8 +150,9 @@ class GraphLap:
         laps is scratch (every lap rebuilds it), so recapturing after an odd
         number of replays changes nothing but which buffer the next lap's J
         starts in."""
-        if self.w.pending_overflow():
-            self.w.settle_overflow()
+        n = self.w.pending_overflow()
+        if n:
+            self.w.settle_overflow(n)
         if self.w.layout_version != self.version:
             t0 = time.perf_counter()
             del self.graphs
@@ -253,8 +256,9 @@ class ShockLap:
             torch.cuda.synchronize()
 
     def settle(self):
-        if self.w.pending_overflow():
-            self.w.settle_overflow()
+        n = self.w.pending_overflow()
+        if n:
+            self.w.settle_overflow(n)
 
 
 class PlainShockLap(ShockLap):

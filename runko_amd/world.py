@@ -95,19 +95,31 @@ def plan_arena(used, free_bytes, reserve_bytes, target=ARENA_TARGET):
     return budget * used // max(total, 1)
 
 
+RELAYOUT_SLACK = 1.5    # bin capacity per particle after a relayout, if the arena keeps ARENA_MIN
+ARENA_MIN = 0.15        # fraction of the pool left spare by that first choice
+
+
 def plan_relayout(demand, caps, grown, capacity):
     """New capacities of every bin for an in-place relayout of a pool of
-    `capacity` slots: bins whose demand exceeds their capacity get `grown`,
-    the others keep theirs; if that does not fit, every bin gets its demand
-    plus slack, then plus a pad, then exactly its demand.  Returns None when
-    even the demand exceeds the pool."""
+    `capacity` slots.  A relayout moves every bin anyway, so bins that
+    emptied give their room back.  First choice: every bin its demand times
+    RELAYOUT_SLACK plus a pad (a bin whose demand exceeds its capacity at
+    least `grown`), if that leaves ARENA_MIN of the pool spare for bins that
+    overflow later; then the same with the set-up's slack; then full bins
+    grown and the others keeping theirs; then every bin its demand plus the
+    set-up's slack, then plus a pad, then exactly its demand.  Returns None
+    when even the demand exceeds the pool."""
     demand = np.asarray(demand, dtype=np.int64)
     caps = np.asarray(caps, dtype=np.int64)
     full = demand > caps
-    for c in (np.where(full, grown, caps), bin_caps(demand), np.where(demand > 0, round32(demand + PAD), 0),
-              demand):
+    loose = bin_caps(demand, RELAYOUT_SLACK)
+    bc = bin_caps(demand)
+    roomy = int((1.0 - ARENA_MIN) * capacity)
+    for c, limit in ((np.where(full, np.maximum(grown, loose), loose), roomy),
+                     (np.where(full, np.maximum(grown, bc), bc), capacity), (np.where(full, grown, caps), capacity),
+                     (bc, capacity), (np.where(demand > 0, round32(demand + PAD), 0), capacity), (demand, capacity)):
         c = np.maximum(c, np.minimum(demand, caps))
-        if int(c.sum()) <= capacity:
+        if int(c.sum()) <= limit:
             return c
     return None
 
@@ -380,7 +392,7 @@ class World:
         L.check(L.lib().rk_pending_overflow(self.h, ctypes.byref(n), _stream()))
         return n.value
 
-    def settle_overflow(self):
+    def settle_overflow(self, pending=None):
         """Give every bin that overflowed room, then re-append the pending
         records (synchronous; no particle is lost).
 
@@ -393,21 +405,23 @@ class World:
         fit and the device has room for a larger pool."""
         t0 = time.perf_counter()
         for _ in range(4):
-            if self.pending_overflow() == 0:
+            # `pending`: the caller's count, read just before (saves a sync)
+            if (self.pending_overflow() if pending is None else pending) == 0:
                 break
+            pending = None
             for s in range(self.nsp):
                 demand = self.count[s].cpu().numpy().view(np.uint32).astype(np.int64)
                 pool = self.pools[s]
                 caps = pool.cap_h
-                full = demand > caps
-                if not np.any(full):
+                idx = np.flatnonzero(demand > caps)
+                if idx.size == 0:
                     continue
-                grown = np.maximum(np.ceil(demand * SLACK).astype(np.int64) + PAD, (3 * caps + 1) // 2)
-                grown = round32(grown)
-                need = int(grown[full].sum())
-                if pool.used + need <= pool.capacity:
-                    self._move_bins(s, full, grown)
+                g = round32(np.maximum(np.ceil(demand[idx] * SLACK).astype(np.int64) + PAD, (3 * caps[idx] + 1) // 2))
+                if pool.used + int(g.sum()) <= pool.capacity:
+                    self._move_bins(s, idx, g)
                 else:
+                    grown = caps.copy()
+                    grown[idx] = g
                     self._relayout(s, demand, grown)
             rc = L.lib().rk_retry_append(self.h, _stream())
             if rc != L.RK_EOVERFLOW:  # overflow again: the next pass gives those bins room
@@ -419,13 +433,11 @@ class World:
         # the mailboxes keep their size (leavers past it spill into the shared
         # record sets); replacing them here would orphan pending records
 
-    def _move_bins(self, s, full, grown):
-        """Full bins of species s to new places in the pool's arena, with
-        capacity `grown`; the other bins stay.  Same buffers: captured graphs
+    def _move_bins(self, s, idx, g):
+        """Bins idx of species s to new places in the pool's arena, with
+        capacities g; the other bins stay.  Same buffers: captured graphs
         stay valid."""
         pool = self.pools[s]
-        idx = np.nonzero(full)[0]
-        g = grown[idx]
         new = pool.used + np.concatenate([[0], np.cumsum(g)[:-1]]).astype(np.int64)
         d_idx = torch.from_numpy(idx.astype(np.int32)).to(self.dev)
         d_new = torch.from_numpy(new).to(self.dev)

@@ -39,12 +39,39 @@ def test_plan_arena_respects_the_reserve_and_raises_when_bins_do_not_fit():
         plan_arena(used, free_bytes=1000 * SLOT_BYTES, reserve_bytes=1)
 
 
-def test_plan_relayout_grows_full_bins_and_keeps_the_rest():
+def test_plan_relayout_gives_every_bin_its_demand_plus_slack():
+    # with room to spare: demand * 1.5 + 64 per bin (multiples of 32), the
+    # full bin at least its grown capacity; the bin that emptied shrinks
     caps = np.array([64, 64, 64])
     demand = np.array([10, 80, 64])
     grown = np.array([0, 192, 0])
     c = plan_relayout(demand, caps, grown, capacity=1000)
-    assert list(c) == [64, 192, 64]
+    assert list(c) == [96, 192, 160]
+    assert list(c) == list(np.maximum(bin_caps(demand, 1.5), np.where(demand > caps, grown, 0)))
+    caps = np.array([512, 64])
+    demand = np.array([8, 100])
+    c = plan_relayout(demand, caps, np.array([0, 256]), capacity=1000)
+    assert list(c) == [96, 256]
+    # the looser slack only while it leaves 15 % of the pool spare: 1.5 gives
+    # 96 + 384 = 480 > 0.85 * 420, the set-up's 1.25 gives 96 + 320 = 416,
+    # which fits the pool
+    c = plan_relayout(np.array([8, 200]), np.array([512, 64]), np.array([0, 256]), capacity=420)
+    assert list(c) == [96, 320]
+    assert list(c) == list(np.maximum(bin_caps([8, 200]), [0, 256]))
+    demand = np.array([1000, 1000])
+    assert list(plan_relayout(demand, np.array([1600, 1600]), np.array([0, 0]), capacity=10**5)) == \
+        list(bin_caps(demand, 1.5))
+    assert list(plan_relayout(demand, np.array([1600, 1600]), np.array([0, 0]), capacity=3000)) == \
+        list(bin_caps(demand))
+
+
+def test_plan_relayout_keeps_capacities_when_the_fresh_layout_does_not_fit():
+    # demand + slack for all: 96 + 192 + 160 = 448 > 330; full bin grown and
+    # the others kept: 64 + 192 + 64 = 320 fits
+    caps = np.array([64, 64, 64])
+    demand = np.array([10, 80, 64])
+    grown = np.array([0, 192, 0])
+    assert list(plan_relayout(demand, caps, grown, capacity=330)) == [64, 192, 64]
 
 
 def test_plan_relayout_falls_back_to_tighter_capacities():
