@@ -1432,6 +1432,44 @@ struct NbrTab {
 // without a record and deposit nothing.
 // PUSHER: 0 Boris, 1 Higuera-Cary, 2 Faraday (the same device functions as
 // the per-tile push, so the fused step moves particles bit-identically).
+// The fused step's two rare paths, for a particle whose stencil leaves the
+// brick's LDS windows (a particle kept in a bin it has left): interpolation
+// from the world lattice and a deposit with global atomics.  They are kept out
+// of line, with scalar arguments and results by value: inlined, their
+// registers counted against the particle loop's and pushed its bin pointers
+// out of SGPRs into VGPR lanes (read back on every particle load and store).
+struct Fields6 {
+  float e0, e1, e2, b0, b1, b2;
+};
+
+__device__ __attribute__((noinline)) Fields6 fused_gather_global(const float* E0, const float* E1, const float* E2,
+                                                                const float* B0, const float* B1, const float* B2,
+                                                                int L0, int L1, int L2, int I, int J, int K, float d0,
+                                                                float d1, float d2, uint32_t* err)
+{
+  Geo g {};
+  g.L[0] = L0;
+  g.L[1] = L1;
+  g.L[2] = L2;
+  FieldPtrs f {{E0, E1, E2}, {B0, B1, B2}, {nullptr, nullptr, nullptr}};
+  const float d[3] = {d0, d1, d2};
+  float E[3], B[3];
+  gather(g, f, MeanWin {{0, 0, 0}, {0, 0, 0}}, nullptr, I, J, K, d, E, B, err);
+  return Fields6 {E[0], E[1], E[2], B[0], B[1], B[2]};
+}
+
+__device__ __attribute__((noinline)) void fused_deposit_global(float* J0, float* J1, float* J2, int L0, int L1, int L2,
+                                                              Zig z, int t0, int t1, int t2, uint32_t* err)
+{
+  Geo g {};
+  g.L[0] = L0;
+  g.L[1] = L1;
+  g.L[2] = L2;
+  FieldPtrs f {{nullptr, nullptr, nullptr}, {nullptr, nullptr, nullptr}, {J0, J1, J2}};
+  const int toffh[3] = {t0, t1, t2};
+  deposit_global(g, f, z, toffh, err);
+}
+
 template<int BR, int FAST, int WALL, int PUSHER, int WGS>
 __device__ __forceinline__ void fused_brick(const Geo& g, const SpParams& sp, const FieldPtrs& f, const MigPtrs& m,
                                             const MigPtrs* mtab, int b, uint32_t* counters, float* jstage,
@@ -1536,9 +1574,17 @@ __device__ __forceinline__ void fused_brick(const Geo& g, const SpParams& sp, co
           dd[d] = pr[d] - float(uint32_t(ci[d]));
         }
         float E[3], Bf[3];
-        if(!gather6(S, ldsM, ci[0] + moff[0], ci[1] + moff[1], ci[2] + moff[2], dd, E, Bf))
-          gather(g, f, MeanWin {{0, 0, 0}, {0, 0, 0}}, nullptr, Bk.toffh[0] + ci[0], Bk.toffh[1] + ci[1],
-                 Bk.toffh[2] + ci[2], dd, E, Bf, &counters[C_STENCIL]);
+        if(!gather6(S, ldsM, ci[0] + moff[0], ci[1] + moff[1], ci[2] + moff[2], dd, E, Bf)) {
+          const Fields6 eb = fused_gather_global(f.E[0], f.E[1], f.E[2], f.B[0], f.B[1], f.B[2], g.L[0], g.L[1], g.L[2],
+                                                 Bk.toffh[0] + ci[0], Bk.toffh[1] + ci[1], Bk.toffh[2] + ci[2], dd[0],
+                                                 dd[1], dd[2], &counters[C_STENCIL]);
+          E[0] = eb.e0;
+          E[1] = eb.e1;
+          E[2] = eb.e2;
+          Bf[0] = eb.b0;
+          Bf[1] = eb.b1;
+          Bf[2] = eb.b2;
+        }
         if constexpr(PUSHER == 1) higuera_cary<FAST>(g, qm, E, Bf, pos, vel);
         else if constexpr(PUSHER == 2) faraday<FAST>(g, qm, E, Bf, pos, vel);
         else boris<FAST>(g, qm, E, Bf, pos, vel);
@@ -1578,7 +1624,9 @@ __device__ __forceinline__ void fused_brick(const Geo& g, const SpParams& sp, co
         }
         // zigzag deposit in the frame of the tile that owns the particle after migration
         const Zig z = zigzag_setup<FAST>(g, x2, vel, qscaled, g.cfl);
-        if(!deposit_win(S, ldsJ, z, off)) deposit_global(g, f, unscaled(z, jinv_f), dtoffh, &counters[C_STENCIL]);
+        if(!deposit_win(S, ldsJ, z, off))
+          fused_deposit_global(f.J[0], f.J[1], f.J[2], g.L[0], g.L[1], g.L[2], unscaled(z, jinv_f), dtoffh[0],
+                               dtoffh[1], dtoffh[2], &counters[C_STENCIL]);
       }
       // leavers take consecutive slots of this brick's mailbox (rank from a
       // block scan: no global atomics); past its capacity they spill into
