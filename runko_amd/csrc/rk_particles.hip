@@ -291,6 +291,12 @@ __device__ inline bool fast_div_range(float x)
   return ax >= 0x1p-100f && ax <= 0x1p100f;
 }
 
+// The IEEE operations behind the fast forms below, kept out of line: with the
+// fast forms enabled they run only for inputs outside the verified ranges,
+// and inlined they would hold registers across the particle loops.
+__device__ __attribute__((noinline)) float ieee_div(float a, float b) { return a / b; }
+__device__ __attribute__((noinline)) float ieee_div_sqrt(float a, float x) { return a / sqrtf(x); }
+
 // FAST: 0 = read the world's flags (fast_div, fast_sqrt); -1 = IEEE forms;
 // 1 = reciprocal x/cfl only; 2 = reciprocal x/cfl and the sqrt/rcp forms
 template<int FAST = 0>
@@ -301,7 +307,7 @@ __device__ inline float div_cfl(const Geo& g, float x)
     const float r = fmaf(-q, g.cfl, x);
     return fmaf(r, g.rcp_cfl, q);
   }
-  return x / g.cfl;
+  return FAST > 0 ? ieee_div(x, g.cfl) : x / g.cfl;
 }
 
 // Correctly rounded sqrt for 2^-60 <= x <= 2^60 from the hardware estimate
@@ -336,7 +342,7 @@ __device__ inline float cfl_over_sqrt(const Geo& g, float X)
 {
   if((FAST > 1 || (FAST == 0 && g.fast_sqrt)) && X >= g.cfl * g.cfl && X <= 0x1p60f)
     return div_near(g.cfl, sqrt_near(X));
-  return g.cfl / sqrtf(X);
+  return FAST > 1 ? ieee_div_sqrt(g.cfl, X) : g.cfl / sqrtf(X);
 }
 
 // c / t for c in {1, 2} and t = 1 + v.v (particle_boris.h:47, zigzag 1/sqrt)
@@ -344,7 +350,7 @@ template<int FAST = 0>
 __device__ inline float two_over(const Geo& g, float t)
 {
   if((FAST > 1 || (FAST == 0 && g.fast_sqrt)) && t >= 1.0f && t <= 0x1p60f) return div_near(2.0f, t);
-  return 2.0f / t;
+  return FAST > 1 ? ieee_div(2.0f, t) : 2.0f / t;
 }
 
 // 1 / t for t = 1 + b.b (particle_faraday.h)
@@ -352,14 +358,14 @@ template<int FAST = 0>
 __device__ inline float one_over(const Geo& g, float t)
 {
   if((FAST > 1 || (FAST == 0 && g.fast_sqrt)) && t >= 1.0f && t <= 0x1p60f) return div_near(1.0f, t);
-  return 1.0f / t;
+  return FAST > 1 ? ieee_div(1.0f, t) : 1.0f / t;
 }
 
 template<int FAST = 0>
 __device__ inline float inv_sqrt(const Geo& g, float t)
 {
   if((FAST > 1 || (FAST == 0 && g.fast_sqrt)) && t >= 1.0f && t <= 0x1p60f) return div_near(1.0f, sqrt_near(t));
-  return 1.0f / sqrtf(t);
+  return FAST > 1 ? ieee_div_sqrt(1.0f, t) : 1.0f / sqrtf(t);
 }
 
 // particle_boris.h:39-59, expression order as written there
@@ -1432,6 +1438,16 @@ struct NbrTab {
 // without a record and deposit nothing.
 // PUSHER: 0 Boris, 1 Higuera-Cary, 2 Faraday (the same device functions as
 // the per-tile push, so the fused step moves particles bit-identically).
+// a leaver's record in a shared record set, when its brick's mailbox is full
+// (rare; out of line for the same reason as the two paths below)
+__device__ __attribute__((noinline)) void fused_spill_record(const MigPtrs* mtab, int set, uint32_t slot, float p0,
+                                                            float p1, float p2, float v0, float v1, float v2,
+                                                            uint64_t id, uint32_t meta)
+{
+  const float p[3] = {p0, p1, p2}, v[3] = {v0, v1, v2};
+  write_record(*mtab, set, slot, p, v, id, meta);
+}
+
 // The fused step's two rare paths, for a particle whose stencil leaves the
 // brick's LDS windows (a particle kept in a bin it has left): interpolation
 // from the world lattice and a deposit with global atomics.  They are kept out
@@ -1663,7 +1679,9 @@ __device__ __forceinline__ void fused_brick(const Geo& g, const SpParams& sp, co
         // records keep the unwrapped position; the append applies the wrap
         const uint32_t meta = lv0 ? meta_pack(s, 0, dir, uint32_t(b)) : meta_pack(s, 1, 13, uint32_t(b));
         if(!spill) write_record(m, 2, uint32_t(int64_t(b) * m.per_brick + local), pos, vel, id, meta);
-        else write_record(*mtab, lv0 ? 0 : 1, lv0 ? slot0 : slot1, pos, vel, id, meta);
+        else
+          fused_spill_record(mtab, lv0 ? 0 : 1, lv0 ? slot0 : slot1, pos[0], pos[1], pos[2], vel[0], vel[1], vel[2], id,
+                             meta);
       }
       if(stays) {
         const uint32_t d = uint32_t(dst);
