@@ -254,7 +254,10 @@ def run_turbulence_long(args):
         e = w.energies()
         err = [int(v) for v in w.error_counters()]
         alive = w.total_particles()
-        rec = dict(lap=lap, wall=wall, field_E=float(e[0]), field_B=float(e[1]),
+        # total energy in code units: sum(E^2 + B^2)/2 + sum_s m_s |q_s| c^2
+        # sum_p (gamma_p - 1) (projects/pic-turbulence/pic.py:52-53 macro-masses)
+        total = float(e[0] + e[1] + sum(m * abs(q) * w.cfl ** 2 * k for m, q, k in zip(w.masses, w.charges, e[2:])))
+        rec = dict(lap=lap, wall=wall, field_E=float(e[0]), field_B=float(e[1]), total=total,
                    kinetic=[float(v) for v in e[2:]], particles=alive, errors=err,
                    bin_moves=w.bin_moves, relayouts=w.relayouts, recaptures=gl.recaptures)
         if lap % 250 == 0 or lap == args.long_run:
@@ -263,7 +266,7 @@ def run_turbulence_long(args):
         torch.cuda.synchronize()
         diag[0] += time.perf_counter() - td
         hist.append(rec)
-        log(f"[long] lap {lap}: field energy B {rec['field_B']:.6e}, E {rec['field_E']:.6e}; kinetic "
+        log(f"[long] lap {lap}: total energy {total:.9e}; field B {rec['field_B']:.6e}, E {rec['field_E']:.6e}; kinetic "
             f"{', '.join(f'{k:.6e}' for k in rec['kinetic'])}; particles {alive}, errors {err}, "
             f"bin moves {w.bin_moves}, relayouts {w.relayouts}, {wall:.1f}s")
         return alive == npart and err == [0, 0]
@@ -289,7 +292,10 @@ def run_turbulence_long(args):
                         f"theta 0.3, sigma 10, 8x8 dB modes, batched lap from HIP graphs",
                laps=laps, ok=ok, particles=npart, wall_s=wall, ms_per_lap=1e3 * wall / max(laps, 1),
                settle_ms_per_lap=1e3 * (w.settle_seconds - settle_s) / max(laps, 1),
-               diagnostics_s=diag[0], bin_moves=w.bin_moves, relayouts=w.relayouts, history=hist)
+               diagnostics_s=diag[0], bin_moves=w.bin_moves, relayouts=w.relayouts,
+               total_energy_change=(hist[-1]["total"] - hist[0]["total"]) / hist[0]["total"],
+               total_energy_max_deviation=max(abs(r["total"] - hist[0]["total"]) for r in hist) / hist[0]["total"],
+               history=hist)
     print(json.dumps(out), flush=True)
     return 0 if ok else 1
 

@@ -1170,8 +1170,16 @@ __global__ void __launch_bounds__(WG)
       RecordDest R;
       int key = -1;
       uint32_t rank = 0;
+      // the record's payload is loaded with its destination, so its latency
+      // overlaps the ranking and the reservation
+      float rv[3] = {0.0f, 0.0f, 0.0f};
+      uint64_t rid = 0;
       if(e < n) {
-        R = record_dest(g, m, 2, base + uint32_t(e));
+        const uint32_t r = base + uint32_t(e);
+#pragma unroll
+        for(int d = 0; d < 3; ++d) rv[d] = m.vel[2][d][r];
+        rid = m.id[2][r];
+        R = record_dest(g, m, 2, r);
         int off = 0;
         bool near = true;
 #pragma unroll
@@ -1195,7 +1203,18 @@ __global__ void __launch_bounds__(WG)
       __syncthreads();
       if(e < n) {
         const uint32_t slot = key >= 0 ? kbase[key] + rank : atomicAdd(&sp.count[R.s][R.db], 1u);
-        place_record(sp, m, 2, base + uint32_t(e), R, slot, counters, ovf, ovf_cap);
+        const uint32_t cap = sp.cap[R.s][R.db];
+        const int64_t q = sp.start[R.s][R.db] + slot;
+        if(slot < cap) {
+#pragma unroll
+          for(int d = 0; d < 3; ++d) {
+            sp.pos[R.s][d][q] = R.p[d];
+            sp.vel[R.s][d][q] = rv[d];
+          }
+          sp.id[R.s][q] = rid;
+        } else {
+          place_record(sp, m, 2, base + uint32_t(e), R, slot, counters, ovf, ovf_cap);  // overflow path
+        }
       }
       __syncthreads();
     }
@@ -2213,7 +2232,9 @@ int launch_append_mailbox(rk_world* w, hipStream_t s)
 
 int launch_append_records(rk_world* w, hipStream_t s)
 {
-  hipLaunchKernelGGL(k_append_records, dim3(512, 2), dim3(WG), 0, s, w->g, make_sp(w), make_mig(w), w->counters,
+  // the sets hold only what spilled past full mailboxes (usually nothing):
+  // a small grid, since every workgroup signs off with a fence and an atomic
+  hipLaunchKernelGGL(k_append_records, dim3(64, 2), dim3(WG), 0, s, w->g, make_sp(w), make_mig(w), w->counters,
                      w->ovf_list, w->ovf_cap);
   RK_HIP(hipGetLastError());
   return RK_OK;
