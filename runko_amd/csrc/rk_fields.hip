@@ -593,11 +593,44 @@ __global__ void __launch_bounds__(256) k_ghost_fold(Geo g, float* F0, float* F1,
   int ii[3], jj[3], kk[3];
   const int mi = images(i, g.N[0], ii);
   const int mj = images(j, g.N[1], jj);
+  // with N >= 2*HALO per axis a node has at most two images per axis: all of
+  // its (up to 7) ghost images are addressed first and loaded together, then
+  // added in the order of the general loop below (bit-identical)
+  const bool two = g.N[0] >= 2 * HALO && g.N[1] >= 2 * HALO && g.N[2] >= 2 * HALO;
   auto fold = [&](int k) {
     const int mk = images(k, g.N[2], kk);
     if(mi * mj * mk == 1) return;
     const uint32_t c = g.at(i, j, k);
     float s0 = F0[c], s1 = F1[c], s2 = F2[c];
+    if(two) {
+      float v[8][3];
+#pragma unroll
+      for(int a = 0; a < 2; ++a)
+#pragma unroll
+        for(int b = 0; b < 2; ++b)
+#pragma unroll
+          for(int e = 0; e < 2; ++e) {
+            const bool ok = a < mi && b < mj && e < mk && (a | b | e) != 0;
+            const uint32_t q = ok ? g.at(ii[a], jj[b], kk[e]) : c;
+            const int n = (a * 2 + b) * 2 + e;
+            v[n][0] = F0[q];
+            v[n][1] = F1[q];
+            v[n][2] = F2[q];
+          }
+#pragma unroll
+      for(int n = 1; n < 8; ++n) {
+        const int a = n >> 2, b = (n >> 1) & 1, e = n & 1;
+        const bool ok = a < mi && b < mj && e < mk;
+        const float t0 = s0 + v[n][0], t1 = s1 + v[n][1], t2 = s2 + v[n][2];
+        s0 = ok ? t0 : s0;
+        s1 = ok ? t1 : s1;
+        s2 = ok ? t2 : s2;
+      }
+      F0[c] = s0;
+      F1[c] = s1;
+      F2[c] = s2;
+      return;
+    }
     for(int a = 0; a < mi; ++a)
       for(int b = 0; b < mj; ++b)
         for(int e = 0; e < mk; ++e) {
