@@ -300,6 +300,82 @@ def run_turbulence_long(args):
     return 0 if ok else 1
 
 
+def run_shock_long(args):
+    """Configuration 3 at its stated length: the reflecting-wall shock of
+    run_shock for args.long_shock laps of the fused lap replayed from HIP
+    graphs, the moving injector every 50 laps (between replays) and the
+    overflow settle after every lap.  Every 100 laps (and at the end): the
+    particle count against the injected total, the error counters and the
+    energies, timed apart from the laps; the run stops at the first error."""
+    import numpy as np
+    import torch
+
+    from runko_amd.init import WorldInjector, shock_world
+    from runko_amd.stepper import GraphLap, ShockLap
+
+    nx, ny, nz = args.shock_cells
+    injloc = args.shock_injloc if args.shock_injloc > 0 else None
+    t0 = time.perf_counter()
+    w, walls, bcs, par = shock_world(nx=nx, ny=ny, nz=nz, tile=args.tile, ppc=8, theta=1e-4, gamma_flow=10.0,
+                                     sigma=0.1, seed=3, injloc=injloc)
+    n0 = w.total_particles()
+    log(f"[long-shock] {nx}x{ny}x{nz}, {n0} particles, set-up {time.perf_counter() - t0:.1f}s")
+    beta = (1.0 - 1.0 / 100.0) ** 0.5
+    inj = None
+    if injloc is not None:
+        inj = WorldInjector(w, injloc=injloc, beta_flow=beta, ppc=8, theta=1e-4, gamma_flow=10.0, walloc=15.0,
+                            n_inj=50, Lx=float(nx))
+    fused = ShockLap(w, walls, bcs)
+    fused()
+    fused.settle()
+    lap = GraphLap(w, lap=fused)
+    hist, diag = [], [0.0]
+
+    def sample(i, wall):
+        td = time.perf_counter()
+        torch.cuda.synchronize()
+        e = w.energies()
+        err = [int(v) for v in w.error_counters()]
+        alive = w.total_particles()
+        injected = inj.injected if inj else 0
+        rec = dict(lap=i, wall=wall, particles=alive, injected=injected, errors=err, field_E=float(e[0]),
+                   field_B=float(e[1]), kinetic=[float(v) for v in e[2:]], bin_moves=w.bin_moves,
+                   relayouts=w.relayouts, new_pools=w.new_pools, recaptures=lap.recaptures,
+                   front=float(inj.injloc) if inj else None)
+        hist.append(rec)
+        diag[0] += time.perf_counter() - td
+        log(f"[long-shock] lap {i}: particles {alive} (+{injected} injected), errors {err}, "
+            f"bin moves {w.bin_moves}, relayouts {w.relayouts}, new pools {w.new_pools}, "
+            f"recaptures {lap.recaptures}, {wall:.1f}s")
+        return alive == n0 + injected and err == [0, 0]
+
+    ok = sample(0, 0.0)
+    diag[0] = 0.0
+    settle0 = w.settle_seconds
+    tl = time.perf_counter()
+    laps = 0
+    for i in range(1, args.long_shock + 1):
+        lap()
+        lap.settle()
+        if inj is not None:
+            inj.inject(i)
+        laps = i
+        if i % 100 == 0 or i == args.long_shock:
+            if not sample(i, time.perf_counter() - tl):
+                ok = False
+                break
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - tl - diag[0]
+    nm = np.mean([r["particles"] for r in hist])
+    out = dict(workload=f"reflecting-wall shock, {nx}x{ny}x{nz} cells in {args.tile}^3 tiles, fused lap from HIP "
+                        f"graphs, moving injector every 50 laps", laps=laps, ok=ok, particles_start=n0,
+               particles_end=hist[-1]["particles"], wall_s=wall, ms_per_lap=1e3 * wall / max(laps, 1),
+               particle_steps_per_s=nm * laps / wall, settle_ms_per_lap=1e3 * (w.settle_seconds - settle0) / max(laps, 1),
+               recaptures=lap.recaptures, diagnostics_s=diag[0], history=hist)
+    print(json.dumps(out), flush=True)
+    return 0 if ok else 1
+
+
 def config0_inputs(seed=42):
     """64^3 cells, 4^3 tiles of 16^3, 8 ppc/species, theta 0.3, sigma 1, E = 0,
     B = B0 z (projects/pic-turbulence/pic.py conventions)."""
@@ -805,6 +881,8 @@ def main():
     ap.add_argument("--ensemble-every", type=int, default=0,
                     help="flat checkpoint spacing of the ensemble rollout (0: chosen from free memory, nested if "
                          "flat does not fit)")
+    ap.add_argument("--long-shock", type=int, default=0, metavar="LAPS",
+                    help="only the reflecting-wall shock for LAPS graph laps with the injector and diagnostics")
     ap.add_argument("--long-run", type=int, default=0, metavar="LAPS",
                     help="only the 512^3 turbulence world for LAPS graph laps with diagnostics every 50 laps")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
@@ -831,6 +909,8 @@ def main():
         return
     if args.long_run:
         sys.exit(run_turbulence_long(args))
+    if args.long_shock:
+        sys.exit(run_shock_long(args))
     if args.copy_probe:
         run_copy_probe(args.copy_probe)
     res = run_turbulence(args)

@@ -97,6 +97,7 @@ def plan_arena(used, free_bytes, reserve_bytes, target=ARENA_TARGET):
 
 RELAYOUT_SLACK = 1.5    # bin capacity per particle after a relayout, if the arena keeps ARENA_MIN
 ARENA_MIN = 0.15        # fraction of the pool left spare by that first choice
+ARENA_LOW = 0.05        # a relayout leaving less spare grows the pool, if the device has room
 
 
 def plan_relayout(demand, caps, grown, capacity):
@@ -200,6 +201,10 @@ class World:
         self.bin_moves = 0  # repacks done inside the arena (no new pool)
         self.relayouts = 0  # in-place relayouts of a pool (arena used up)
         self.new_pools = 0  # pools replaced by larger ones
+        # a relayout that would leave less than ARENA_LOW of its pool spare
+        # allocates a larger pool instead when the device has room (False:
+        # always relayout in place, so buffer addresses never change)
+        self.grow_pools = True
         self.settle_seconds = 0.0  # wall time spent repacking and re-appending
         self.count = [torch.zeros((self.nbricks,), dtype=torch.int32, device=self.dev) for _ in range(self.nsp)]
         self.pools = [_Pool(self.nbricks, np.zeros(self.nbricks, dtype=np.int64), self.dev) for _ in range(self.nsp)]
@@ -453,13 +458,20 @@ class World:
 
     def _relayout(self, s, demand, grown):
         """Relays out species s in place with every bin that overflowed grown
-        (plan_relayout); falls back to a larger pool only if the relayout
-        cannot hold the demand and the device has room for one."""
+        (plan_relayout).  A larger pool replaces it when the relayout cannot
+        hold the demand, or would leave less than ARENA_LOW of the pool spare
+        (relayouts would then follow one another every few laps) and the
+        device has room for one."""
         pool = self.pools[s]
         caps = plan_relayout(demand, pool.cap_h, grown, pool.capacity)
         if caps is None:
             self._grow_pool(s, demand)
             return
+        if self.grow_pools and pool.capacity - int(caps.sum()) < ARENA_LOW * pool.capacity:
+            plan = self._plan_grow(demand)
+            if plan is not None:
+                self._repack(s, *plan)
+                return
         free = free_device_bytes(self.dev)
         stage_bytes = int(min(getattr(self, "stage_bytes", STAGE_MAX), max(free // 2, 1 << 20)))
         stage = torch.empty((stage_bytes,), dtype=torch.uint8, device=self.dev)
@@ -473,21 +485,30 @@ class World:
         self.repacks += 1
         self.relayouts += 1
 
-    def _grow_pool(self, s, demand):
-        """A larger pool for species s: only when the device can hold it
-        beside the old one and the reserve."""
+    def _plan_grow(self, demand):
+        """(bin capacities, arena) of a new pool for `demand`, or None when the
+        device cannot hold it beside the old one and the reserve."""
         caps = bin_caps(demand)
         used = int(caps.sum())
         _, total = torch.cuda.mem_get_info(self.dev)
         free = free_device_bytes(self.dev)
         reserve = max(RESERVE_MIN, int(RESERVE_FRAC * total))
-        arena = plan_arena([used], free, reserve)[0] if free > reserve + used * SLOT_BYTES else -1
-        if arena < 0:
+        if free <= reserve + used * SLOT_BYTES:
+            return None
+        return caps, plan_arena([used], free, reserve)[0]
+
+    def _grow_pool(self, s, demand):
+        """A larger pool for species s: only when the device can hold it
+        beside the old one and the reserve."""
+        plan = self._plan_grow(demand)
+        if plan is None:
+            caps = bin_caps(demand)
+            free = free_device_bytes(self.dev)
             raise RuntimeError(f"runko_amd: species {s} holds {int(demand.sum())} particles, more than its pool's "
                                f"{self.pools[s].capacity} slots, and the device has {free / 2**30:.1f} GiB free "
-                               f"(a larger pool needs {used * SLOT_BYTES / 2**30:.1f} GiB beside the "
-                               f"{reserve / 2**30:.1f} GiB reserve)")
-        self._repack(s, caps, arena)
+                               f"(a larger pool needs {int(caps.sum()) * SLOT_BYTES / 2**30:.1f} GiB beside the "
+                               f"reserve)")
+        self._repack(s, *plan)
 
     def _repack(self, s, newcaps, arena=None):
         newp = _Pool(self.nbricks, newcaps, self.dev, arena)

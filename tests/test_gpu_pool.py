@@ -98,6 +98,7 @@ def test_overflow_paths_keep_buffers_and_particles(reference, arena, stage):
     nb = (SETUP["N"] // 8) ** 3
     slots = {0: 0, "small": nb * per_bin // 20, "large": 4 * nb * per_bin}[arena]
     w, _ = turbulence_world(**SETUP, layout=dict(slack=1.0, pad=0, arena=slots))
+    w.grow_pools = False  # the in-place paths: no pool may be replaced
     if stage == "small":
         # tables (28 B per bin) plus room for about three bins: many batches per pass
         w.stage_bytes = 28 * w.nbricks + 256 + 3 * 32 * (per_bin + 512)
@@ -123,6 +124,7 @@ def test_overflow_under_graph_replay(reference):
     per_bin = 8 ** 3 * SETUP["ppc"]
     nb = (SETUP["N"] // 8) ** 3
     w, _ = turbulence_world(**SETUP, layout=dict(slack=1.0, pad=0, arena=nb * per_bin // 20))
+    w.grow_pools = False  # the in-place paths: no pool may be replaced
     n0 = w.total_particles()
     addr = _addresses(w)
     lap = _run(w, 4, graph=True)
@@ -132,4 +134,16 @@ def test_overflow_under_graph_replay(reference):
     assert lap.recaptures == 0
     assert w.total_particles() == n0
     _check_bins(w)
+    _compare(reference, w)
+
+
+def test_tight_relayout_grows_the_pool(reference):
+    # no arena: the first relayout would leave under 5 % of the pool spare,
+    # so with room on the device the species gets a larger pool instead
+    w, _ = turbulence_world(**SETUP, layout=dict(slack=1.0, pad=0, arena=0))
+    n0 = w.total_particles()
+    _run(w, 4)
+    print(f"new pools {w.new_pools}, relayouts {w.relayouts}, moves {w.bin_moves}")
+    assert w.new_pools > 0
+    assert w.total_particles() == n0
     _compare(reference, w)
