@@ -1330,11 +1330,11 @@ __device__ inline SegVals segment_values(float Fx, float Fy, float Fz, float Wx,
 // and order-free (the window is bit-reproducible), and ds_add_u64 runs about
 // twice as fast as ds_add_f64 on gfx950.  The rounding unit is 2^-30 max|q|,
 // far below the float rounding of a node's current.
-__device__ inline uint64_t to_fixed(float v)
+__device__ inline int fixed_of(float v)
 {
   int r;
   asm("v_cvt_rpi_i32_f32 %0, %1" : "=v"(r) : "v"(v));
-  return uint64_t(int64_t(r));
+  return r;
 }
 
 __device__ inline void lds_add_u64(uint64_t* p, uint64_t v)
@@ -1342,34 +1342,42 @@ __device__ inline void lds_add_u64(uint64_t* p, uint64_t v)
   __hip_atomic_fetch_add(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
-// add the 12 contributions of one zigzag segment at window node base p
+// add 12 fixed-point contributions (the node order of segment_values) at
+// window node base p
 template<int BR>
-__device__ inline void emit_window(const Shape<BR>& S, uint64_t* p, const SegVals& a)
+__device__ inline void emit_fixed(const Shape<BR>& S, uint64_t* p, const int (&r)[12])
 {
   const int sx = S.D(1) * S.D(2), sy = S.D(2), V = S.DV();
   uint64_t* px = p;
   uint64_t* py = p + V;
   uint64_t* pz = p + 2 * V;
-  lds_add_u64(px, to_fixed(a.v[0]));
-  lds_add_u64(py, to_fixed(a.v[1]));
-  lds_add_u64(pz, to_fixed(a.v[2]));
-  lds_add_u64(py + sx, to_fixed(a.v[3]));
-  lds_add_u64(pz + sx, to_fixed(a.v[4]));
-  lds_add_u64(px + sy, to_fixed(a.v[5]));
-  lds_add_u64(pz + sy, to_fixed(a.v[6]));
-  lds_add_u64(px + 1, to_fixed(a.v[7]));
-  lds_add_u64(py + 1, to_fixed(a.v[8]));
-  lds_add_u64(px + sy + 1, to_fixed(a.v[9]));
-  lds_add_u64(py + sx + 1, to_fixed(a.v[10]));
-  lds_add_u64(pz + sx + sy, to_fixed(a.v[11]));
+  auto x = [](int v) { return uint64_t(int64_t(v)); };
+  lds_add_u64(px, x(r[0]));
+  lds_add_u64(py, x(r[1]));
+  lds_add_u64(pz, x(r[2]));
+  lds_add_u64(py + sx, x(r[3]));
+  lds_add_u64(pz + sx, x(r[4]));
+  lds_add_u64(px + sy, x(r[5]));
+  lds_add_u64(pz + sy, x(r[6]));
+  lds_add_u64(px + 1, x(r[7]));
+  lds_add_u64(py + 1, x(r[8]));
+  lds_add_u64(px + sy + 1, x(r[9]));
+  lds_add_u64(py + sx + 1, x(r[10]));
+  lds_add_u64(pz + sx + sy, x(r[11]));
 }
 
 // Deposit into the brick's current window; z carries the scaled charge.
 // off[d] maps a tile-haloed node index of the owning tile to a window index
 // (periodic image included).  False when a node falls outside the window
-// (the caller then uses HBM).  Both segments are added as the reference's
-// atomic deposit adds them, one after the other (the second at the same
-// nodes when the particle stays in its cell).
+// (the caller then uses HBM).  Each segment's 12 contributions are rounded to
+// fixed point as the reference's atomic deposit adds them, one segment after
+// the other.  A particle that stays in its cell adds both segments at the
+// same 12 nodes: their integer sums go in with one set of atomics, which
+// leaves every window sum unchanged (integer adds are exact and order-free).
+// The sums cannot overflow: both segments then move the same way along each
+// axis, so |F1| + |F2| = |q| |x2 - x1| < |q| on every axis, every weight lies
+// in [0, 1], and each contribution is at most 2^30 (max|q| * jscale) before
+// rounding to an integer.
 template<int BR>
 __device__ inline bool deposit_win(const Shape<BR>& S, uint64_t* ldsJ, const Zig& z, const int off[3])
 {
@@ -1382,10 +1390,21 @@ __device__ inline bool deposit_win(const Shape<BR>& S, uint64_t* ldsJ, const Zig
     in = in && a[d] >= 0 && b[d] >= 0 && a[d] + 1 < S.D(d) && b[d] + 1 < S.D(d);
   }
   if(!in) return false;
-  emit_window(S, ldsJ + (a[0] * S.D(1) + a[1]) * S.D(2) + a[2],
-              segment_values(z.Fx1, z.Fy1, z.Fz1, z.Wx1, z.Wy1, z.Wz1));
-  emit_window(S, ldsJ + (b[0] * S.D(1) + b[1]) * S.D(2) + b[2],
-              segment_values(z.Fx2, z.Fy2, z.Fz2, z.Wx2, z.Wy2, z.Wz2));
+  const SegVals v1 = segment_values(z.Fx1, z.Fy1, z.Fz1, z.Wx1, z.Wy1, z.Wz1);
+  const SegVals v2 = segment_values(z.Fx2, z.Fy2, z.Fz2, z.Wx2, z.Wy2, z.Wz2);
+  int r1[12], r2[12];
+#pragma unroll
+  for(int n = 0; n < 12; ++n) {
+    r1[n] = fixed_of(v1.v[n]);
+    r2[n] = fixed_of(v2.v[n]);
+  }
+  const bool same = a[0] == b[0] && a[1] == b[1] && a[2] == b[2];
+  if(same) {
+#pragma unroll
+    for(int n = 0; n < 12; ++n) r1[n] += r2[n];
+  }
+  emit_fixed(S, ldsJ + (a[0] * S.D(1) + a[1]) * S.D(2) + a[2], r1);
+  if(!same) emit_fixed(S, ldsJ + (b[0] * S.D(1) + b[1]) * S.D(2) + b[2], r2);
   return true;
 }
 
