@@ -282,9 +282,10 @@ int rk_move_bins(rk_world* w, int32_t s, int32_t n, const int32_t* bins, const i
 /* In-place relayout of species s (synchronous): every bin b gets capacity
  * new_cap[b] (host uint32[n_bricks], >= its stored count), bins packed in the
  * order of their current starts; *new_start (host int64[n_bricks]) receives
- * the new starts.  tmp is a caller device buffer of tmp_bytes (at least the
- * tables, 28 B per brick, plus 32 B per particle of the largest bin; more
- * means fewer passes).  RK_EOVERFLOW if the capacities exceed the pool.
+ * the new starts; every bin is copied once, through tmp.  tmp is a caller
+ * device buffer of tmp_bytes (at least the tables, 28 B per brick, plus 32 B
+ * per particle of the largest bin; more means fewer batches).  RK_EOVERFLOW
+ * if the capacities exceed the pool.
  * Also the in-place counterpart of particle.h:471-509: no second pool. */
 int rk_relayout(rk_world* w, int32_t s, const uint32_t* new_cap, void* tmp, int64_t tmp_bytes,
                 int64_t* new_start, void* stream);

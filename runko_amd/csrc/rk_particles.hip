@@ -2457,13 +2457,17 @@ int launch_move_bins(rk_world* w, int s, int n, const int32_t* bins, const int64
 }
 
 // In-place relayout of species s (synchronous).  Bins are taken in the order
-// of their current starts.  Pass 1 packs them tightly (bin i at t_i = sum of
-// the stored counts before it, t_i <= its old start); pass 2 spreads them to
-// f_i = sum of the new capacities before it (f_i >= t_i).  Each pass moves
-// batches of whole bins through the staging buffer: pass 1 front to back
-// (a batch's target lies below every source not yet read), pass 2 back to
-// front (a batch's target lies above every source not yet read), so no
-// particle is overwritten before it has been copied.
+// of their current starts s_i; bin i goes to f_i = sum of the new capacities
+// before it, with its n_i stored particles.  One pass moves each bin once:
+// first the bins with f_i < s_i in ascending order, then those with f_i > s_i
+// in descending order (a bin with f_i = s_i stays).  No bin lands on the old
+// place of a bin not yet moved: a bin moving towards the front ends below
+// s_i + n_i <= s_j for every later bin j, and lies above s_j + n_j < f_j + n_j
+// <= f_i for every earlier bin j that moves towards the end; a bin moving
+// towards the end starts above s_i >= s_j + n_j for every earlier bin j; the
+// new places are disjoint.  Batches of whole bins go through the staging
+// buffer (gathered, then scattered), so a bin's old and new places may
+// overlap.
 int relayout_species(rk_world* w, int s, const uint32_t* new_cap, void* tmp, int64_t tmp_bytes, int64_t* new_start,
                      hipStream_t st)
 {
@@ -2479,10 +2483,10 @@ int relayout_species(rk_world* w, int s, const uint32_t* new_cap, void* tmp, int
   std::vector<int32_t> order(nbs);
   for(int b = 0; b < nb; ++b) order[size_t(b)] = b;
   std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return start[size_t(a)] < start[size_t(b)]; });
-  // sorted tables: old start, tight start, final start, stored count
-  std::vector<int64_t> so(nbs), to(nbs), fo(nbs);
+  // sorted tables: old start, final start, stored count
+  std::vector<int64_t> so(nbs), fo(nbs);
   std::vector<uint32_t> n(nbs);
-  int64_t t = 0, f = 0, end = 0;
+  int64_t f = 0, end = 0;
   uint32_t nmax = 0;
   for(int i = 0; i < nb; ++i) {
     const int b = order[size_t(i)];
@@ -2498,10 +2502,8 @@ int relayout_species(rk_world* w, int s, const uint32_t* new_cap, void* tmp, int
     }
     end = std::max(end, start[size_t(b)] + int64_t(cap[size_t(b)]));
     so[size_t(i)] = start[size_t(b)];
-    to[size_t(i)] = t;
     fo[size_t(i)] = f;
     n[size_t(i)] = stored;
-    t += stored;
     f += new_cap[b];
     nmax = std::max(nmax, stored);
   }
@@ -2510,6 +2512,23 @@ int relayout_species(rk_world* w, int s, const uint32_t* new_cap, void* tmp, int
               std::to_string(p.capacity));
     return RK_EOVERFLOW;
   }
+  // the moves in processing order: old start, new start, staging offset, count
+  std::vector<int64_t> mso, mfo, mst;
+  std::vector<uint32_t> mn;
+  int64_t st_off = 0;
+  auto add_move = [&](int i) {
+    if(n[size_t(i)] == 0 || so[size_t(i)] == fo[size_t(i)]) return;
+    mso.push_back(so[size_t(i)]);
+    mfo.push_back(fo[size_t(i)]);
+    mst.push_back(st_off);
+    mn.push_back(n[size_t(i)]);
+    st_off += n[size_t(i)];
+  };
+  for(int i = 0; i < nb; ++i)
+    if(fo[size_t(i)] < so[size_t(i)]) add_move(i);
+  for(int i = nb - 1; i >= 0; --i)
+    if(fo[size_t(i)] > so[size_t(i)]) add_move(i);
+  const int nm = int(mn.size());
   // staging: the three offset tables, the count table, then [7][W] particles
   const int64_t table = int64_t(nb) * (3 * sizeof(int64_t) + sizeof(uint32_t));
   const int64_t tab_al = (table + 255) / 256 * 256;
@@ -2520,50 +2539,35 @@ int relayout_species(rk_world* w, int s, const uint32_t* new_cap, void* tmp, int
   }
   char* base = static_cast<char*>(tmp);
   int64_t* d_so = reinterpret_cast<int64_t*>(base);
-  int64_t* d_to = d_so + nb;
-  int64_t* d_fo = d_to + nb;
-  uint32_t* d_n = reinterpret_cast<uint32_t*>(d_fo + nb);
-  RK_HIP(hipMemcpyAsync(d_so, so.data(), sizeof(int64_t) * size_t(nb), hipMemcpyHostToDevice, st));
-  RK_HIP(hipMemcpyAsync(d_to, to.data(), sizeof(int64_t) * size_t(nb), hipMemcpyHostToDevice, st));
-  RK_HIP(hipMemcpyAsync(d_fo, fo.data(), sizeof(int64_t) * size_t(nb), hipMemcpyHostToDevice, st));
-  RK_HIP(hipMemcpyAsync(d_n, n.data(), sizeof(uint32_t) * size_t(nb), hipMemcpyHostToDevice, st));
+  int64_t* d_fo = d_so + nb;
+  int64_t* d_st = d_fo + nb;
+  uint32_t* d_n = reinterpret_cast<uint32_t*>(d_st + nb);
+  if(nm > 0) {
+    RK_HIP(hipMemcpyAsync(d_so, mso.data(), sizeof(int64_t) * size_t(nm), hipMemcpyHostToDevice, st));
+    RK_HIP(hipMemcpyAsync(d_fo, mfo.data(), sizeof(int64_t) * size_t(nm), hipMemcpyHostToDevice, st));
+    RK_HIP(hipMemcpyAsync(d_st, mst.data(), sizeof(int64_t) * size_t(nm), hipMemcpyHostToDevice, st));
+    RK_HIP(hipMemcpyAsync(d_n, mn.data(), sizeof(uint32_t) * size_t(nm), hipMemcpyHostToDevice, st));
+  }
   PoolPtrs stage {};
   float* fbase = reinterpret_cast<float*>(base + tab_al);
   for(int d = 0; d < 6; ++d) stage.f[d] = fbase + int64_t(d) * W;
   stage.id = reinterpret_cast<uint64_t*>(fbase + 6 * W);
   const PoolPtrs pool = pool_ptrs(p);
-  auto contiguous = [&](const PoolPtrs& src, int64_t s0, const PoolPtrs& dst, int64_t d0, int64_t len) -> int {
-    for(int d = 0; d < 6; ++d)
-      RK_HIP(hipMemcpyAsync(dst.f[d] + d0, src.f[d] + s0, sizeof(float) * size_t(len), hipMemcpyDeviceToDevice, st));
-    RK_HIP(hipMemcpyAsync(dst.id + d0, src.id + s0, sizeof(uint64_t) * size_t(len), hipMemcpyDeviceToDevice, st));
-    return RK_OK;
-  };
-  // pass 1, front to back; the leading bins already packed stay
-  int i0 = 0;
-  while(i0 < nb && so[size_t(i0)] == to[size_t(i0)]) ++i0;
-  while(i0 < nb) {
-    int i1 = i0;
+  // batches of consecutive moves whose particles fit the staging buffer
+  int k0 = 0;
+  while(k0 < nm) {
+    int k1 = k0;
     int64_t T = 0;
-    while(i1 < nb && T + n[size_t(i1)] <= W) T += n[size_t(i1++)];
-    const int ns = i1 - i0;
-    hipLaunchKernelGGL(k_copy_segments, dim3(unsigned(std::min(ns, 8192))), dim3(WG), 0, st, pool, stage, d_so + i0,
-                       int64_t(0), d_to + i0, to[size_t(i0)], d_n + i0, ns);
+    while(k1 < nm && T + mn[size_t(k1)] <= W) T += mn[size_t(k1++)];
+    const int ns = k1 - k0;
+    const unsigned grid = unsigned(std::min(ns, 8192));
+    hipLaunchKernelGGL(k_copy_segments, dim3(grid), dim3(WG), 0, st, pool, stage, d_so + k0, int64_t(0), d_st + k0,
+                       mst[size_t(k0)], d_n + k0, ns);
     RK_HIP(hipGetLastError());
-    if(int rc = contiguous(stage, 0, pool, to[size_t(i0)], T)) return rc;
-    i0 = i1;
-  }
-  // pass 2, back to front
-  int i1 = nb;
-  while(i1 > 0) {
-    int j = i1;
-    int64_t T = 0;
-    while(j > 0 && T + n[size_t(j - 1)] <= W) T += n[size_t(--j)];
-    const int ns = i1 - j;
-    if(int rc = contiguous(pool, to[size_t(j)], stage, 0, T)) return rc;
-    hipLaunchKernelGGL(k_copy_segments, dim3(unsigned(std::min(ns, 8192))), dim3(WG), 0, st, stage, pool, d_to + j,
-                       to[size_t(j)], d_fo + j, int64_t(0), d_n + j, ns);
+    hipLaunchKernelGGL(k_copy_segments, dim3(grid), dim3(WG), 0, st, stage, pool, d_st + k0, mst[size_t(k0)],
+                       d_fo + k0, int64_t(0), d_n + k0, ns);
     RK_HIP(hipGetLastError());
-    i1 = j;
+    k0 = k1;
   }
   for(int i = 0; i < nb; ++i) {
     const int b = order[size_t(i)];
