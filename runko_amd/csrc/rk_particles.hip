@@ -1150,15 +1150,20 @@ __device__ inline int global_brick(const Geo& g, int b, int d)
 // the slots of each (species, neighbour) pair are ranked with LDS atomics and
 // reserved with one global atomic per pair.
 __global__ void __launch_bounds__(WG)
-  k_append_mailbox(Geo g, SpParams sp, MigPtrs m, int nbins, uint32_t* counters, uint32_t* ovf, int64_t ovf_cap)
+  k_append_mailbox(Geo g, SpParams sp, MigPtrs m, int nbins, uint32_t* counters, uint32_t* ovf, int64_t ovf_cap, int span)
 {
   constexpr int NK = RK_MAX_SPECIES * 27;
   __shared__ uint32_t cnt[NK];
   __shared__ uint32_t kbase[NK];
   __shared__ int kdb[NK];
-  for(int b = int(blockIdx.x); b < nbins; b += int(gridDim.x)) {
+  // one block per brick, with the fused step's XCD-aware brick map: a brick's
+  // neighbours mostly run on the same XCD, so the appends to one destination
+  // bin's tail mostly go through one L2
+  {
+    const int b = xcd_brick(int(blockIdx.x), span);
+    if(b >= nbins) return;  // uniform over the block
     const int n = int(m.mcount[b]);
-    if(n == 0) continue;  // uniform over the block
+    if(n == 0) return;  // uniform over the block
     const uint32_t base = uint32_t(int64_t(b) * m.per_brick);
     int gsrc[3];
 #pragma unroll
@@ -2242,9 +2247,10 @@ int launch_append(rk_world* w, int set, hipStream_t s)
 
 int launch_append_mailbox(rk_world* w, hipStream_t s)
 {
-  const int grid = w->g.nbricks;  // one block per brick (measured faster than a capped grid)
-  hipLaunchKernelGGL(k_append_mailbox, dim3(unsigned(grid)), dim3(WG), 0, s, w->g, make_sp(w), make_mig(w),
-                     w->g.nbricks, w->counters, w->ovf_list, w->ovf_cap);
+  // one block per brick (measured faster than a capped grid)
+  const int span = xcd_span(w->g, w->g.nbricks);
+  hipLaunchKernelGGL(k_append_mailbox, dim3(xcd_grid(w->g.nbricks, span)), dim3(WG), 0, s, w->g, make_sp(w),
+                     make_mig(w), w->g.nbricks, w->counters, w->ovf_list, w->ovf_cap, span);
   RK_HIP(hipGetLastError());
   return RK_OK;
 }
